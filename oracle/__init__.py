@@ -1,0 +1,179 @@
+"""ctypes binding of the CPU oracle (libcd_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP engine is compared against.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libcd_oracle.so")
+_lib = None
+
+i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.or_det_exp.argtypes = [C.c_double]
+        L.or_det_exp.restype = C.c_double
+        L.or_dasdennis.argtypes = [C.c_int, C.c_int, f32p]
+        L.or_sl_cds.argtypes = [u8p, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32)]
+        L.or_eval.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, C.c_float, i8p, C.c_int,
+                              f32p, i8p, i32p]
+        L.or_cai_cpb_ref.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, i8p, C.c_int, C.c_int,
+                                     f32p, f64p]
+        L.or_nds.argtypes = [f32p, C.c_int, C.c_int, i32p, C.POINTER(C.c_int32)]
+        L.or_select.argtypes = [f32p, C.c_int, C.c_int, C.c_uint32, C.c_uint64, f32p, i32p, i32p, f32p, i32p]
+        L.or_job_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, i8p, C.c_char_p,
+                                    f32p, f32p, C.c_float]
+        L.or_job_create.restype = C.c_void_p
+        L.or_job_init.argtypes = [C.c_void_p]
+        L.or_job_step.argtypes = [C.c_void_p, C.c_int]
+        L.or_job_read.argtypes = [C.c_void_p, i8p, f32p, i8p, i32p, i32p, i32p, f32p]
+        L.or_job_generation.argtypes = [C.c_void_p]
+        L.or_job_destroy.argtypes = [C.c_void_p]
+        L.or_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def set_threads(n):
+    return lib().or_set_threads(int(n))
+
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def det_exp(x):
+    return lib().or_det_exp(float(x))
+
+
+def dasdennis(M, H):
+    from math import comb
+    out = np.zeros(comb(M + H - 1, H) * M, np.float32)
+    n = lib().or_dasdennis(M, H, out)
+    return out.reshape(n, M)
+
+
+NT = {"A": 0, "C": 1, "G": 2, "U": 3}
+
+
+def codes(seq):
+    if isinstance(seq, str):
+        return np.array([NT[c] for c in seq], np.uint8)
+    return np.ascontiguousarray(seq, np.uint8)
+
+
+def sl(seq, mode=0):
+    c = codes(seq)
+    l, p, q = C.c_int32(), C.c_int32(), C.c_int32()
+    rc = lib().or_sl_cds(c, len(c), mode, C.byref(l), C.byref(p), C.byref(q))
+    assert rc == 0
+    return (l.value, p.value, q.value) if mode == 0 else l.value
+
+
+def evaluate(aa, tables, syn):
+    """syn [count][m][L] int8 -> obj [count][6], idx [count][12], pql [count][3]."""
+    syn = np.ascontiguousarray(syn, np.int8)
+    count, m, L = syn.shape
+    obj = np.zeros((count, 6), np.float32)
+    idx = np.zeros((count, 12), np.int8)
+    pql = np.zeros((count, 3), np.int32)
+    rc = lib().or_eval(L, m, np.ascontiguousarray(aa, np.int8), tables.codon_bytes(), tables.weights,
+                       tables.cps, tables.gc3_ref, syn, count, obj, idx, pql)
+    assert rc == 0
+    return obj, idx, pql
+
+
+def cai_cpb_reference(aa, tables, syn, threads=256):
+    syn = np.ascontiguousarray(syn, np.int8)
+    count, m, L = syn.shape
+    o32 = np.zeros((count, m, 2), np.float32)
+    o64 = np.zeros((count, m, 2), np.float64)
+    rc = lib().or_cai_cpb_ref(L, m, np.ascontiguousarray(aa, np.int8), tables.codon_bytes(), tables.weights,
+                              tables.cps, syn, count, threads, o32, o64)
+    assert rc == 0
+    return o32, o64
+
+
+def nds(obj, N):
+    obj = np.ascontiguousarray(obj, np.float32)
+    P = obj.shape[0]
+    fo = np.zeros(P, np.int32)
+    nf = C.c_int32()
+    lib().or_nds(obj, P, N, fo, C.byref(nf))
+    return fo, nf.value
+
+
+def select(obj, N, H, gen, seed, extremes):
+    obj = np.ascontiguousarray(obj, np.float32)
+    ext = np.ascontiguousarray(extremes, np.float32).copy()
+    sorted_out = np.zeros(N, np.int32)
+    fo = np.zeros(2 * N, np.int32)
+    norm = np.zeros(18, np.float32)
+    flags = np.zeros(2, np.int32)
+    lib().or_select(obj, N, H, gen, seed, ext, sorted_out, fo, norm, flags)
+    return dict(sorted=sorted_out, front_of=fo, extremes=ext.reshape(6, 6), ideal=norm[:6], worst=norm[6:12],
+                nadir=norm[12:], n_cut=int(flags[0]), hyp_exception=int(flags[1]))
+
+
+class Job:
+    """Whole-loop oracle job (main.cu:467-787)."""
+
+    def __init__(self, aa, tables, m, N, H, Pm, seed):
+        self.aa = np.ascontiguousarray(aa, np.int8)
+        self.L, self.m, self.N = len(aa), m, N
+        self.h = lib().or_job_create(self.L, m, N, H, Pm, seed, self.aa, tables.codon_bytes(), tables.weights,
+                                     tables.cps, tables.gc3_ref)
+        if not self.h:
+            raise ValueError("invalid job")
+
+    def init(self):
+        lib().or_job_init(self.h)
+
+    def step(self, gens=1):
+        lib().or_job_step(self.h, gens)
+
+    @property
+    def generation(self):
+        return lib().or_job_generation(self.h)
+
+    def read(self):
+        P = 2 * self.N
+        g = np.zeros((P, self.m, self.L), np.int8)
+        obj = np.zeros((P, 6), np.float32)
+        idx = np.zeros((P, 12), np.int8)
+        pql = np.zeros((P, 3), np.int32)
+        srt = np.zeros(self.N, np.int32)
+        fo = np.zeros(P, np.int32)
+        st = np.zeros(54, np.float32)
+        lib().or_job_read(self.h, g, obj, idx, pql, srt, fo, st)
+        return dict(genome=g, obj=obj, idx=idx, pql=pql, sorted=srt, front_of=fo, extremes=st[:36].reshape(6, 6),
+                    ideal=st[36:42], worst=st[42:48], nadir=st[48:54])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_job_destroy(self.h)
+            self.h = None
