@@ -1,0 +1,736 @@
+// cd_api.hip -- C ABI of the codon-design engine (include/codon_nsga3.h):
+// job lifecycle, table preparation, work lists and the per-generation
+// hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cd_kernels.h"
+#include "codon_nsga3.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(CD_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+const int8_t SYN_START[21] = {0, 4, 6, 8, 10, 12, 16, 18, 21, 23, 29, 30, 32, 36, 38, 44, 50, 54, 58, 59, 61};
+const int8_t SYN_NUM[21] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4, 1, 2, 3};
+const char AMINO[] = "ACDEFGHIKLMNPQRSTVWYZ";
+// Standard genetic code, codons enumerated U C A G per position ('Z' = stop).
+const char CODE[] = "FFLLSSSSYYZZCCZWLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
+
+int nt_code(char c) {
+    switch (c) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'U': return 3;
+    default: return -1;
+    }
+}
+
+char translate(const uint8_t *c3) {  // codes A0 C1 G2 U3 -> index in U C A G order
+    static const int ucag[4] = {2, 1, 3, 0};
+    return CODE[ucag[c3[0]] * 16 + ucag[c3[1]] * 4 + ucag[c3[2]]];
+}
+
+struct HostTables {
+    uint8_t nt[192];
+    int64_t lw[64];
+    int64_t cpfx[61 * 61];
+    float cps[61 * 61];
+    uint8_t hs[64 * 64];
+    uint8_t nd[64 * 64];
+    uint8_t gc3[64];
+};
+
+int build_tables(const cd_tables &t, HostTables &h) {
+    if (!t.codons || !t.weights || !t.cps) return fail(CD_E_INVALID, "null table pointer");
+    for (int i = 0; i < 192; ++i) {
+        const int c = nt_code(t.codons[i]);
+        if (c < 0) return fail(CD_E_INVALID, "codon table letters must be A, C, G or U");
+        h.nt[i] = (uint8_t)c;
+    }
+    for (int a = 0; a < 21; ++a)
+        for (int g = SYN_START[a]; g < SYN_START[a] + SYN_NUM[a]; ++g)
+            if (translate(&h.nt[3 * g]) != AMINO[a])
+                return fail(CD_E_INVALID, "codon table block does not encode its amino acid");
+    for (int g = 0; g < 64; ++g) {
+        const float w = t.weights[g];
+        if (!(w > 0.f) || w > 1.f) return fail(CD_E_INVALID, "CAI weights must lie in (0, 1]");
+        h.lw[g] = llrint(std::log((double)w) * 0x1p40);
+    }
+    for (int i = 0; i < 61 * 61; ++i) {
+        const float v = t.cps[i];
+        if (!(std::fabs(v) <= 1000.f)) return fail(CD_E_INVALID, "codon-pair scores must be finite and |v| <= 1000");
+        h.cps[i] = v;
+        h.cpfx[i] = llrint((double)v * 0x1p40);
+    }
+    auto is_stop = [&](int a, int b, int c) {  // the table's last three codons (utils.cuh:101-112)
+        for (int s = 61; s < 64; ++s)
+            if (h.nt[3 * s] == a && h.nt[3 * s + 1] == b && h.nt[3 * s + 2] == c) return true;
+        return false;
+    };
+    for (int a = 0; a < 64; ++a) {
+        for (int b = 0; b < 64; ++b) {
+            h.hs[a * 64 + b] = (uint8_t)(is_stop(h.nt[3 * a + 1], h.nt[3 * a + 2], h.nt[3 * b]) +
+                                         is_stop(h.nt[3 * a + 2], h.nt[3 * b], h.nt[3 * b + 1]));
+            int d = 0;
+            for (int k = 0; k < 3; ++k) d += h.nt[3 * a + k] != h.nt[3 * b + k];
+            h.nd[a * 64 + b] = (uint8_t)d;
+        }
+        h.gc3[a] = (h.nt[3 * a + 2] == 1 || h.nt[3 * a + 2] == 2);
+    }
+    return CD_OK;
+}
+
+// Das-Dennis simplex lattice in the order of combinationRepetition
+// (sorting.cuh:1285-1354): multisets of H objectives in lexicographic order,
+// coordinate = H-fold float sum of 1/H over the objective's multiplicity.
+std::vector<float> das_dennis(int H) {
+    std::vector<float> out;
+    std::vector<int> x(H, 0);
+    const float step = 1.f / (float)H;
+    while (true) {
+        float v[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < H; ++i) v[x[i]] += step;
+        out.insert(out.end(), v, v + 6);
+        int i = H - 1;
+        while (i >= 0 && x[i] == 5) --i;
+        if (i < 0) break;
+        const int nv = x[i] + 1;
+        for (int j = i; j < H; ++j) x[j] = nv;
+    }
+    return out;
+}
+
+int64_t n_ref(int H) {
+    int64_t r = 1;
+    for (int i = 1; i <= 5; ++i) r = r * (H + i) / i;
+    return r;
+}
+
+// Bernoulli threshold: a 32-bit draw x succeeds when x < floor(p * 2^32).
+uint64_t bern_threshold_host(float p) {
+    double t = (double)p * 4294967296.0;
+    if (!(t > 0.0)) return 0;
+    if (t >= 18446744073709551615.0) return ~0ull;
+    return (uint64_t)t;
+}
+
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Sizes {
+    size_t total = 0;
+    size_t add(size_t b) {
+        size_t o = total;
+        total += a256(b);
+        return o;
+    }
+};
+
+}  // namespace
+
+struct cd_job {
+    cd::DevJob h{};
+    cd::DevJob *d_self = nullptr;
+    int L = 0, m = 0, N = 0, H = 0, R = 0, P = 0, W = 0;
+    void *mem = nullptr;
+    size_t mem_bytes = 0;
+    hipStream_t stream = nullptr;
+    int2 *d_off = nullptr;
+    int4 *d_dom = nullptr;
+    int2 *d_asc = nullptr;
+    int n_dom = 0, n_asc = 0;
+    size_t lds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool inited = false;
+    bool selection_only = false;
+};
+
+struct cd_batch {
+    std::vector<cd_job *> jobs;
+    cd::DevJob *d_jobs = nullptr;
+    int2 *d_off = nullptr;
+    int4 *d_dom = nullptr;
+    int2 *d_asc = nullptr;
+    int n_off = 0, n_dom = 0, n_asc = 0;
+    size_t lds = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+// Lay out every device buffer of a job in one allocation.
+struct Layout {
+    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], sorted, front, np, dom,
+        ref_of, dist, bucket, niche, flag, st, self, off, domi, asc;
+    size_t total;
+};
+
+Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bool sel_only) {
+    Sizes s;
+    Layout o{};
+    if (!sel_only) {
+        o.aa = s.add(L);
+        o.nt = s.add(192);
+        o.lw = s.add(64 * 8);
+        o.cpfx = s.add(61 * 61 * 8);
+        o.cps = s.add(61 * 61 * 4);
+        o.hs = s.add(4096);
+        o.nd = s.add(4096);
+        o.gc3 = s.add(64);
+        for (int b = 0; b < 2; ++b) {
+            o.genome[b] = s.add((size_t)P * m * L);
+            o.oidx[b] = s.add((size_t)P * 12);
+            o.pql[b] = s.add((size_t)P * 16);
+            o.cst[b] = s.add((size_t)P * m * sizeof(cd::CdsStat));
+        }
+        o.off = s.add((size_t)N * sizeof(int2));
+    }
+    for (int b = 0; b < (sel_only ? 1 : 2); ++b) o.obj[b] = s.add((size_t)P * 6 * 4);
+    if (sel_only) o.obj[1] = o.obj[0];
+    o.ref = s.add((size_t)R * 6 * 4);
+    o.sorted = s.add((size_t)N * 4);
+    o.front = s.add((size_t)P * 4);
+    o.np = s.add((size_t)P * 4);
+    o.dom = s.add((size_t)P * W * 8);
+    o.ref_of = s.add((size_t)P * 4);
+    o.dist = s.add((size_t)P * 4);
+    o.bucket = s.add((size_t)P * 4);
+    o.niche = s.add((size_t)(5 * R + P) * 4);
+    o.flag = s.add((size_t)P);
+    o.st = s.add(sizeof(cd::SelState));
+    o.self = s.add(sizeof(cd::DevJob));
+    o.domi = s.add((size_t)n_dom * sizeof(int4));
+    o.asc = s.add((size_t)n_asc * sizeof(int2));
+    o.total = s.total;
+    return o;
+}
+
+void work_counts(int P, int W, int *n_dom, int *n_asc) {
+    *n_dom = ((P + 255) / 256) * W;
+    *n_asc = (P + 255) / 256;
+}
+
+int validate(const cd_job_desc *d) {
+    if (!d) return fail(CD_E_INVALID, "null descriptor");
+    // main.cu:485-489
+    if (d->N <= 0) return fail(CD_E_INVALID, "population size N must be positive");
+    if (d->m < 2 || d->m > cd::MAX_M) return fail(CD_E_INVALID, "CDS count m must be in [2, 16]");
+    if (!(d->Pm >= 0.f && d->Pm <= 1.f)) return fail(CD_E_INVALID, "mutation probability must be in [0, 1]");
+    if (d->L < 3 || d->L > 100000) return fail(CD_E_INVALID, "L (codons incl. stop) must be in [3, 100000]");
+    if (d->H < 1 || n_ref(d->H) > (1 << 20)) return fail(CD_E_INVALID, "H must be >= 1 with at most 2^20 reference points");
+    if (2LL * d->N > 64LL * cd::MAX_W) return fail(CD_E_INVALID, "pool 2N exceeds 16384");
+    if (!d->aa) return fail(CD_E_INVALID, "null amino-acid sequence");
+    for (int k = 0; k < d->L; ++k) {  // findAminoIndex, utils.cuh:60-86
+        if (d->aa[k] < 0 || d->aa[k] > 20) return fail(CD_E_INVALID, "amino-acid index out of range");
+        if ((d->aa[k] == 20) != (k == d->L - 1)) return fail(CD_E_INVALID, "the stop (20) must be exactly the last codon");
+    }
+    if (3LL * d->L >= (1 << 20)) return fail(CD_E_INVALID, "sequence too long for stem positions");
+    return CD_OK;
+}
+
+int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out) {
+    cd_job *J = new cd_job();
+    J->selection_only = sel_only;
+    J->L = sel_only ? 0 : d->L;
+    J->m = sel_only ? 0 : d->m;
+    J->N = d->N;
+    J->H = d->H;
+    J->R = (int)n_ref(d->H);
+    J->P = P_override > 0 ? P_override : 2 * d->N;
+    J->W = (J->P + 63) / 64;
+    work_counts(J->P, J->W, &J->n_dom, &J->n_asc);
+    Layout o = layout(J->L, J->m, J->N, J->P, J->R, J->W, J->n_dom, J->n_asc, sel_only);
+    hipError_t e = hipMalloc(&J->mem, o.total);
+    if (e != hipSuccess) {
+        delete J;
+        return fail(CD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    J->mem_bytes = o.total;
+    char *base = (char *)J->mem;
+    e = hipMemset(J->mem, 0, o.total);
+    std::vector<char> host(o.total, 0);  // staged image of the tables and descriptors
+    cd::DevJob &h = J->h;
+    h.L = J->L;
+    h.m = J->m;
+    h.N = J->N;
+    h.P = J->P;
+    h.R = J->R;
+    h.W = J->W;
+    h.den_sl = sel_only ? 1 : (3 * J->L - 4) / 2;
+    h.seed = d->seed;
+    h.thr_pm = bern_threshold_host(d->Pm);
+    h.thr_2pm = bern_threshold_host(d->Pm * 2.0f);
+    h.gc3_ref = d->tables.gc3_ref;
+    if (!sel_only) {
+        HostTables t;
+        int rc = build_tables(d->tables, t);
+        if (rc != CD_OK) {
+            (void)hipFree(J->mem);
+            delete J;
+            return rc;
+        }
+        std::memcpy(&host[o.aa], d->aa, J->L);
+        std::memcpy(&host[o.nt], t.nt, sizeof(t.nt));
+        std::memcpy(&host[o.lw], t.lw, sizeof(t.lw));
+        std::memcpy(&host[o.cpfx], t.cpfx, sizeof(t.cpfx));
+        std::memcpy(&host[o.cps], t.cps, sizeof(t.cps));
+        std::memcpy(&host[o.hs], t.hs, sizeof(t.hs));
+        std::memcpy(&host[o.nd], t.nd, sizeof(t.nd));
+        std::memcpy(&host[o.gc3], t.gc3, sizeof(t.gc3));
+        h.aa = (const int8_t *)(base + o.aa);
+        h.nt = (const uint8_t *)(base + o.nt);
+        h.lw = (const int64_t *)(base + o.lw);
+        h.cpfx = (const int64_t *)(base + o.cpfx);
+        h.cps = (const float *)(base + o.cps);
+        h.hs = (const uint8_t *)(base + o.hs);
+        h.nd = (const uint8_t *)(base + o.nd);
+        h.gc3 = (const uint8_t *)(base + o.gc3);
+        for (int b = 0; b < 2; ++b) {
+            h.genome[b] = (int8_t *)(base + o.genome[b]);
+            h.oidx[b] = (int8_t *)(base + o.oidx[b]);
+            h.pql[b] = (int32_t *)(base + o.pql[b]);
+            h.cst[b] = (cd::CdsStat *)(base + o.cst[b]);
+        }
+        int2 *off = (int2 *)&host[o.off];
+        for (int c = 0; c < J->N; ++c) off[c] = make_int2(0, c);
+        J->d_off = (int2 *)(base + o.off);
+        J->lds = cd::design_lds_bytes(J->L, J->m);
+    }
+    auto ref = das_dennis(d->H);
+    std::memcpy(&host[o.ref], ref.data(), ref.size() * sizeof(float));
+    h.refpts = (const float *)(base + o.ref);
+    h.obj[0] = (float *)(base + o.obj[0]);
+    h.obj[1] = (float *)(base + o.obj[1]);
+    h.sorted = (int32_t *)(base + o.sorted);
+    h.front_of = (int32_t *)(base + o.front);
+    h.np = (int32_t *)(base + o.np);
+    h.dom = (uint64_t *)(base + o.dom);
+    h.ref_of = (int32_t *)(base + o.ref_of);
+    h.dist = (float *)(base + o.dist);
+    h.bucket = (int32_t *)(base + o.bucket);
+    h.niche = (int32_t *)(base + o.niche);
+    h.flag = (uint8_t *)(base + o.flag);
+    h.st = (cd::SelState *)(base + o.st);
+    std::memcpy(&host[o.self], &h, sizeof(h));
+    J->d_self = (cd::DevJob *)(base + o.self);
+    int4 *dom = (int4 *)&host[o.domi];
+    int q = 0;
+    for (int t = 0; t < (J->P + 255) / 256; ++t)
+        for (int w = 0; w < J->W; ++w) dom[q++] = make_int4(0, t * 256, w, 0);
+    int2 *asc = (int2 *)&host[o.asc];
+    for (int t = 0; t < J->n_asc; ++t) asc[t] = make_int2(0, t * 256);
+    J->d_dom = (int4 *)(base + o.domi);
+    J->d_asc = (int2 *)(base + o.asc);
+    if (e == hipSuccess) e = hipMemcpy(J->mem, host.data(), o.total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreate(&J->stream);
+    if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
+    if (e != hipSuccess) {
+        (void)hipFree(J->mem);
+        delete J;
+        return fail(CD_E_DEVICE, std::string("job setup: ") + hipGetErrorString(e));
+    }
+    *out = J;
+    return CD_OK;
+}
+
+hipError_t enqueue_generation(const cd::DevJob *jobs, int njobs, const int2 *off, int n_off, const int4 *dom, int n_dom,
+                              const int2 *asc, int n_asc, size_t lds, hipStream_t s) {
+    hipError_t e = cd::launch_design(cd::MODE_OFFSPRING, jobs, off, n_off, lds, s);
+    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, s);
+    if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
+    if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
+    if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, s);
+    return e;
+}
+
+int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const cd::DevJob *jobs, int njobs, const int2 *off,
+            int n_off, const int4 *dom, int n_dom, const int2 *asc, int n_asc, size_t lds) {
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_generation(jobs, njobs, off, n_off, dom, n_dom, asc, n_asc, lds, s);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(CD_E_DEVICE, std::string("end capture: ") + hipGetErrorString(e2));
+    HIPCHK(hipGraphInstantiate(ge, graph, nullptr, nullptr, 0));
+    *g = graph;
+    return CD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cd_last_error(void) { return g_err.c_str(); }
+
+int64_t cd_job_device_bytes(int32_t L, int32_t m, int32_t N, int32_t H) {
+    if (L < 3 || m < 2 || N < 1 || H < 1) return -1;
+    const int P = 2 * N, W = (P + 63) / 64;
+    int n_dom, n_asc;
+    work_counts(P, W, &n_dom, &n_asc);
+    return (int64_t)layout(L, m, N, P, (int)n_ref(H), W, n_dom, n_asc, false).total;
+}
+
+int cd_job_create(const cd_job_desc *desc, cd_job **out) {
+    if (!out) return fail(CD_E_INVALID, "null output handle");
+    int rc = validate(desc);
+    if (rc != CD_OK) return rc;
+    return create_job(desc, false, 0, out);
+}
+
+int cd_job_init(cd_job *J) {
+    if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
+    // the offspring work list doubles as the (job, design) list of the init launch
+    HIPCHK(cd::launch_design(cd::MODE_INIT, J->d_self, J->d_off, J->N, J->lds, J->stream));
+    HIPCHK(cd::launch_init_state(J->d_self, 1, J->stream));
+    J->inited = true;
+    return CD_OK;
+}
+
+int cd_job_step(cd_job *J, int32_t generations) {
+    if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
+    if (!J->inited) return fail(CD_E_STATE, "cd_job_step before cd_job_init");
+    if (generations < 0) return fail(CD_E_INVALID, "negative generation count");
+    if (!J->gexec) {
+        int rc = capture(J->stream, &J->graph, &J->gexec, J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc,
+                         J->n_asc, J->lds);
+        if (rc != CD_OK) return rc;
+    }
+    for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(J->gexec, J->stream));
+    return CD_OK;
+}
+
+int cd_job_step_timed(cd_job *J, int32_t generations, float *ms) {
+    if (!J) return fail(CD_E_INVALID, "null job");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, J->stream));
+    int rc = cd_job_step(J, generations);
+    if (rc != CD_OK) return rc;
+    HIPCHK(hipEventRecord(b, J->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    if (ms) *ms = t;
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    return CD_OK;
+}
+
+int cd_job_sync(cd_job *J) {
+    if (!J) return fail(CD_E_INVALID, "null job");
+    HIPCHK(hipStreamSynchronize(J->stream));
+    return CD_OK;
+}
+
+int cd_job_generation(cd_job *J, int32_t *generation) {
+    if (!J || !generation) return fail(CD_E_INVALID, "null argument");
+    HIPCHK(hipStreamSynchronize(J->stream));
+    cd::SelState st;
+    HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
+    *generation = (int32_t)st.gen;
+    return CD_OK;
+}
+
+int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql, int32_t *sorted, int32_t *front_of,
+                float *state) {
+    if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
+    if (!J->inited) return fail(CD_E_STATE, "cd_job_read before cd_job_init");
+    HIPCHK(hipStreamSynchronize(J->stream));
+    cd::SelState st;
+    HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
+    const int b = (int)(st.gen & 1u), P = J->P;
+    if (genome) HIPCHK(hipMemcpy(genome, J->h.genome[b], (size_t)P * J->m * J->L, hipMemcpyDeviceToHost));
+    if (obj) HIPCHK(hipMemcpy(obj, J->h.obj[b], (size_t)P * 6 * 4, hipMemcpyDeviceToHost));
+    if (idx) HIPCHK(hipMemcpy(idx, J->h.oidx[b], (size_t)P * 12, hipMemcpyDeviceToHost));
+    if (pql) {
+        std::vector<int32_t> t((size_t)P * 4);
+        HIPCHK(hipMemcpy(t.data(), J->h.pql[b], t.size() * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < P; ++i)
+            for (int k = 0; k < 3; ++k) pql[i * 3 + k] = t[(size_t)i * 4 + k];
+    }
+    if (sorted) HIPCHK(hipMemcpy(sorted, J->h.sorted, (size_t)J->N * 4, hipMemcpyDeviceToHost));
+    if (front_of) HIPCHK(hipMemcpy(front_of, J->h.front_of, (size_t)P * 4, hipMemcpyDeviceToHost));
+    if (state) {
+        std::memcpy(state, st.ext, sizeof(st.ext));
+        std::memcpy(state + 36, st.ideal, sizeof(st.ideal));
+        std::memcpy(state + 42, st.worst, sizeof(st.worst));
+        std::memcpy(state + 48, st.nadir, sizeof(st.nadir));
+    }
+    return CD_OK;
+}
+
+int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
+    if (!J || J->selection_only || !kernel_ms) return fail(CD_E_INVALID, "null argument");
+    if (!J->inited) return fail(CD_E_STATE, "cd_job_profile before cd_job_init");
+    hipEvent_t ev[6];
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int g = 0; g < generations; ++g) {
+        HIPCHK(hipEventRecord(ev[0], J->stream));
+        HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream));
+        HIPCHK(hipEventRecord(ev[1], J->stream));
+        HIPCHK(cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, J->stream));
+        HIPCHK(hipEventRecord(ev[2], J->stream));
+        HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
+        HIPCHK(hipEventRecord(ev[3], J->stream));
+        HIPCHK(cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream));
+        HIPCHK(hipEventRecord(ev[4], J->stream));
+        HIPCHK(cd::launch_niche(J->d_self, 1, J->stream));
+        HIPCHK(hipEventRecord(ev[5], J->stream));
+        HIPCHK(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; ++k) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            acc[k] += t;
+        }
+    }
+    for (int k = 0; k < 5; ++k) kernel_ms[k] = generations > 0 ? (float)(acc[k] / generations) : 0.f;
+    for (auto &e : ev) HIPCHK(hipEventDestroy(e));
+    return CD_OK;
+}
+
+int cd_job_cds_scanned(cd_job *J, int64_t *count) {
+    if (!J || !count) return fail(CD_E_INVALID, "null argument");
+    HIPCHK(hipStreamSynchronize(J->stream));
+    cd::SelState st;
+    HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
+    *count = (int64_t)st.cds_scanned;
+    return CD_OK;
+}
+
+int cd_job_destroy(cd_job *J) {
+    if (!J) return CD_OK;
+    if (J->stream) (void)hipStreamSynchronize(J->stream);
+    if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
+    if (J->graph) (void)hipGraphDestroy(J->graph);
+    if (J->stream) (void)hipStreamDestroy(J->stream);
+    if (J->mem) (void)hipFree(J->mem);
+    delete J;
+    return CD_OK;
+}
+
+int cd_eval_batch(const cd_job_desc *desc, const int8_t *genome, int32_t count, float *obj, int8_t *idx,
+                  int32_t *pql) {
+    if (!desc || !genome || count <= 0) return fail(CD_E_INVALID, "bad eval arguments");
+    cd_job_desc d = *desc;
+    d.N = (count + 1) / 2;
+    d.H = 1;
+    d.Pm = 0.f;
+    int rc = validate(&d);
+    if (rc != CD_OK) return rc;
+    cd_job *J = nullptr;
+    rc = create_job(&d, false, 0, &J);
+    if (rc != CD_OK) return rc;
+    const size_t G = (size_t)J->m * J->L;
+    std::vector<int2> items(count);
+    for (int i = 0; i < count; ++i) items[i] = make_int2(0, i);
+    int2 *d_items = nullptr;
+    hipError_t e = hipMalloc(&d_items, sizeof(int2) * count);
+    if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), sizeof(int2) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(J->h.genome[0], genome, G * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = cd::launch_design(cd::MODE_EVAL, J->d_self, d_items, count, J->lds, J->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+    if (e == hipSuccess && obj) e = hipMemcpy(obj, J->h.obj[0], (size_t)count * 24, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && idx) e = hipMemcpy(idx, J->h.oidx[0], (size_t)count * 12, hipMemcpyDeviceToHost);
+    std::vector<int32_t> t((size_t)count * 4);
+    if (e == hipSuccess && pql) e = hipMemcpy(t.data(), J->h.pql[0], t.size() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && pql)
+        for (int i = 0; i < count; ++i)
+            for (int k = 0; k < 3; ++k) pql[i * 3 + k] = t[(size_t)i * 4 + k];
+    if (d_items) (void)hipFree(d_items);
+    cd_job_destroy(J);
+    if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("eval: ") + hipGetErrorString(e));
+    return CD_OK;
+}
+
+static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint32_t gen, uint64_t seed,
+                         const float *extremes, bool full, cd_job **out) {
+    if (!obj || P <= 0 || N <= 0 || N > P || P > 64 * cd::MAX_W || H < 1)
+        return fail(CD_E_INVALID, "bad selection arguments");
+    cd_job_desc d{};
+    d.N = N;
+    d.H = H;
+    d.seed = seed;
+    cd_job *J = nullptr;
+    int rc = create_job(&d, true, P, &J);
+    if (rc != CD_OK) return rc;
+    cd::SelState st{};
+    st.gen = gen;
+    if (extremes) std::memcpy(st.ext, extremes, sizeof(st.ext));
+    hipError_t e = hipMemcpy(J->h.obj[0], obj, (size_t)P * 24, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(J->h.st, &st, sizeof(st), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
+    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, J->stream);
+    if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->stream);
+    if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream);
+    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+    if (e != hipSuccess) {
+        cd_job_destroy(J);
+        return fail(CD_E_DEVICE, std::string("selection: ") + hipGetErrorString(e));
+    }
+    *out = J;
+    return CD_OK;
+}
+
+int cd_nds(const float *obj, int32_t P, int32_t N, int32_t *front_of, int32_t *n_fronts) {
+    cd_job *J = nullptr;
+    int rc = run_selection(obj, P, N, 1, 0, 0, nullptr, false, &J);
+    if (rc != CD_OK) return rc;
+    cd::SelState st;
+    hipError_t e = hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && front_of) e = hipMemcpy(front_of, J->h.front_of, (size_t)P * 4, hipMemcpyDeviceToHost);
+    if (n_fronts) *n_fronts = st.last + 1;
+    cd_job_destroy(J);
+    if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("nds: ") + hipGetErrorString(e));
+    return CD_OK;
+}
+
+int cd_select(const float *obj, int32_t N, int32_t H, uint32_t gen, uint64_t seed, float *extremes, int32_t *sorted,
+              int32_t *front_of, float *norm, int32_t *flags) {
+    cd_job *J = nullptr;
+    int rc = run_selection(obj, 2 * N, N, H, gen, seed, extremes, true, &J);
+    if (rc != CD_OK) return rc;
+    cd::SelState st;
+    hipError_t e = hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sorted) e = hipMemcpy(sorted, J->h.sorted, (size_t)N * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && front_of) e = hipMemcpy(front_of, J->h.front_of, (size_t)2 * N * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        if (extremes) std::memcpy(extremes, st.ext, sizeof(st.ext));
+        if (norm) {
+            std::memcpy(norm, st.ideal, 24);
+            std::memcpy(norm + 6, st.worst, 24);
+            std::memcpy(norm + 12, st.nadir, 24);
+        }
+        if (flags) {
+            flags[0] = st.n_cut;
+            flags[1] = st.hyp;
+        }
+    }
+    cd_job_destroy(J);
+    if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("select: ") + hipGetErrorString(e));
+    return CD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form
+// ---------------------------------------------------------------------------
+int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
+    if (!jobs || count <= 0 || !out) return fail(CD_E_INVALID, "bad batch arguments");
+    cd_batch *B = new cd_batch();
+    std::vector<cd::DevJob> descs;
+    std::vector<int2> off, asc;
+    std::vector<int4> dom;
+    for (int j = 0; j < count; ++j) {
+        cd_job *J = jobs[j];
+        if (!J || J->selection_only || !J->inited) {
+            delete B;
+            return fail(CD_E_STATE, "batch jobs must be created and initialised");
+        }
+        for (int k = 0; k < j; ++k)
+            if (jobs[k] == J) {
+                delete B;
+                return fail(CD_E_INVALID, "a job appears twice in the batch");
+            }
+        (void)hipStreamSynchronize(J->stream);
+        B->jobs.push_back(J);
+        descs.push_back(J->h);
+        for (int c = 0; c < J->N; ++c) off.push_back(make_int2(j, c));
+        for (int t = 0; t < (J->P + 255) / 256; ++t) {
+            asc.push_back(make_int2(j, t * 256));
+            for (int w = 0; w < J->W; ++w) dom.push_back(make_int4(j, t * 256, w, 0));
+        }
+        B->lds = std::max(B->lds, J->lds);
+    }
+    B->n_off = (int)off.size();
+    B->n_dom = (int)dom.size();
+    B->n_asc = (int)asc.size();
+    hipError_t e = hipMalloc(&B->d_jobs, sizeof(cd::DevJob) * count);
+    if (e == hipSuccess) e = hipMalloc(&B->d_off, sizeof(int2) * off.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * dom.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * asc.size());
+    if (e == hipSuccess) e = hipMemcpy(B->d_jobs, descs.data(), sizeof(cd::DevJob) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_off, off.data(), sizeof(int2) * off.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_dom, dom.data(), sizeof(int4) * dom.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_asc, asc.data(), sizeof(int2) * asc.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamCreate(&B->stream);
+    if (e == hipSuccess && B->lds > 48 * 1024) e = cd::set_design_lds_limit(B->lds);
+    if (e != hipSuccess) {
+        cd_batch_destroy(B);
+        return fail(CD_E_DEVICE, std::string("batch setup: ") + hipGetErrorString(e));
+    }
+    *out = B;
+    return CD_OK;
+}
+
+int cd_batch_step(cd_batch *B, int32_t generations) {
+    if (!B) return fail(CD_E_INVALID, "null batch");
+    if (!B->gexec) {
+        int rc = capture(B->stream, &B->graph, &B->gexec, B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom,
+                         B->n_dom, B->d_asc, B->n_asc, B->lds);
+        if (rc != CD_OK) return rc;
+    }
+    for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(B->gexec, B->stream));
+    return CD_OK;
+}
+
+int cd_batch_sync(cd_batch *B) {
+    if (!B) return fail(CD_E_INVALID, "null batch");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    return CD_OK;
+}
+
+int cd_batch_step_timed(cd_batch *B, int32_t generations, float *ms) {
+    if (!B) return fail(CD_E_INVALID, "null batch");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, B->stream));
+    int rc = cd_batch_step(B, generations);
+    if (rc != CD_OK) return rc;
+    HIPCHK(hipEventRecord(b, B->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    if (ms) *ms = t;
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    return CD_OK;
+}
+
+int cd_batch_destroy(cd_batch *B) {
+    if (!B) return CD_OK;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    if (B->d_jobs) (void)hipFree(B->d_jobs);
+    if (B->d_off) (void)hipFree(B->d_off);
+    if (B->d_dom) (void)hipFree(B->d_dom);
+    if (B->d_asc) (void)hipFree(B->d_asc);
+    delete B;
+    return CD_OK;
+}
+
+}  // extern "C"

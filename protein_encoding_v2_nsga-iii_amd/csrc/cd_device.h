@@ -1,0 +1,170 @@
+// cd_device.h -- device-side primitives of the codon-design engine (gfx950,
+// wave64).  Included by the kernel translation unit only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cd_layout.h"
+
+namespace cd {
+
+__constant__ int8_t c_syn_start[21] = {0, 4, 6, 8, 10, 12, 16, 18, 21, 23, 29, 30, 32, 36, 38, 44, 50, 54, 58, 59, 61};
+__constant__ int8_t c_syn_num[21] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4, 1, 2, 3};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint64_t seed,
+                                       uint32_t out[4]) {
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+__device__ __forceinline__ uint32_t uniform_int(uint32_t x, uint32_t k) {
+    return (uint32_t)(((uint64_t)x * k) >> 32);
+}
+__device__ __forceinline__ bool bern(uint32_t x, uint64_t thr) { return (uint64_t)x < thr; }
+__host__ __device__ inline uint64_t bern_threshold(float p) {
+    double t = (double)p * 4294967296.0;
+    if (!(t > 0.0)) return 0;
+    if (t >= 18446744073709551615.0) return ~0ull;
+    return (uint64_t)t;
+}
+
+// exp(x): x = k ln2 + r, Taylor series of exp(r) to degree 14 in Horner form
+// with fused multiply-adds.  Only IEEE double operations, so host and device
+// produce the same bits.
+struct ExpTable {
+    double c[15];
+};
+constexpr ExpTable make_exp_table() {
+    ExpTable t{};
+    double f = 1.0;
+    for (int j = 0; j < 15; ++j) {
+        if (j > 0) f *= (double)j;
+        t.c[j] = 1.0 / f;
+    }
+    return t;
+}
+__constant__ ExpTable c_exp = make_exp_table();
+
+__device__ inline double det_exp(double x) {
+    const double INV_LN2 = 0x1.71547652b82fep0;
+    const double LN2_HI = 0x1.62e42fefa39efp-1;
+    const double LN2_LO = 0x1.abc9e3b39803fp-56;
+    double k = __builtin_rint(x * INV_LN2);
+    double r = __builtin_fma(-k, LN2_HI, x);
+    r = __builtin_fma(-k, LN2_LO, r);
+    double p = c_exp.c[14];
+#pragma unroll
+    for (int j = 13; j >= 0; --j) p = __builtin_fma(p, r, c_exp.c[j]);
+    return __builtin_ldexp(p, (int)k);
+}
+
+// Stem key: larger is better (longer stem, then smaller p, then smaller q).
+__device__ __forceinline__ uint64_t stem_key(int l, int p, int q) {
+    return ((uint64_t)l << 40) | ((uint64_t)(0xFFFFF - p) << 20) | (uint64_t)(0xFFFFF - q);
+}
+__device__ __forceinline__ void stem_unkey(uint64_t k, int &l, int &p, int &q) {
+    l = (int)(k >> 40);
+    if (l == 0) {
+        p = q = 0;
+    } else {
+        p = 0xFFFFF - (int)((k >> 20) & 0xFFFFF);
+        q = 0xFFFFF - (int)(k & 0xFFFFF);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 / workgroup reductions
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Sum of up to 4 int64 values over the workgroup; every thread gets the
+// totals.  red must hold 4*(blockDim/64) entries.
+template <int NV>
+__device__ inline void block_sum_i64(int64_t (&v)[NV], int64_t *red) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[i * 16 + wid] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        int64_t s = 0;
+        for (int w = 0; w < nw; ++w) s += red[i * 16 + w];
+        v[i] = s;
+    }
+    __syncthreads();
+}
+
+__device__ inline uint64_t block_max_u64(uint64_t v, uint64_t *red) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    v = wave_max_u64(v);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    uint64_t m = 0;
+    for (int w = 0; w < nw; ++w) m = red[w] > m ? red[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// Exclusive prefix sum of one int per thread over the workgroup; returns the
+// prefix and sets *total.  red holds blockDim/64 ints.
+__device__ inline int block_scan_excl(int v, int *red, int *total) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    __syncthreads();
+    if (lane == 63) red[wid] = incl;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        if (w < wid) before += red[w];
+        tot += red[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return before + incl - v;
+}
+
+}  // namespace cd

@@ -1,0 +1,29 @@
+// cd_kernels.h -- host-side launchers of the codon-design kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "cd_layout.h"
+
+namespace cd {
+
+enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
+
+size_t design_lds_bytes(int L, int m);
+hipError_t set_design_lds_limit(size_t bytes);
+
+// items: (job, design or child)
+hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s);
+// items: (job, first row, word, 0)
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, hipStream_t s);
+// one workgroup per job
+hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);
+// items: (job, first design)
+hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s);
+// one workgroup per job
+hipError_t launch_niche(const DevJob *jobs, int njobs, hipStream_t s);
+hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s);
+hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s);
+
+}  // namespace cd

@@ -1,0 +1,1166 @@
+// cd_kernels.hip -- HIP kernels of the codon-design engine for gfx950.
+//
+// One generation is five launches over every resident job:
+//   k_design<OFFSPRING>  one workgroup per child: survivor copy, mutation,
+//                        evaluation (mutationKernel, main.cu:121-240)
+//   k_dominance          bit-packed eps-domination matrix (sorting.cuh:994-1023)
+//   k_fronts             one workgroup per job: front peeling, ideal/worst,
+//                        extreme points, hyperplane/MNDF nadir
+//                        (sorting.cuh:974-1089, 132-972)
+//   k_associate          nearest reference line per ranked design
+//                        (sorting.cuh:1105-1180)
+//   k_niche              one workgroup per job: niche counts, level sweeps,
+//                        survivor list (sorting.cuh:1182-1279)
+// Work lists map blockIdx.x to (job, item) so the same kernels advance one
+// job or a batch of uneven jobs.
+#include <cfloat>
+
+#include "cd_device.h"
+#include "cd_kernels.h"
+
+namespace cd {
+
+__host__ __device__ constexpr int align16(int x) { return (x + 15) & ~15; }
+
+size_t design_lds_bytes(int L, int m) {
+    return (size_t)align16(m * L) + align16(L) + align16(3 * L) + (size_t)m * sizeof(CdsStat) + 16;
+}
+
+struct DesignShared {
+    float pobj[NOBJ];
+    int32_t pidx[2 * NOBJ];
+    int32_t ppql[4];
+    int32_t dirty[MAX_M];
+    int32_t hd[MAX_M * (MAX_M - 1) / 2];
+    int64_t red[64];
+    uint64_t redu[16];
+    int32_t misc[16];
+};
+
+// ---------------------------------------------------------------------------
+// Stem scan of one CDS held as nucleotide codes in LDS: per anti-diagonal
+// i + j = S, walk i upwards and count consecutive complementary pairs with
+// j - i > MIN_HAIRPIN (calMaximumSL, utils.cuh:712-888).  Returns the best
+// stem key of this thread.
+// ---------------------------------------------------------------------------
+__device__ uint64_t sl_scan(const uint8_t *s, int n) {
+    uint64_t best = 0;
+    int bl = 0;
+    for (int S = MIN_HAIRPIN + 1 + (int)threadIdx.x; S <= 2 * n - 3; S += blockDim.x) {
+        const int i0 = max(0, S - (n - 1));
+        const int i1 = (S - (MIN_HAIRPIN + 1)) >> 1;
+        int run = 0;
+        for (int i = i0; i <= i1; ++i) {
+            if ((s[i] ^ s[S - i]) == 3) {
+                ++run;
+                if (run >= bl) {
+                    const uint64_t k = stem_key(run, i - run + 1, S - i);
+                    if (k > best) {
+                        best = k;
+                        bl = run;
+                    }
+                }
+            } else {
+                run = 0;
+            }
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation of the CDS flagged dirty, then the six objectives
+// (calMinimumCAI/CBP/HSC/HD, calMaximumGC3/SL, utils.cuh:271-888).
+// ---------------------------------------------------------------------------
+__device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, CdsStat *cst, DesignShared &sh,
+                         float *obj, int32_t *oidx, int32_t *opql) {
+    const int L = J.L, m = J.m, n = 3 * L;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    if (tid == 0) {
+        unsigned long long nd = 0;
+        for (int c = 0; c < m; ++c) nd += sh.dirty[c] != 0;
+        atomicAdd(&J.st->cds_scanned, nd);
+    }
+    for (int c = 0; c < m; ++c) {
+        if (!sh.dirty[c]) continue;
+        const uint8_t *g = cod + c * L;
+        int64_t v[4] = {0, 0, 0, 0};
+        for (int k = tid; k < L; k += NT) {
+            const int a = g[k];
+            v[0] += J.lw[a];
+            v[3] += J.gc3[a];
+            if (k < L - 1) {
+                const int b = g[k + 1];
+                v[2] += J.hs[a * NCODON + b];
+                if (k < L - 2) v[1] += J.cpfx[a * NSENSE + b];
+            }
+            ntb[3 * k] = J.nt[3 * a];
+            ntb[3 * k + 1] = J.nt[3 * a + 1];
+            ntb[3 * k + 2] = J.nt[3 * a + 2];
+        }
+        block_sum_i64<4>(v, sh.red);  // also publishes ntb
+        uint64_t key = sl_scan(ntb, n);
+        key = block_max_u64(key, sh.redu);
+        if (tid == 0) {
+            int l, p, q;
+            stem_unkey(key, l, p, q);
+            CdsStat s;
+            s.cai = v[0];
+            s.cpb = v[1];
+            s.hsc = (int32_t)v[2];
+            s.gc3 = (int32_t)v[3];
+            s.l = l;
+            s.p = p;
+            s.q = q;
+            s.pad = 0;
+            cst[c] = s;
+        }
+        __syncthreads();
+    }
+    // pairwise Hamming distances, four pairs per reduction
+    const int npair = m * (m - 1) / 2;
+    for (int p0 = 0; p0 < npair; p0 += 4) {
+        int64_t v[4] = {0, 0, 0, 0};
+        int pi[4], pj[4];
+        for (int u = 0; u < 4; ++u) {
+            int q = p0 + u, i = 0;
+            if (q >= npair) {
+                pi[u] = pj[u] = -1;
+                continue;
+            }
+            while (q >= m - 1 - i) {
+                q -= m - 1 - i;
+                ++i;
+            }
+            pi[u] = i;
+            pj[u] = i + 1 + q;
+        }
+        for (int k = tid; k < L; k += NT)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pi[u] >= 0) v[u] += J.nd[cod[pi[u] * L + k] * NCODON + cod[pj[u] * L + k]];
+        block_sum_i64<4>(v, sh.red);
+        if (tid == 0)
+            for (int u = 0; u < 4 && p0 + u < npair; ++u) sh.hd[p0 + u] = (int32_t)v[u];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float cai_v = 0.f, cpb_v = 0.f, hsc_v = 0.f, gc_v = 0.f, hd_v = 0.f;
+        for (int i = 0; i < 2 * NOBJ; ++i) oidx[i] = 0;
+        int bl = 0, bp = 0, bq = 0, bc = m - 1;
+        for (int c = 0; c < m; ++c) {
+            const CdsStat s = cst[c];
+            const float cai = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
+            if (c == 0 || -cai >= cai_v) {  // ties -> last CDS, utils.cuh:327
+                cai_v = -cai;
+                oidx[0] = c;
+            }
+            const float cpb = (float)(((double)s.cpb * 0x1p-40) / (double)(L - 2));
+            if (c == 0 || -cpb >= cpb_v) {
+                cpb_v = -cpb;
+                oidx[2] = c;
+            }
+            const float hq = (float)s.hsc / (float)(L - 1);
+            if (c == 0 || -hq >= hsc_v) {
+                hsc_v = -hq;
+                oidx[4] = c;
+            }
+            const float frac = (float)s.gc3 / (float)L;
+            const float dev = fabsf(frac - J.gc3_ref);
+            if (c == 0 || dev >= gc_v) {
+                gc_v = dev;
+                oidx[8] = c;
+                oidx[9] = (frac - J.gc3_ref) < 0 ? GC_UP : GC_DOWN;
+            }
+            if (s.l >= bl) {
+                bl = s.l;
+                bp = s.p;
+                bq = s.q;
+                bc = c;
+            }
+        }
+        int q = 0;
+        for (int i = 0; i < m - 1; ++i)
+            for (int j = i + 1; j < m; ++j, ++q) {
+                const float dq = (float)sh.hd[q] / (float)n;
+                if ((i == 0 && j == 1) || -dq >= hd_v) {
+                    hd_v = -dq;
+                    oidx[6] = i;
+                    oidx[7] = j;
+                }
+            }
+        obj[0] = cai_v;
+        obj[1] = cpb_v;
+        obj[2] = hsc_v;
+        obj[3] = hd_v;
+        obj[4] = gc_v;
+        obj[5] = (float)bl / (float)J.den_sl;
+        oidx[10] = bc;
+        opql[0] = bp;
+        opql[1] = bq;
+        opql[2] = bl;
+        opql[3] = 0;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Mutation operators (include/mutation.cuh) on global codon indices in LDS.
+// Each thread owns whole codons; operators that read neighbours only touch
+// codons of the other parity or of other CDS, as in the reference.
+// ---------------------------------------------------------------------------
+struct MutArgs {
+    const DevJob *J;
+    uint8_t *cod;
+    const int8_t *aa;
+    uint32_t gen, child;
+};
+
+// mutationRandom, mutation.cuh:16-63
+__device__ void mut_random(const MutArgs &A, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L, mL = J.m * L;
+    for (int t = threadIdx.x; t < mL; t += blockDim.x) {
+        const int c = t / L, k = t - c * L;
+        uint32_t r[4];
+        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
+        const int a = A.aa[k], ns = c_syn_num[a];
+        if (!bern(r[0], J.thr_pm) || ns == 1) continue;
+        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        int nw = (int)uniform_int(r[1], (uint32_t)(ns - 1));
+        if (nw >= cur) ++nw;
+        A.cod[t] = (uint8_t)(base + nw);
+        dirty[c] = 1;
+    }
+}
+
+// mutationCAI with SELECT_UPPER_RANDOM, mutation.cuh:65-142
+__device__ void mut_cai(const MutArgs &A, int cds, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L;
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        const int t = cds * L + k;
+        uint32_t r[4];
+        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
+        const int a = A.aa[k], ns = c_syn_num[a];
+        if (!bern(r[0], J.thr_pm) || ns == 1) continue;
+        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        if (cur != ns - 1) {
+            A.cod[t] = (uint8_t)(base + cur + 1 + (int)uniform_int(r[1], (uint32_t)(ns - 1 - cur)));
+            dirty[cds] = 1;
+        }
+    }
+}
+
+// mutationCBP with SELECT_UPPER_RANDOM, mutation.cuh:144-262
+__device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L, parity = (int)(A.gen & 1u);
+    for (int k = threadIdx.x; k < L - 1; k += blockDim.x) {
+        if ((k & 1) != parity) continue;
+        const int t = cds * L + k;
+        uint32_t r[4];
+        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
+        const int a = A.aa[k], ns = c_syn_num[a];
+        if (!bern(r[0], J.thr_2pm) || ns == 1) continue;
+        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
+        const int right = A.cod[t + 1];
+        auto score = [&](int g) -> float {
+            float v = 0.f;
+            if (left >= 0) v += J.cps[left * NSENSE + g];
+            if (right < NSENSE) v += J.cps[g * NSENSE + right];
+            return v;
+        };
+        const float curv = score(base + cur);
+        int cand[6], nc = 0;
+        for (int j = 0; j < ns; ++j)
+            if (j != cur && score(base + j) > curv) cand[nc++] = j;
+        if (nc > 0) {
+            A.cod[t] = (uint8_t)(base + cand[uniform_int(r[1], (uint32_t)nc)]);
+            dirty[cds] = 1;
+        }
+    }
+}
+
+// mutationHSC with SELECT_UPPER_RANDOM, mutation.cuh:264-388
+__device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L, parity = (int)(A.gen & 1u);
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        if ((k & 1) != parity) continue;
+        const int t = cds * L + k;
+        uint32_t r[4];
+        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
+        const int a = A.aa[k], ns = c_syn_num[a];
+        if (!bern(r[0], J.thr_2pm) || ns == 1) continue;
+        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
+        const int right = (k < L - 1) ? (int)A.cod[t + 1] : -1;
+        auto count = [&](int g) -> int {
+            int v = 0;
+            if (left >= 0) v += J.hs[left * NCODON + g];
+            if (right >= 0) v += J.hs[g * NCODON + right];
+            return v;
+        };
+        const int curv = count(base + cur);
+        int cand[6], nc = 0;
+        for (int j = 0; j < ns; ++j)
+            if (j != cur && count(base + j) > curv) cand[nc++] = j;
+        if (nc > 0) {
+            A.cod[t] = (uint8_t)(base + cand[uniform_int(r[1], (uint32_t)nc)]);
+            dirty[cds] = 1;
+        }
+    }
+}
+
+__device__ __forceinline__ void slot_draws(const MutArgs &A, uint32_t slot, uint32_t d[6]) {
+    uint32_t a[4], b[4];
+    philox(A.gen, A.child, slot, TAG_MUT, A.J->seed, a);
+    philox(A.gen, A.child, slot, TAG_MUTB, A.J->seed, b);
+    d[0] = a[0];
+    d[1] = a[1];
+    d[2] = a[2];
+    d[3] = a[3];
+    d[4] = b[0];
+    d[5] = b[1];
+}
+
+// indexArrayShuffling, utils.cuh:203-218
+__device__ __forceinline__ void shuffle(const uint32_t *draws, int n, int8_t *arr) {
+    for (int i = 0; i < 6; ++i) arr[i] = (int8_t)(i < n ? i : -1);
+    int d = 0;
+    for (int i = n - 1; i > 0; --i) {
+        const int j = (int)uniform_int(draws[d++], (uint32_t)(i + 1));
+        const int8_t tmp = arr[i];
+        arr[i] = arr[j];
+        arr[j] = tmp;
+    }
+}
+
+// mutationHD, mutation.cuh:390-534
+__device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L, m = J.m;
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        uint32_t d[6];
+        slot_draws(A, (uint32_t)(cds1 * L + k), d);
+        const int a = A.aa[k], ns = c_syn_num[a];
+        if (!bern(d[0], J.thr_pm) || ns == 1) continue;
+        int8_t order[6];
+        shuffle(d + 1, ns, order);
+        const int base = c_syn_start[a], cur = A.cod[cds1 * L + k] - base;
+        const int g2 = A.cod[cds2 * L + k];
+        auto min_hd = [&](int g) -> int {
+            int mn = 127;
+            for (int o = 0; o < m; ++o) {
+                if (o == cds1) continue;
+                const int t = J.nd[g * NCODON + A.cod[o * L + k]];
+                mn = t < mn ? t : mn;
+            }
+            return mn;
+        };
+        const int cur_pair = J.nd[(base + cur) * NCODON + g2];
+        const int cur_mhd = min_hd(base + cur);
+        int best_pair = -1, best_mhd = -1, best = -1;
+        for (int j = 0; j < ns; ++j) {
+            const int sh = order[j];
+            if (sh == cur) continue;
+            const int np_ = J.nd[(base + sh) * NCODON + g2];
+            const int nm = min_hd(base + sh);
+            if (nm > cur_mhd && nm > best_mhd) {
+                best_mhd = nm;
+                best = sh;
+            } else if (best_mhd == -1 && nm == cur_mhd && np_ > cur_pair && np_ > best_pair) {
+                best_pair = np_;
+                best = sh;
+            }
+        }
+        if (best_mhd != -1 || best_pair != -1) {
+            A.cod[cds1 * L + k] = (uint8_t)(base + best);
+            dirty[cds1] = 1;
+        }
+    }
+}
+
+// mutationGC3, mutation.cuh:536-653 (probability = the parent's GC3 deviation)
+__device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L;
+    const uint64_t thr = bern_threshold(dev);
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        uint32_t d[6];
+        slot_draws(A, (uint32_t)(cds * L + k), d);
+        const int a = A.aa[k], ns = c_syn_num[a];
+        if (!bern(d[0], thr) || ns == 1) continue;
+        int8_t order[6];
+        shuffle(d + 1, ns, order);
+        const int base = c_syn_start[a], cur = A.cod[cds * L + k] - base;
+        const int curg = J.gc3[base + cur];
+        for (int j = 0; j < ns; ++j) {
+            const int sh = order[j];
+            if (sh == cur) continue;
+            const int ng = J.gc3[base + sh];
+            if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
+                A.cod[cds * L + k] = (uint8_t)(base + sh);
+                dirty[cds] = 1;
+                break;
+            }
+        }
+    }
+}
+
+// mutationSL, mutation.cuh:775-969.  The accept test of the reference
+// (mutation.cuh:843,896,947) compares the parent's SL objective with the
+// integer quotient l_new / ((n-4)/2); it only needs to know whether the CDS
+// still has any stem and whether it is one full-length hairpin, which two
+// O(n) workgroup checks decide.
+__device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t *dirty) {
+    const DevJob &J = *A.J;
+    const int L = J.L, n = 3 * L, tid = threadIdx.x, NT = blockDim.x;
+    const int cds = sh.pidx[10];
+    const int P0 = sh.ppql[0], Lst = sh.ppql[2];
+    const int l_idx = P0 / 3, r_idx = (P0 + Lst - 1) / 3;
+    const int mid = (r_idx + l_idx) / 2;
+    const int stem_len = r_idx - l_idx + 1, half = stem_len / 2;
+    const bool odd = (stem_len % 2) == 1;
+    const int l_st0 = odd ? mid - 1 : mid, r_st0 = mid + 1;
+    const int den = J.den_sl;
+    const float parent_sl = sh.pobj[5];
+    const int trials = (odd ? 1 : 0) + 2 * half;
+    int *first = sh.misc;      // [4]
+    int *last = sh.misc + 4;   // [4]
+    int *full = sh.misc + 8;   // [3]
+    int *ctl = sh.misc + 11;   // [0] changed, [1] saved, [2] terminate, [3] cur_L
+    if (tid == 0) ctl[3] = Lst;
+    __syncthreads();
+    for (int t = 0; t < trials; ++t) {
+        int pos;
+        if (odd && t == 0) {
+            pos = mid;
+        } else {
+            const int u = t - (odd ? 1 : 0);
+            pos = (u & 1) ? r_st0 + (u >> 1) : l_st0 - (u >> 1);
+        }
+        if (tid == 0) {
+            uint32_t r[4];
+            philox(A.gen, A.child, (uint32_t)t, TAG_SL, J.seed, r);
+            const int a = A.aa[pos], ns = c_syn_num[a], base = c_syn_start[a];
+            ctl[0] = 0;
+            ctl[1] = A.cod[cds * L + pos];
+            if (bern(r[0], J.thr_pm) && ns > 1) {
+                const int cur = A.cod[cds * L + pos] - base;
+                int nw = (int)uniform_int(r[1], (uint32_t)(ns - 1));
+                if (nw == cur) ++nw;  // mutation.cuh:827 compares with ==
+                A.cod[cds * L + pos] = (uint8_t)(base + nw);
+                ctl[0] = 1;
+            }
+        }
+        if (tid < 4) {
+            first[tid] = 0x7FFFFFFF;
+            last[tid] = -1;
+        }
+        if (tid < 3) full[tid] = 1;
+        __syncthreads();
+        for (int k = tid; k < L; k += NT) {
+            const int g = A.cod[cds * L + k];
+            ntb[3 * k] = J.nt[3 * g];
+            ntb[3 * k + 1] = J.nt[3 * g + 1];
+            ntb[3 * k + 2] = J.nt[3 * g + 2];
+        }
+        __syncthreads();
+        int f[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, l4[4] = {-1, -1, -1, -1};
+        int okA = 1, okB = 1, okC = 1;
+        for (int i = tid; i < n; i += NT) {
+            const int b = ntb[i];
+            f[b] = min(f[b], i);
+            l4[b] = max(l4[b], i);
+        }
+        for (int i = tid; i < den; i += NT) {
+            okA &= ((ntb[i] ^ ntb[n - 1 - i]) == 3);
+            if (n & 1) {
+                okB &= ((ntb[i] ^ ntb[n - 2 - i]) == 3);
+                okC &= ((ntb[1 + i] ^ ntb[n - 1 - i]) == 3);
+            }
+        }
+        for (int b = 0; b < 4; ++b) {
+            if (f[b] != 0x7FFFFFFF) atomicMin(&first[b], f[b]);
+            if (l4[b] >= 0) atomicMax(&last[b], l4[b]);
+        }
+        if (!okA) atomicAnd(&full[0], 0);
+        if (!okB) atomicAnd(&full[1], 0);
+        if (!okC) atomicAnd(&full[2], 0);
+        __syncthreads();
+        if (tid == 0) {
+            bool any = false;
+            for (int a = 0; a < 4; ++a) {
+                const int b = a ^ 3;
+                if (first[a] != 0x7FFFFFFF && last[b] >= 0 && last[b] - first[a] > MIN_HAIRPIN) any = true;
+            }
+            bool is_full = false;
+            if (den > 0) {
+                if (full[0] && (n - 1 - 2 * (den - 1)) > MIN_HAIRPIN) is_full = true;
+                if ((n & 1) && (full[1] || full[2]) && (n - 2 - 2 * (den - 1)) > MIN_HAIRPIN) is_full = true;
+            }
+            if (any) ctl[3] = is_full ? den : 0;
+            const float rhs = (float)(den > 0 ? ctl[3] / den : 0);
+            ctl[2] = parent_sl > rhs;
+            if (!ctl[2] && ctl[0]) A.cod[cds * L + pos] = (uint8_t)ctl[1];
+            if (ctl[2] && ctl[0]) dirty[cds] = 1;
+        }
+        __syncthreads();
+        if (ctl[2]) break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_design: one workgroup per design.
+//   INIT       random genome (design 0: best-CAI codons), genPopulation,
+//              utils.cuh:1047-1104, then evaluation into buffer 0
+//   EVAL       evaluation of genome[0][d] into buffer 0
+//   OFFSPRING  survivor copy, mutation, evaluation (mutationKernel)
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items) {
+    extern __shared__ __align__(16) uint8_t dyn[];
+    __shared__ DesignShared sh;
+    const int2 it = items[blockIdx.x];
+    const DevJob &J = jobs[it.x];
+    const int L = J.L, m = J.m, mL = m * L, N = J.N;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    uint8_t *cod = dyn;
+    int8_t *aa = (int8_t *)(dyn + align16(mL));
+    uint8_t *ntb = (uint8_t *)aa + align16(L);
+    CdsStat *cst = (CdsStat *)(ntb + align16(3 * L));
+
+    uint32_t gen = 0;
+    int dst = 0, out_slot = it.y;
+    for (int k = tid; k < L; k += NT) aa[k] = J.aa[k];
+    if (tid < MAX_M) sh.dirty[tid] = (MODE != 2);
+    __syncthreads();
+
+    if (MODE == 0) {
+        const int d = it.y;
+        for (int t = tid; t < mL; t += NT) {
+            const int k = t % L, a = aa[k], ns = c_syn_num[a];
+            int s;
+            if (d == 0) {
+                s = ns - 1;
+            } else {
+                uint32_t r[4];
+                philox(0u, (uint32_t)d, (uint32_t)t, TAG_INIT, J.seed, r);
+                s = (int)uniform_int(r[0], (uint32_t)ns);
+            }
+            cod[t] = (uint8_t)(c_syn_start[a] + s);
+        }
+    } else if (MODE == 1) {
+        const int8_t *src = J.genome[0] + (size_t)it.y * mL;
+        for (int t = tid; t < mL; t += NT) cod[t] = (uint8_t)(c_syn_start[aa[t % L]] + src[t]);
+    } else {
+        const int c = it.y;
+        gen = J.st->gen;
+        const int src = (int)(gen & 1u);
+        dst = src ^ 1;
+        out_slot = N + c;
+        const int parent = J.sorted[c];
+        const int8_t *pg = J.genome[src] + (size_t)parent * mL;
+        int8_t *sg = J.genome[dst] + (size_t)c * mL;
+        for (int t = tid; t < mL; t += NT) {
+            const int8_t v = pg[t];
+            sg[t] = v;
+            cod[t] = (uint8_t)(c_syn_start[aa[t % L]] + v);
+        }
+        // survivor records (copySolution, utils.cuh:1106-1147)
+        if (tid < NOBJ) {
+            const float v = J.obj[src][(size_t)parent * NOBJ + tid];
+            J.obj[dst][(size_t)c * NOBJ + tid] = v;
+            sh.pobj[tid] = v;
+        }
+        if (tid < 2 * NOBJ) {
+            const int8_t v = J.oidx[src][(size_t)parent * 2 * NOBJ + tid];
+            J.oidx[dst][(size_t)c * 2 * NOBJ + tid] = v;
+            sh.pidx[tid] = v;
+        }
+        if (tid < 4) {
+            const int32_t v = J.pql[src][(size_t)parent * 4 + tid];
+            J.pql[dst][(size_t)c * 4 + tid] = v;
+            sh.ppql[tid] = v;
+        }
+        if (tid < m) {
+            const CdsStat s = J.cst[src][(size_t)parent * m + tid];
+            J.cst[dst][(size_t)c * m + tid] = s;
+            cst[tid] = s;
+        }
+        if (tid == 0) {
+            J.np[c] = 0;
+            J.np[N + c] = 0;
+        }
+        __syncthreads();
+        uint32_t r[4];
+        philox(gen, (uint32_t)c, 0u, TAG_TYPE, J.seed, r);
+        const int type = (int)uniform_int(r[0], (uint32_t)(NOBJ + 1));  // main.cu:172-178
+        const MutArgs A{&J, cod, aa, gen, (uint32_t)c};
+        switch (type) {  // main.cu:181-211
+        case 0: mut_random(A, sh.dirty); break;
+        case 1: mut_cai(A, sh.pidx[0], sh.dirty); break;
+        case 2: mut_cbp(A, sh.pidx[2], sh.dirty); break;
+        case 3: mut_hsc(A, sh.pidx[4], sh.dirty); break;
+        case 4: mut_hd(A, sh.pidx[6], sh.pidx[7], sh.dirty); break;
+        case 5:
+            if (sh.pidx[9] == GC_UP || sh.pidx[9] == GC_DOWN) mut_gc3(A, sh.pidx[8], sh.pidx[9], sh.pobj[4], sh.dirty);
+            break;
+        default: mut_sl(A, ntb, sh, sh.dirty); break;
+        }
+    }
+    __syncthreads();
+
+    float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
+    __shared__ float s_obj[NOBJ];
+    __shared__ int32_t s_idx[2 * NOBJ], s_pql[4];
+    evaluate(J, cod, ntb, cst, sh, s_obj, s_idx, s_pql);
+    int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
+    for (int t = tid; t < mL; t += NT) og[t] = (int8_t)(cod[t] - c_syn_start[aa[t % L]]);
+    if (tid < NOBJ) oobj[tid] = s_obj[tid];
+    if (tid < 2 * NOBJ) J.oidx[dst][(size_t)out_slot * 2 * NOBJ + tid] = (int8_t)s_idx[tid];
+    if (tid < 4) J.pql[dst][(size_t)out_slot * 4 + tid] = s_pql[tid];
+    if (tid < m) J.cst[dst][(size_t)out_slot * m + tid] = cst[tid];
+}
+
+// ---------------------------------------------------------------------------
+// Selection
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pool_buffer(const DevJob &J) { return (int)((J.st->gen + 1u) & 1u); }
+
+// dominate(), sorting.cuh:94-130: a dominates b
+__device__ __forceinline__ bool dominates(const float *a, const float *b) {
+    bool any = false, bad = false;
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        const bool sig = fabsf(a[k] - b[k]) > F_PRECISION;
+        any |= sig;
+        bad |= sig && (a[k] > b[k]);
+    }
+    return any && !bad;
+}
+
+// Bit i of dom[j][w] (i = 64w + bit) is set when design i dominates design j;
+// np[j] counts the dominators.  item = (job, first row, word, 0).
+__global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items) {
+    __shared__ float cand[64 * NOBJ];
+    const int4 it = items[blockIdx.x];
+    const DevJob &J = jobs[it.x];
+    const int P = J.P, W = J.W, word = it.z;
+    const float *obj = J.obj[pool_buffer(J)];
+    for (int t = threadIdx.x; t < 64 * NOBJ; t += blockDim.x) {
+        const int i = word * 64 + t / NOBJ;
+        cand[t] = (i < P) ? obj[(size_t)i * NOBJ + t % NOBJ] : 0.f;
+    }
+    __syncthreads();
+    const int r = it.y + (int)threadIdx.x;
+    if (r >= P) return;
+    float mine[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) mine[k] = obj[(size_t)r * NOBJ + k];
+    uint64_t bits = 0;
+    const int nb = min(64, P - word * 64);
+    for (int b = 0; b < nb; ++b) {
+        const int i = word * 64 + b;
+        if (i != r && dominates(cand + b * NOBJ, mine)) bits |= 1ull << b;
+    }
+    J.dom[(size_t)r * W + word] = bits;
+    if (bits) atomicAdd(&J.np[r], (int)__popcll(bits));
+}
+
+__device__ __forceinline__ float asf(const float *x, const float *ideal, int axis) {  // sorting.cuh:428-449
+    float mx = -FLT_MAX;
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        float t = x[k] - ideal[k];
+        if ((double)t < 0.001) t = 0.f;
+        const float v = t / (k == axis ? 1.f : F_PRECISION);
+        if (v > mx) mx = v;
+    }
+    return mx;
+}
+
+// GaussianElimination and helpers, sorting.cuh:451-599 (E x = 1)
+__device__ void gauss(const float *E, float *sol) {
+    float AB[NOBJ][NOBJ + 1];
+    for (int r = 0; r < NOBJ; ++r) {
+        for (int c = 0; c < NOBJ; ++c) AB[r][c] = E[r * NOBJ + c];
+        AB[r][NOBJ] = 1.0f;
+    }
+    for (int col = 0; col < NOBJ; ++col) {
+        if (fabsf(AB[col][col]) <= 1e-6f) {
+            int row = col + 1;
+            for (; row < NOBJ; ++row)
+                if (fabsf(AB[row][col]) > 1e-6f) break;
+            if (row < NOBJ)
+                for (int c = col; c < NOBJ + 1; ++c) AB[col][c] += AB[row][c];
+        }
+        for (int row = col + 1; row < NOBJ; ++row) {
+            const float f = AB[row][col] / AB[col][col];
+            for (int c = col; c < NOBJ + 1; ++c) AB[row][c] -= f * AB[col][c];
+        }
+    }
+    for (int row = NOBJ - 1; row >= 0; --row) {
+        float s = 0.f;
+        for (int c = row + 1; c < NOBJ; ++c) s += AB[row][c];
+        float x = AB[row][NOBJ];
+        if (row != NOBJ - 1) x -= s;
+        x /= AB[row][row];
+        AB[row][NOBJ] = x;
+        AB[row][row] = 1.0f;
+        for (int r = 0; r < row; ++r) AB[r][row] *= x;
+    }
+    for (int r = 0; r < NOBJ; ++r) sol[r] = AB[r][NOBJ];
+}
+
+__device__ __forceinline__ void argmin_pair(float &v, int &id, float ov, int oid) {
+    if (ov < v || (ov == v && oid < id)) {
+        v = ov;
+        id = oid;
+    }
+}
+
+// One workgroup per job: nonDominatedSorting's peeling (sorting.cuh:1026-1086),
+// updateIdealValue/updateWorstValue, findExtremePoints, updateNadirValue_HYP
+// with the MNDF fallback (sorting.cuh:132-972).
+__global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    const int P = J.P, W = J.W, N = J.N;
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+    const float *obj = J.obj[pool_buffer(J)];
+    __shared__ uint64_t ranked[MAX_W], newm[MAX_W];
+    __shared__ int s_cnt, s_new;
+    __shared__ float red_v[16 * 18];
+    __shared__ int red_i[16];
+    __shared__ float ext[NOBJ * NOBJ], ideal[NOBJ], worst[NOBJ], f0max[NOBJ];
+    __shared__ float win[NOBJ];
+
+    for (int w = tid; w < W; w += NT) {
+        ranked[w] = 0;
+        newm[w] = 0;
+    }
+    if (tid == 0) {
+        s_cnt = 0;
+        s_new = 0;
+    }
+    if (tid < NOBJ * NOBJ) ext[tid] = J.st->ext[tid];
+    __syncthreads();
+    for (int r = tid; r < P; r += NT) {
+        const bool f0 = J.np[r] == 0;
+        J.front_of[r] = f0 ? 0 : -1;
+        if (f0) atomicOr((unsigned long long *)&ranked[r >> 6], 1ull << (r & 63));
+    }
+    __syncthreads();
+    {
+        int c = 0;
+        for (int w = tid; w < W; w += NT) c += __popcll(ranked[w]);
+        atomicAdd(&s_cnt, c);
+    }
+    __syncthreads();
+    int f = 0, last_size = s_cnt;
+    while (s_cnt < N) {
+        ++f;
+        // a design joins front f when all its dominators are ranked
+        for (int r = wid; r < P; r += nw) {
+            if ((ranked[r >> 6] >> (r & 63)) & 1ull) continue;
+            bool any = false;
+            for (int w0 = 0; w0 < W && !any; w0 += 64) {
+                const int w = w0 + lane;
+                const uint64_t x = (w < W) ? (J.dom[(size_t)r * W + w] & ~ranked[w]) : 0ull;
+                any = __any(x != 0ull);
+            }
+            if (!any && lane == 0) {
+                atomicOr((unsigned long long *)&newm[r >> 6], 1ull << (r & 63));
+                J.front_of[r] = f;
+            }
+        }
+        __syncthreads();
+        {
+            int c = 0;
+            for (int w = tid; w < W; w += NT) c += __popcll(newm[w]);
+            atomicAdd(&s_new, c);
+        }
+        __syncthreads();
+        if (s_new == 0) {  // a dominance cycle: every remaining design forms the front
+            for (int r = tid; r < P; r += NT)
+                if (!((ranked[r >> 6] >> (r & 63)) & 1ull)) {
+                    atomicOr((unsigned long long *)&newm[r >> 6], 1ull << (r & 63));
+                    J.front_of[r] = f;
+                }
+            __syncthreads();
+            if (tid == 0) s_new = P - s_cnt;
+            __syncthreads();
+        }
+        for (int w = tid; w < W; w += NT) {
+            ranked[w] |= newm[w];
+            newm[w] = 0;
+        }
+        last_size = s_new;
+        __syncthreads();
+        if (tid == 0) {
+            s_cnt += s_new;
+            s_new = 0;
+        }
+        __syncthreads();
+    }
+    const int ranked_n = s_cnt;
+    // ideal / worst over the pool, maximum over F0
+    float mn[NOBJ], mx[NOBJ], fm[NOBJ];
+    for (int k = 0; k < NOBJ; ++k) {
+        mn[k] = FLT_MAX;
+        mx[k] = -FLT_MAX;
+        fm[k] = -FLT_MAX;
+    }
+    for (int r = tid; r < P; r += NT) {
+        const bool f0 = J.front_of[r] == 0;
+        for (int k = 0; k < NOBJ; ++k) {
+            const float v = obj[(size_t)r * NOBJ + k];
+            mn[k] = fminf(mn[k], v);
+            mx[k] = fmaxf(mx[k], v);
+            if (f0) fm[k] = fmaxf(fm[k], v);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < NOBJ; ++k) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+            fm[k] = fmaxf(fm[k], __shfl_xor(fm[k], o, 64));
+        }
+    if (lane == 0)
+        for (int k = 0; k < NOBJ; ++k) {
+            red_v[wid * 18 + k] = mn[k];
+            red_v[wid * 18 + 6 + k] = mx[k];
+            red_v[wid * 18 + 12 + k] = fm[k];
+        }
+    __syncthreads();
+    if (tid < NOBJ) {
+        float a = FLT_MAX, b = -FLT_MAX, c = -FLT_MAX;
+        for (int w = 0; w < nw; ++w) {
+            a = fminf(a, red_v[w * 18 + tid]);
+            b = fmaxf(b, red_v[w * 18 + 6 + tid]);
+            c = fmaxf(c, red_v[w * 18 + 12 + tid]);
+        }
+        ideal[tid] = a;
+        worst[tid] = b;
+        f0max[tid] = c;
+    }
+    __syncthreads();
+    // findExtremePoints: candidates are F0 (ascending index) then the previous extremes
+    for (int axis = 0; axis < NOBJ; ++axis) {
+        float bv = FLT_MAX;
+        int bi = 0x7FFFFFFF;
+        for (int r = tid; r < P; r += NT)
+            if (J.front_of[r] == 0) argmin_pair(bv, bi, asf(obj + (size_t)r * NOBJ, ideal, axis), r);
+        if (tid < NOBJ) argmin_pair(bv, bi, asf(ext + tid * NOBJ, ideal, axis), P + tid);
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(bv, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            argmin_pair(bv, bi, ov, oi);
+        }
+        if (lane == 0) {
+            red_v[wid] = bv;
+            red_i[wid] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float v = red_v[0];
+            int id = red_i[0];
+            for (int w = 1; w < nw; ++w) argmin_pair(v, id, red_v[w], red_i[w]);
+            if (id != 0x7FFFFFFF && v < FLT_MAX)
+                for (int k = 0; k < NOBJ; ++k)
+                    win[k] = (id >= P) ? ext[(id - P) * NOBJ + k] : obj[(size_t)id * NOBJ + k];
+            else
+                for (int k = 0; k < NOBJ; ++k) win[k] = ext[axis * NOBJ + k];
+        }
+        __syncthreads();
+        if (tid < NOBJ) ext[axis * NOBJ + tid] = win[tid];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        SelState *st = J.st;
+        float x[NOBJ], icpt[NOBJ];
+        gauss(ext, x);
+        bool exc = false;
+        for (int t = 0; t < NOBJ; ++t) {
+            float res = 0.f;
+            for (int i = 0; i < NOBJ; ++i) res += x[i] * (ext[t * NOBJ + i] - ideal[i]);
+            if (fabsf(res - 1.f) > F_PRECISION || isnan(res) || isinf(res)) exc = true;
+            icpt[t] = 1.f / x[t];
+            if (icpt[t] <= F_PRECISION || isnan(icpt[t]) || isinf(icpt[t])) exc = true;
+        }
+        for (int k = 0; k < NOBJ; ++k) {
+            st->ideal[k] = ideal[k];
+            st->worst[k] = worst[k];
+            st->nadir[k] = exc ? (((f0max[k] - ideal[k]) > F_PRECISION) ? f0max[k] : worst[k]) : ideal[k] + icpt[k];
+        }
+        for (int i = 0; i < NOBJ * NOBJ; ++i) st->ext[i] = ext[i];
+        st->hyp = exc;
+        st->ranked = ranked_n;
+        st->last = f;
+        st->base = (f == 0) ? 0 : ranked_n - last_size;
+        st->n_cut = (ranked_n == N);
+    }
+}
+
+__device__ __forceinline__ float perp_dist(const float *f, const float *w) {  // utils.cuh:1149-1171
+    float num = 0.f, den = 0.f;
+#pragma unroll
+    for (int i = 0; i < NOBJ; ++i) {
+        num += w[i] * f[i];
+        den += w[i] * w[i];
+    }
+    const float k = num / den;
+    float d = 0.f;
+#pragma unroll
+    for (int i = 0; i < NOBJ; ++i) {
+        const float t = k * w[i] - f[i];
+        d += t * t;
+    }
+    return sqrtf(d);
+}
+
+// Nearest reference line of every ranked design (sorting.cuh:1105-1180).
+// item = (job, first design).
+__global__ __launch_bounds__(256) void k_associate(const DevJob *__restrict__ jobs, const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const DevJob &J = jobs[it.x];
+    const SelState *st = J.st;
+    if (st->n_cut) return;
+    const int d = it.y + (int)threadIdx.x;
+    if (d >= J.P || J.front_of[d] < 0) return;
+    const float *obj = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
+    float f[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        const float den = st->nadir[k] - st->ideal[k];
+        f[k] = (den == 0.f) ? 0.f : (obj[k] - st->ideal[k]) / den;
+    }
+    float best = FLT_MAX;
+    int br = 0;
+    for (int r = 0; r < J.R; ++r) {
+        const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
+        if (dd < best) {
+            best = dd;
+            br = r;
+        }
+    }
+    J.ref_of[d] = br;
+    J.dist[d] = best;
+}
+
+__device__ __forceinline__ uint64_t niche_prio(const DevJob &J, uint32_t gen, int d) {
+    uint32_t r[4];
+    philox(gen, (uint32_t)d, 0u, TAG_NICHE, J.seed, r);
+    return ((uint64_t)r[0] << 32) | (uint32_t)d;
+}
+
+// Block-wide sum of one int per thread.
+__device__ __forceinline__ int block_sum_int(int v, int *red) {
+    int total;
+    block_scan_excl(v, red, &total);
+    return total;
+}
+
+// One workgroup per job: niche counts, the level sweeps of
+// referenceBasedSorting (sorting.cuh:1182-1279) in closed form, and the
+// survivor list in ascending pool index.  Ends the generation.
+__global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    SelState *st = J.st;
+    const int P = J.P, N = J.N, R = J.R, tid = threadIdx.x, NT = blockDim.x;
+    const uint32_t gen = st->gen;
+    const int last = st->last;
+    __shared__ int red[16];
+    __shared__ int s_carry;
+    if (!st->n_cut) {
+        int *cnt = J.niche, *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R;
+        uint32_t *prio_hi = (uint32_t *)(kp + R);
+        for (int r = tid; r < R; r += NT) {
+            cnt[r] = 0;
+            rem[r] = 0;
+            fill[r] = 0;
+        }
+        __syncthreads();
+        for (int d = tid; d < P; d += NT) {
+            const int f = J.front_of[d];
+            if (f >= 0) atomicAdd(f < last ? &cnt[J.ref_of[d]] : &rem[J.ref_of[d]], 1);
+        }
+        if (tid == 0) s_carry = 0;
+        __syncthreads();
+        for (int r0 = 0; r0 < R; r0 += NT) {  // off = exclusive scan of rem
+            const int r = r0 + tid;
+            int total;
+            const int ex = block_scan_excl(r < R ? rem[r] : 0, red, &total);
+            if (r < R) off[r] = s_carry + ex;
+            __syncthreads();
+            if (tid == 0) s_carry += total;
+            __syncthreads();
+        }
+        for (int d = tid; d < P; d += NT) {
+            if (J.front_of[d] == last) {
+                const int r = J.ref_of[d];
+                J.bucket[off[r] + atomicAdd(&fill[r], 1)] = d;
+                prio_hi[d] = (uint32_t)(niche_prio(J, gen, d) >> 32);
+            }
+        }
+        __syncthreads();
+        const int need = N - st->base;
+        // picks made through level i: C(i) = sum_r clamp(i + 1 - cnt[r], 0, rem[r])
+        auto picks_through = [&](int i) -> int {
+            int s = 0;
+            for (int r = tid; r < R; r += NT) s += min(max(i + 1 - cnt[r], 0), rem[r]);
+            return block_sum_int(s, red);
+        };
+        int lo = 0, hi = P;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (picks_through(mid) >= need)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int istar = lo;
+        const int prevC = istar > 0 ? picks_through(istar - 1) : 0;
+        const int need_final = need - prevC;
+        if (tid == 0) s_carry = 0;
+        __syncthreads();
+        for (int r0 = 0; r0 < R; r0 += NT) {  // final level: the first eligible refs in index order
+            const int r = r0 + tid;
+            const int e = (r < R) && cnt[r] <= istar && istar < cnt[r] + rem[r];
+            int total;
+            const int ex = block_scan_excl(e, red, &total);
+            if (r < R) kp[r] = min(max(istar - cnt[r], 0), rem[r]) + ((e && s_carry + ex < need_final) ? 1 : 0);
+            __syncthreads();
+            if (tid == 0) s_carry += total;
+            __syncthreads();
+        }
+        for (int d = tid; d < P; d += NT) {
+            const int f = J.front_of[d];
+            uint8_t sel = (f >= 0 && f < last);
+            if (f == last) {
+                const int r = J.ref_of[d], k = kp[r];
+                const int *b = J.bucket + off[r];
+                const int nb = rem[r];
+                int lvl0 = -1;
+                if (k > 0 && cnt[r] == 0) {  // level 0 takes the nearest candidate
+                    float bd = FLT_MAX;
+                    for (int t = 0; t < nb; ++t) {
+                        const int e = b[t];
+                        const float de = J.dist[e];
+                        if (lvl0 < 0 || de < bd || (de == bd && e < lvl0)) {
+                            bd = de;
+                            lvl0 = e;
+                        }
+                    }
+                }
+                if (d == lvl0) {
+                    sel = 1;
+                } else {
+                    const int kk = k - (lvl0 >= 0 ? 1 : 0);
+                    if (kk > 0) {
+                        const uint64_t my = ((uint64_t)prio_hi[d] << 32) | (uint32_t)d;
+                        int rank = 0;
+                        for (int t = 0; t < nb; ++t) {
+                            const int e = b[t];
+                            if (e == d || e == lvl0) continue;
+                            rank += (((uint64_t)prio_hi[e] << 32) | (uint32_t)e) < my;
+                        }
+                        sel = rank < kk;
+                    }
+                }
+            }
+            J.flag[d] = sel;
+        }
+    } else {
+        for (int d = tid; d < P; d += NT) J.flag[d] = J.front_of[d] >= 0;
+    }
+    __syncthreads();
+    // survivors in ascending pool index
+    const int ch = (P + NT - 1) / NT, d0 = tid * ch, d1 = min(P, d0 + ch);
+    int c = 0;
+    for (int d = d0; d < d1; ++d) c += J.flag[d];
+    int total;
+    int pos = block_scan_excl(c, red, &total);
+    for (int d = d0; d < d1; ++d)
+        if (J.flag[d] && pos < N) J.sorted[pos++] = d;
+    __syncthreads();
+    if (tid == 0) st->gen = gen + 1;
+}
+
+// Initial state after k_design<INIT>: identity survivor list, extreme points
+// from the first designs (main.cu:101, 106-113).
+__global__ void k_init_state(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    SelState *st = J.st;
+    for (int i = threadIdx.x; i < J.N; i += blockDim.x) J.sorted[i] = i;
+    for (int i = threadIdx.x; i < J.P; i += blockDim.x) J.front_of[i] = -1;
+    if (threadIdx.x < NOBJ * NOBJ) {
+        const int e = threadIdx.x / NOBJ, k = threadIdx.x % NOBJ;
+        st->ext[threadIdx.x] = (e < J.N) ? J.obj[0][(size_t)e * NOBJ + k] : 0.f;
+    }
+    if (threadIdx.x == 0) {
+        st->gen = 0;
+        st->n_cut = st->hyp = st->ranked = st->last = st->base = 0;
+        st->cds_scanned = 0;
+    }
+}
+
+// Clear the dominator counts before a standalone sort (the generation loop
+// clears them inside k_design<OFFSPRING>).
+__global__ void k_zero_np(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    for (int i = threadIdx.x; i < J.P; i += blockDim.x) J.np[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    switch (mode) {
+    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items); break;
+    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items); break;
+    default: hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items); break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items);
+    return hipGetLastError();
+}
+hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_fronts, dim3(njobs), dim3(1024), 0, s, jobs);
+    return hipGetLastError();
+}
+hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_associate, dim3(n), dim3(256), 0, s, jobs, items);
+    return hipGetLastError();
+}
+hipError_t launch_niche(const DevJob *jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(1024), 0, s, jobs);
+    return hipGetLastError();
+}
+hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_state, dim3(njobs), dim3(256), 0, s, jobs);
+    return hipGetLastError();
+}
+hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_np, dim3(njobs), dim3(256), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t set_design_lds_limit(size_t bytes) {
+    hipError_t e = hipSuccess;
+    e = hipFuncSetAttribute((const void *)k_design<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_design<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_design<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace cd

@@ -1,0 +1,85 @@
+// cd_layout.h -- memory layout of the codon-design engine's device state,
+// shared by the kernels and the host API.
+#pragma once
+
+#include <stdint.h>
+
+namespace cd {
+
+constexpr int NOBJ = 6;                  // common.cuh:23
+constexpr int NCODON = 64;               // common.cuh:17
+constexpr int NSENSE = 61;               // TOTAL_CODON_NUM - STOP_CODON_NUM
+constexpr int STOP_AA = 20;              // 'Z'
+constexpr int MIN_HAIRPIN = 4;           // common.cuh:34
+constexpr float F_PRECISION = 0.000001f; // common.cuh:53
+constexpr int GC_UP = 1, GC_DOWN = -1;   // common.cuh:42-43
+constexpr int MAX_M = 16;                // CDS copies per design supported
+constexpr int MAX_W = 256;               // dominance words per row: pools up to 16384
+
+// Philox stream tags: counter = (generation, design | child | ref, slot, tag)
+constexpr uint32_t TAG_INIT = 0x494E4954u;
+constexpr uint32_t TAG_TYPE = 0x54595045u;
+constexpr uint32_t TAG_MUT = 0x4D555441u;
+constexpr uint32_t TAG_MUTB = 0x4D555442u;
+constexpr uint32_t TAG_SL = 0x534C5452u;
+constexpr uint32_t TAG_NICHE = 0x4E494348u;
+
+// Per-CDS statistics kept with every design so that a child recomputes only
+// the CDS its operator changed.
+struct CdsStat {
+    int64_t cai;  // sum of 2^40-scaled log weights
+    int64_t cpb;  // sum of 2^40-scaled pair scores over pairs k,k+1, k < L-2
+    int32_t hsc;  // hidden stop codons over junctions k|k+1, k < L-1
+    int32_t gc3;  // codons whose third base is G or C
+    int32_t l, p, q;  // longest stem and its position (canonical tie rule)
+    int32_t pad;
+};
+
+// Selection state of one job (the reference keeps it in __device__ globals,
+// common.cuh:46-59).
+struct SelState {
+    uint32_t gen;      // generations completed
+    int32_t n_cut;     // fronts filled N exactly (no niching)
+    int32_t hyp;       // hyperplane exception (MNDF nadir)
+    int32_t ranked;    // designs ranked
+    int32_t last;      // index of the last (cut) front
+    int32_t base;      // designs in fronts before the last
+    uint32_t pad;
+    unsigned long long cds_scanned;  // CDS whose objectives were recomputed (cumulative)
+    float ext[NOBJ * NOBJ];
+    float ideal[NOBJ], worst[NOBJ], nadir[NOBJ];
+};
+
+// Device view of one job.  Pools ping-pong on generation parity: generation
+// g reads buffer g&1 and writes buffer (g+1)&1.
+struct DevJob {
+    int32_t L, m, N, P, R, W, den_sl, pad0;
+    uint64_t seed, thr_pm, thr_2pm;
+    float gc3_ref, pad1;
+    const int8_t *aa;      // [L]
+    const uint8_t *nt;     // [64*3] nucleotide codes A=0 C=1 G=2 U=3
+    const int64_t *lw;     // [64]
+    const int64_t *cpfx;   // [61*61]
+    const float *cps;      // [61*61]
+    const uint8_t *hs;     // [64*64] hidden stops across junction a|b
+    const uint8_t *nd;     // [64*64] nucleotide mismatches
+    const uint8_t *gc3;    // [64]
+    const float *refpts;   // [R*6]
+    int8_t *genome[2];     // [P][m][L]
+    float *obj[2];         // [P][6]
+    int8_t *oidx[2];       // [P][12]
+    int32_t *pql[2];       // [P][4] (p, q, l, 0)
+    CdsStat *cst[2];       // [P][m]
+    int32_t *sorted;       // [N]
+    int32_t *front_of;     // [P]
+    int32_t *np;           // [P] dominator counts
+    uint64_t *dom;         // [P][W]: bit i of row j set when i dominates j
+    int32_t *ref_of;       // [P]
+    float *dist;           // [P]
+    int32_t *bucket;       // [P]
+    int32_t *niche;        // [6][R]: count, rem, off, fill, kpick, spare
+    uint8_t *flag;         // [P]
+    SelState *st;
+};
+
+}  // namespace cd
