@@ -109,8 +109,10 @@ int cd_job_read(cd_job *job, int8_t *genome, float *obj, int8_t *idx, int32_t *p
 
 /* Run `generations` generations launching every kernel separately and
  * report the mean duration in milliseconds of each kernel, measured with HIP
- * events: kernel_ms[0] offspring (mutate+evaluate), [1] dominance matrix,
- * [2] fronts + normalisation, [3] association, [4] niching. */
+ * events on the job's stream: kernel_ms[0] offspring (survivor copy,
+ * mutation, all six objectives), [1] dominance matrix, [2] fronts +
+ * normalisation, [3] association, [4] niching. */
+#define CD_PROFILE_KERNELS 5
 int cd_job_profile(cd_job *job, int32_t generations, float *kernel_ms);
 
 /* Number of CDS whose objectives were recomputed since cd_job_init (children
@@ -143,6 +145,17 @@ int cd_batch_step(cd_batch *batch, int32_t generations);
 int cd_batch_step_timed(cd_batch *batch, int32_t generations, float *ms);
 int cd_batch_sync(cd_batch *batch);
 int cd_batch_destroy(cd_batch *batch);
+
+/* Quality indicators on the host (replace the reference's prebuilt `hv`
+ * program, main.cu:839-858, and MinEuclid, utils.cuh:42-58).
+ * cd_hypervolume: exact hypervolume (minimisation) of n points in d
+ * dimensions against ref (WFG algorithm).
+ * cd_normalize: main.cu:789-835's normalisation of raw records [n][6].
+ * cd_min_distance_spread: minimum distance to the origin and maximum spread
+ * of n normalised points [n][6]. */
+int cd_hypervolume(const double *pts, int32_t n, int32_t d, const double *ref, double *out);
+int cd_normalize(const float *obj, int32_t n, float *out);
+int cd_min_distance_spread(const float *pts, int32_t n, float *min_dist, float *spread);
 
 /* Device bytes a job with these sizes holds (for admission by HBM budget). */
 int64_t cd_job_device_bytes(int32_t L, int32_t m, int32_t N, int32_t H);

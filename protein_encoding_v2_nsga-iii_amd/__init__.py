@@ -20,13 +20,15 @@ LIB_PATH = os.path.join(HERE, "libcd_engine.so")
 CSRC = os.path.join(HERE, "csrc")
 
 CD_OK = 0
+# kernels timed by cd_job_profile: offspring, dominance, fronts, associate, niche
+PROFILE_KERNELS = 5
 ERRORS = {-1: "CD_E_INVALID", -2: "CD_E_NOMEM", -3: "CD_E_DEVICE", -4: "CD_E_STATE", -5: "CD_E_BUDGET"}
 
 # Every entry point of include/codon_nsga3.h
 SYMBOLS = ("cd_job_create", "cd_job_init", "cd_job_step", "cd_job_step_timed", "cd_job_sync", "cd_job_generation",
            "cd_job_read", "cd_job_profile", "cd_job_cds_scanned", "cd_job_destroy", "cd_eval_batch", "cd_nds", "cd_select",
            "cd_batch_create", "cd_batch_step", "cd_batch_step_timed", "cd_batch_sync", "cd_batch_destroy",
-           "cd_job_device_bytes", "cd_last_error")
+           "cd_job_device_bytes", "cd_last_error", "cd_hypervolume", "cd_normalize", "cd_min_distance_spread")
 
 
 class CdError(RuntimeError):
@@ -88,6 +90,10 @@ def lib():
         L.cd_job_device_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.cd_job_device_bytes.restype = C.c_int64
         L.cd_last_error.restype = C.c_char_p
+        f64 = P(np.float64, flags="C_CONTIGUOUS")
+        L.cd_hypervolume.argtypes = [f64, C.c_int32, C.c_int32, f64, C.POINTER(C.c_double)]
+        L.cd_normalize.argtypes = [f32, C.c_int32, f32]
+        L.cd_min_distance_spread.argtypes = [f32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -144,7 +150,7 @@ class Job:
         return g.value
 
     def profile(self, generations):
-        out = np.zeros(5, np.float32)
+        out = np.zeros(PROFILE_KERNELS, np.float32)
         _check(lib().cd_job_profile(self.h, generations, out))
         return out
 
@@ -237,3 +243,30 @@ def select(obj, N, H, gen, seed, extremes):
     _check(lib().cd_select(obj, N, H, gen, seed, ext, srt, fo, norm, flags))
     return dict(sorted=srt, front_of=fo, extremes=ext.reshape(6, 6), ideal=norm[:6], worst=norm[6:12],
                 nadir=norm[12:], n_cut=int(flags[0]), hyp_exception=int(flags[1]))
+
+
+def hypervolume(points, ref=None):
+    """Exact hypervolume (minimisation) of points [n][d] against ref (default all ones)."""
+    pts = np.ascontiguousarray(points, np.float64)
+    n, d = pts.shape
+    r = np.ones(d) if ref is None else np.ascontiguousarray(ref, np.float64)
+    out = C.c_double()
+    _check(lib().cd_hypervolume(pts, n, d, r, C.byref(out)))
+    return out.value
+
+
+def normalize(obj):
+    """main.cu:789-835 normalisation of raw objective records [n][6]."""
+    obj = np.ascontiguousarray(obj, np.float32)
+    out = np.zeros_like(obj)
+    _check(lib().cd_normalize(obj, len(obj), out))
+    return out
+
+
+def quality(obj):
+    """(hypervolume, min distance to ideal, spread) of a pool of raw records,
+    as the reference reports them (normalised, reference point 1^6)."""
+    z = normalize(obj)
+    md, sp = C.c_float(), C.c_float()
+    _check(lib().cd_min_distance_spread(z, len(z), C.byref(md), C.byref(sp)))
+    return hypervolume(z.astype(np.float64)), md.value, sp.value

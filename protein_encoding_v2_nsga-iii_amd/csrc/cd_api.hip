@@ -353,9 +353,31 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     return CD_OK;
 }
 
-hipError_t enqueue_generation(const cd::DevJob *jobs, int njobs, const int2 *off, int n_off, const int4 *dom, int n_dom,
-                              const int2 *asc, int n_asc, size_t lds, hipStream_t s) {
-    hipError_t e = cd::launch_design(cd::MODE_OFFSPRING, jobs, off, n_off, lds, s);
+struct Launch {
+    const cd::DevJob *jobs;
+    int njobs;
+    const int2 *off;
+    int n_off;
+    const int4 *dom;
+    int n_dom;
+    const int2 *asc;
+    int n_asc;
+    size_t lds;
+};
+
+// Evaluate the designs of `items` (design kernel in `mode`).
+hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
+    return cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
+}
+
+hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
+    hipError_t e = enqueue_designs(w, cd::MODE_OFFSPRING, w.off, w.n_off, s);
+    const cd::DevJob *jobs = w.jobs;
+    const int njobs = w.njobs;
+    const int4 *dom = w.dom;
+    const int n_dom = w.n_dom;
+    const int2 *asc = w.asc;
+    const int n_asc = w.n_asc;
     if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, s);
     if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
     if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
@@ -363,10 +385,9 @@ hipError_t enqueue_generation(const cd::DevJob *jobs, int njobs, const int2 *off
     return e;
 }
 
-int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const cd::DevJob *jobs, int njobs, const int2 *off,
-            int n_off, const int4 *dom, int n_dom, const int2 *asc, int n_asc, size_t lds) {
+int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_generation(jobs, njobs, off, n_off, dom, n_dom, asc, n_asc, lds, s);
+    hipError_t e = enqueue_generation(w, s);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &graph);
     if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
@@ -374,6 +395,10 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const cd::DevJob *
     HIPCHK(hipGraphInstantiate(ge, graph, nullptr, nullptr, 0));
     *g = graph;
     return CD_OK;
+}
+
+Launch job_launch(cd_job *J) {
+    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds};
 }
 
 }  // namespace
@@ -400,7 +425,7 @@ int cd_job_create(const cd_job_desc *desc, cd_job **out) {
 int cd_job_init(cd_job *J) {
     if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
     // the offspring work list doubles as the (job, design) list of the init launch
-    HIPCHK(cd::launch_design(cd::MODE_INIT, J->d_self, J->d_off, J->N, J->lds, J->stream));
+    HIPCHK(enqueue_designs(job_launch(J), cd::MODE_INIT, J->d_off, J->N, J->stream));
     HIPCHK(cd::launch_init_state(J->d_self, 1, J->stream));
     J->inited = true;
     return CD_OK;
@@ -411,8 +436,7 @@ int cd_job_step(cd_job *J, int32_t generations) {
     if (!J->inited) return fail(CD_E_STATE, "cd_job_step before cd_job_init");
     if (generations < 0) return fail(CD_E_INVALID, "negative generation count");
     if (!J->gexec) {
-        int rc = capture(J->stream, &J->graph, &J->gexec, J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc,
-                         J->n_asc, J->lds);
+        int rc = capture(J->stream, &J->graph, &J->gexec, job_launch(J));
         if (rc != CD_OK) return rc;
     }
     for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(J->gexec, J->stream));
@@ -483,14 +507,16 @@ int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql
 int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
     if (!J || J->selection_only || !kernel_ms) return fail(CD_E_INVALID, "null argument");
     if (!J->inited) return fail(CD_E_STATE, "cd_job_profile before cd_job_init");
-    hipEvent_t ev[6];
+    const int NK = CD_PROFILE_KERNELS;
+    hipEvent_t ev[NK + 1];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-    double acc[5] = {0, 0, 0, 0, 0};
+    double acc[NK] = {0};
+    const Launch w = job_launch(J);
     for (int g = 0; g < generations; ++g) {
         HIPCHK(hipEventRecord(ev[0], J->stream));
         HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream));
         HIPCHK(hipEventRecord(ev[1], J->stream));
-        HIPCHK(cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, J->stream));
+        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
         HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));
@@ -499,13 +525,13 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(cd::launch_niche(J->d_self, 1, J->stream));
         HIPCHK(hipEventRecord(ev[5], J->stream));
         HIPCHK(hipEventSynchronize(ev[5]));
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < NK; ++k) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
             acc[k] += t;
         }
     }
-    for (int k = 0; k < 5; ++k) kernel_ms[k] = generations > 0 ? (float)(acc[k] / generations) : 0.f;
+    for (int k = 0; k < NK; ++k) kernel_ms[k] = generations > 0 ? (float)(acc[k] / generations) : 0.f;
     for (auto &e : ev) HIPCHK(hipEventDestroy(e));
     return CD_OK;
 }
@@ -549,7 +575,7 @@ int cd_eval_batch(const cd_job_desc *desc, const int8_t *genome, int32_t count, 
     hipError_t e = hipMalloc(&d_items, sizeof(int2) * count);
     if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), sizeof(int2) * count, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(J->h.genome[0], genome, G * count, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = cd::launch_design(cd::MODE_EVAL, J->d_self, d_items, count, J->lds, J->stream);
+    if (e == hipSuccess) e = enqueue_designs(job_launch(J), cd::MODE_EVAL, d_items, count, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
     if (e == hipSuccess && obj) e = hipMemcpy(obj, J->h.obj[0], (size_t)count * 24, hipMemcpyDeviceToHost);
     if (e == hipSuccess && idx) e = hipMemcpy(idx, J->h.oidx[0], (size_t)count * 12, hipMemcpyDeviceToHost);
@@ -687,8 +713,9 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
 int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
-        int rc = capture(B->stream, &B->graph, &B->gexec, B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom,
-                         B->n_dom, B->d_asc, B->n_asc, B->lds);
+        const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom, B->n_dom, B->d_asc, B->n_asc,
+                       B->lds};
+        int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }
     for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(B->gexec, B->stream));

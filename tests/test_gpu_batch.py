@@ -1,0 +1,61 @@
+"""Batched form: uneven jobs advanced together by one launch sequence give
+exactly the pools each job gives alone (and the oracle gives)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_jobs(cdp, specs):
+    T = cdp.tables
+    out = []
+    for i, (res, m, N, H, seed) in enumerate(specs):
+        aa = T.synth_protein(res, seed)
+        tab = T.synth_tables(seed)
+        out.append((aa, tab, m, N, H, 0.1, seed))
+    return out
+
+
+SPECS = [(50, 2, 32, 2, 11), (120, 4, 64, 3, 12), (37, 8, 16, 1, 13), (300, 2, 128, 4, 14), (75, 3, 48, 2, 15)]
+
+
+def test_batch_equals_alone(cdp, orc):
+    print("batch", flush=True)
+    specs = make_jobs(cdp, SPECS)
+    G = 7
+    alone = []
+    for s in specs:
+        j = cdp.Job(*s).init()
+        j.step(G)
+        alone.append(j.read())
+        j.close()
+    jobs = [cdp.Job(*s).init() for s in specs]
+    b = cdp.Batch(jobs)
+    b.step(3)
+    b.step(G - 3)
+    b.sync()
+    for j, a, s in zip(jobs, alone, specs):
+        r = j.read()
+        for k in ("genome", "obj", "idx", "pql", "sorted", "front_of", "extremes", "nadir"):
+            assert np.array_equal(r[k], a[k]), k
+    # and the oracle agrees with one of them
+    aa, tab, m, N, H, Pm, seed = specs[1]
+    o = orc.Job(aa, tab, m, N, H, Pm, seed)
+    o.init()
+    o.step(G)
+    ro = o.read()
+    for k in ("genome", "obj", "sorted", "front_of"):
+        assert np.array_equal(jobs[1].read()[k], ro[k]), k
+    b.close()
+    for j in jobs:
+        j.close()
+
+
+def test_batch_rejects_uninitialised(cdp):
+    specs = make_jobs(cdp, SPECS[:2])
+    j0 = cdp.Job(*specs[0]).init()
+    j1 = cdp.Job(*specs[1])
+    with pytest.raises(cdp.CdError):
+        cdp.Batch([j0, j1])
+    with pytest.raises(cdp.CdError):
+        cdp.Batch([j0, j0])
