@@ -17,13 +17,18 @@
 
 #include "cd_device.h"
 #include "cd_kernels.h"
+#include "cd_scan.h"
 
 namespace cd {
 
 __host__ __device__ constexpr int align16(int x) { return (x + 15) & ~15; }
 
+// dynamic LDS of k_design: codons [m*L], amino acids [L], nucleotide codes
+// [3L], CDS statistics [m], bit planes of the sequence and its reverse
+// complement [2][scan_words(3L)]
 size_t design_lds_bytes(int L, int m) {
-    return (size_t)align16(m * L) + align16(L) + align16(3 * L) + (size_t)m * sizeof(CdsStat) + 16;
+    return (size_t)align16(m * L) + align16(L) + align16(3 * L) + align16(m * (int)sizeof(CdsStat)) +
+           (size_t)2 * scan_words(3 * L) * sizeof(uint2) + 16;
 }
 
 struct DesignShared {
@@ -38,42 +43,54 @@ struct DesignShared {
 };
 
 // ---------------------------------------------------------------------------
-// Stem scan of one CDS held as nucleotide codes in LDS: per anti-diagonal
-// i + j = S, walk i upwards and count consecutive complementary pairs with
-// j - i > MIN_HAIRPIN (calMaximumSL, utils.cuh:712-888).  Returns the best
-// stem key of this thread.
+// Longest stem of one CDS held as nucleotide codes in LDS (calMaximumSL,
+// utils.cuh:712-888): the sequence and its reverse complement are packed into
+// bit planes with wave ballots, then each thread scans anti-diagonals
+// S = 5 + tid, 5 + tid + blockDim, ... bit-parallel (cd_scan.h) and the
+// workgroup takes the maximum key (longest, then smallest p, then smallest q).
+// seed_from, when given, is the parent's record of this CDS: the length of
+// the parent's stem as it stands in the child is a stem the child has, so the
+// scan starts with that window.
 // ---------------------------------------------------------------------------
-__device__ uint64_t sl_scan(const uint8_t *s, int n) {
-    uint64_t best = 0;
-    int bl = 0;
-    for (int S = MIN_HAIRPIN + 1 + (int)threadIdx.x; S <= 2 * n - 3; S += blockDim.x) {
-        const int i0 = max(0, S - (n - 1));
-        const int i1 = (S - (MIN_HAIRPIN + 1)) >> 1;
-        int run = 0;
-        for (int i = i0; i <= i1; ++i) {
-            if ((s[i] ^ s[S - i]) == 3) {
-                ++run;
-                if (run >= bl) {
-                    const uint64_t k = stem_key(run, i - run + 1, S - i);
-                    if (k > best) {
-                        best = k;
-                        bl = run;
-                    }
-                }
-            } else {
-                run = 0;
-            }
+__device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p, int q) {
+    if (l <= 0) return 0;
+    const int i = p + l - 1, j = q;
+    int k = 0;
+    while (i - k >= 0 && j + k < n && ((ntb[i - k] ^ ntb[j + k]) == 3)) ++k;
+    return k;
+}
+
+__device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, DesignShared &sh) {
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+    const int W = scan_words(n);
+    uint2 *sp = planes, *rp = planes + W;
+    for (int p0 = wid * 64; p0 < W * 32; p0 += nw * 64) {  // W is even: words w, w+1 < W
+        const int pos = p0 + lane;
+        const int c = pos < n ? ntb[pos] : 0;
+        const int rc = pos < n ? (ntb[n - 1 - pos] ^ 3) : 0;
+        const uint64_t bl = __ballot(c & 1), bh = __ballot(c >> 1), rl = __ballot(rc & 1), rh = __ballot(rc >> 1);
+        if (lane == 0) {
+            const int w = p0 >> 5;
+            sp[w] = make_uint2((uint32_t)bl, (uint32_t)bh);
+            sp[w + 1] = make_uint2((uint32_t)(bl >> 32), (uint32_t)(bh >> 32));
+            rp[w] = make_uint2((uint32_t)rl, (uint32_t)rh);
+            rp[w + 1] = make_uint2((uint32_t)(rl >> 32), (uint32_t)(rh >> 32));
         }
     }
-    return best;
+    if (tid == 0) sh.misc[0] = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q) : 0;
+    __syncthreads();
+    int K = max(sh.misc[0], 1);
+    uint64_t best = 0;
+    for (int S = MIN_HAIRPIN + 1 + tid; S <= 2 * n - 3; S += NT) scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
+    return block_max_u64(best, sh.redu);
 }
 
 // ---------------------------------------------------------------------------
 // Evaluation of the CDS flagged dirty, then the six objectives
 // (calMinimumCAI/CBP/HSC/HD, calMaximumGC3/SL, utils.cuh:271-888).
 // ---------------------------------------------------------------------------
-__device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, CdsStat *cst, DesignShared &sh,
-                         float *obj, int32_t *oidx, int32_t *opql) {
+__device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, uint2 *planes, CdsStat *cst,
+                         bool have_parent, DesignShared &sh, float *obj, int32_t *oidx, int32_t *opql) {
     const int L = J.L, m = J.m, n = 3 * L;
     const int tid = threadIdx.x, NT = blockDim.x;
     if (tid == 0) {
@@ -99,8 +116,7 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, CdsS
             ntb[3 * k + 2] = J.nt[3 * a + 2];
         }
         block_sum_i64<4>(v, sh.red);  // also publishes ntb
-        uint64_t key = sl_scan(ntb, n);
-        key = block_max_u64(key, sh.redu);
+        const uint64_t key = stem_cds(ntb, n, planes, have_parent ? &cst[c] : nullptr, sh);
         if (tid == 0) {
             int l, p, q;
             stem_unkey(key, l, p, q);
@@ -532,6 +548,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     int8_t *aa = (int8_t *)(dyn + align16(mL));
     uint8_t *ntb = (uint8_t *)aa + align16(L);
     CdsStat *cst = (CdsStat *)(ntb + align16(3 * L));
+    uint2 *planes = (uint2 *)((uint8_t *)cst + align16(m * (int)sizeof(CdsStat)));
 
     uint32_t gen = 0;
     int dst = 0, out_slot = it.y;
@@ -617,7 +634,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
     __shared__ float s_obj[NOBJ];
     __shared__ int32_t s_idx[2 * NOBJ], s_pql[4];
-    evaluate(J, cod, ntb, cst, sh, s_obj, s_idx, s_pql);
+    evaluate(J, cod, ntb, planes, cst, MODE == 2, sh, s_obj, s_idx, s_pql);
     int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
     for (int t = tid; t < mL; t += NT) og[t] = (int8_t)(cod[t] - c_syn_start[aa[t % L]]);
     if (tid < NOBJ) oobj[tid] = s_obj[tid];

@@ -1,0 +1,130 @@
+// cd_scan.h -- longest hairpin stem of one CDS, bit-parallel along each
+// anti-diagonal (calMaximumSL, utils.cuh:712-888).
+//
+// A stem with innermost pair (i, j), j - i > MIN_HAIRPIN, is a run of
+// consecutive Watson-Crick pairs (i, j), (i-1, j+1), ...  All pairs of one
+// run share the anti-diagonal S = i + j.  With rc the reverse complement of
+// the sequence (rc[t] = comp(s[n-1-t])), pair (i, S-i) is complementary
+// exactly when s[i] == rc[i + n-1-S], so along anti-diagonal S the match bits
+// of 32 consecutive positions i are the equality bits of two 32-base windows,
+// one of s and one of rc, both read forwards.  Sequences are held as two bit
+// planes (low and high bit of the 2-bit code, A=0 C=1 G=2 U=3), so a 32-cell
+// chunk costs four funnel shifts and three logic operations.
+//
+// Runs are found without walking cells: the doubling chain A2 = M & M<<1,
+// A4 = A2 & A2<<2, ... marks the positions that end a window of k matches,
+// a window of K (the longest stem this lane knows, at least the seed) is
+// assembled from it, and only the run ends carrying such a window are
+// measured exactly.  Every run of length >= K is measured, and K never
+// exceeds the longest stem of the CDS, so the maximum over all lanes is exact
+// including the tie rule (longest, then smallest p, then smallest q).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cd {
+
+// Words of one bit plane for n bases, with two zero words of padding (reads
+// run one word past the last base), rounded to an even count so that
+// 64-position slabs of the plane build never leave the plane.
+__host__ __device__ inline int scan_words(int n) { return (((n + 31) >> 5) + 2 + 1) & ~1; }
+
+// ({hi, lo} >> sh) truncated to 32 bits, 0 <= sh <= 31 (v_alignbit_b32).
+__host__ __device__ inline uint32_t scan_funnel(uint32_t hi, uint32_t lo, int sh) {
+    return (uint32_t)(((((uint64_t)hi) << 32) | lo) >> sh);
+}
+
+// Stream X delayed by `off` positions (0 <= off <= 31): bit k of the result is
+// X at position k - off, where x is X's current chunk and xp its previous one.
+__host__ __device__ inline uint32_t scan_back(uint32_t x, uint32_t xp, int off) {
+    return off == 0 ? x : scan_funnel(x, xp, 32 - off);
+}
+
+__host__ __device__ inline uint64_t scan_key(int l, int p, int q) {
+    return ((uint64_t)l << 40) | ((uint64_t)(0xFFFFF - p) << 20) | (uint64_t)(0xFFFFF - q);
+}
+
+// Scan anti-diagonal S of a sequence of n bases whose planes are sp (sequence,
+// .x low bits, .y high bits) and rp (reverse complement).  K is the lane's
+// window (at least 1; raised when a longer stem is found), best its best
+// key.  Pairs (i, S-i) with j - i > min_hairpin are valid.
+__host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, int n, int S, int min_hairpin, int &K,
+                                              uint64_t &best) {
+    const int i0 = S > n - 1 ? S - (n - 1) : 0;
+    const int i1 = (S - (min_hairpin + 1)) >> 1;
+    if (i1 < i0) return;
+    const int len = i1 - i0 + 1;
+    const int t0 = i0 + (n - 1 - S);
+    const int nch = (len + 31) >> 5;
+    const int ss = i0 & 31, st = t0 & 31;
+    int ws = i0 >> 5, wt = t0 >> 5;
+    uint2 slo = sp[ws], rlo = rp[wt];
+    uint32_t Mp = 0, A2p = 0, A4p = 0, A8p = 0, Wp = 0;
+    int carry = 0;  // matches ending at the last position before the previous chunk
+    for (int c = 0; c <= nch; ++c) {
+        uint32_t M = 0;
+        if (c < nch) {
+            const uint2 shi = sp[ws + 1], rhi = rp[wt + 1];
+            const uint32_t xl = scan_funnel(shi.x, slo.x, ss) ^ scan_funnel(rhi.x, rlo.x, st);
+            const uint32_t xh = scan_funnel(shi.y, slo.y, ss) ^ scan_funnel(rhi.y, rlo.y, st);
+            M = ~(xl | xh);
+            const int rem = len - 32 * c;
+            if (rem < 32) M &= (1u << rem) - 1u;
+            slo = shi;
+            rlo = rhi;
+            ++ws;
+            ++wt;
+        }
+        if (c > 0) {
+            // run ends of the previous chunk: a window of K ends there and the
+            // next position does not match
+            uint32_t E = Wp & ~scan_funnel(M, Mp, 1);
+            while (E) {
+                const int k = __builtin_ctz(E);
+                E &= E - 1u;
+                const uint32_t z = ~Mp & ((2u << k) - 1u);
+                const int l = z ? k - (31 - __builtin_clz(z)) : k + 1 + carry;
+                const int ie = i0 + 32 * (c - 1) + k;
+                const uint64_t key = scan_key(l, ie - l + 1, S - ie);
+                if (key > best) best = key;
+                if (l > K) K = l;
+            }
+            carry = (Mp == ~0u) ? carry + 32 : __builtin_clz(~Mp);
+        }
+        const uint32_t A2 = M & scan_funnel(M, Mp, 31);
+        const uint32_t A4 = A2 & scan_funnel(A2, A2p, 30);
+        const uint32_t A8 = A4 & scan_funnel(A4, A4p, 28);
+        // window of Kw = min(K, 16) matches ending at each position
+        const int Kw = K < 16 ? K : 16;
+        uint32_t W;
+        int off;
+        if (Kw == 16) {
+            W = A8 & scan_funnel(A8, A8p, 24);
+            off = 16;
+        } else {
+            W = ~0u;
+            off = 0;
+        }
+        if (Kw & 8) {
+            W &= scan_back(A8, A8p, off);
+            off += 8;
+        }
+        if (Kw & 4) {
+            W &= scan_back(A4, A4p, off);
+            off += 4;
+        }
+        if (Kw & 2) {
+            W &= scan_back(A2, A2p, off);
+            off += 2;
+        }
+        if (Kw & 1) W &= scan_back(M, Mp, off);
+        Wp = W;
+        Mp = M;
+        A2p = A2;
+        A4p = A4;
+        A8p = A8;
+    }
+}
+
+}  // namespace cd

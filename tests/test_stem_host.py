@@ -1,8 +1,7 @@
-"""The engine's bit-sliced stem scan (csrc/cd_stem.h), compiled for the host
-and run lane by lane, against the oracle's closed form (with its tie rule) on
-random, repetitive, skewed and planted-hairpin sequences.  Stems shorter
-than the smallest window (4) are out of the bit-sliced pass's scope: the
-kernel rescans those CDS exactly, which the GPU parity tests cover."""
+"""The engine's stem scan (csrc/cd_scan.h), compiled for the host with the
+address sanitizer and run lane by lane, against the oracle's closed form
+(with its tie rule) on random, repetitive, skewed and planted-hairpin
+sequences, with and without a seed (a stem length known to exist)."""
 import os
 import shutil
 import subprocess
@@ -21,8 +20,9 @@ def harness(tmp_path_factory):
     if not os.path.exists(HIPCC):
         pytest.skip("hipcc not available")
     out = str(tmp_path_factory.mktemp("stem") / "stem_host_check")
-    subprocess.run([HIPCC, "-O2", "-std=c++17", "--offload-host-only", "-x", "hip", "-I", CSRC,
-                    os.path.join(HERE, "stem_host_check.cpp"), "-o", out], check=True, capture_output=True)
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "--offload-host-only", "-x", "hip", "-Xarch_host",
+                    "-fsanitize=address", "-I", CSRC, os.path.join(HERE, "stem_host_check.cpp"), "-o", out],
+                   check=True, capture_output=True)
     return out
 
 
@@ -30,36 +30,56 @@ def sequences(seed, count, nmax):
     rng = np.random.default_rng(seed)
     out = []
     for t in range(count):
-        n = int(rng.integers(6, nmax))
-        kind = t % 4
+        n = int(rng.integers(1, nmax))
+        kind = t % 5
         if kind == 0:
             s = rng.integers(0, 4, n)
-        elif kind == 1:
+        elif kind == 1:  # short repeats with a few substitutions: long runs
             s = np.resize(rng.integers(0, 4, int(rng.integers(1, 7))), n)
             f = rng.random(n) < 0.03
             s[f] = rng.integers(0, 4, int(f.sum()))
-        elif kind == 2:
+        elif kind == 2:  # planted hairpin of 5..60 pairs
             s = rng.integers(0, 4, n)
             k = int(rng.integers(5, max(6, min(60, n // 3))))
             i = int(rng.integers(0, max(1, n - 2 * k - 8)))
             if i + 2 * k + 5 <= n:
                 s[i + k + 5:i + 2 * k + 5] = [COMP[x] for x in s[i:i + k][::-1]]
-        else:
+        elif kind == 3:  # A/U rich: many equal-length stems, exercises the tie rule
             s = rng.choice(4, size=n, p=[0.45, 0.05, 0.05, 0.45])
+        else:  # perfect palindrome: one stem spanning the sequence
+            h = rng.integers(0, 4, max(1, n // 2))
+            s = np.concatenate([h, rng.integers(0, 4, n % 2 + 6), [COMP[x] for x in h[::-1]]])
         out.append(np.asarray(s, np.uint8))
     return out
 
 
-@pytest.mark.parametrize("lanes,seed_t", [(256, 0), (1, 0), (7, 6), (1024, 0)])
-def test_bit_sliced_stem_matches_oracle(orc, harness, lanes, seed_t):
-    seqs = sequences(lanes + seed_t, 160, 1300)
-    inp = "\n".join("".join(map(str, s)) for s in seqs) + "\n"
-    out = subprocess.run([harness, str(lanes), str(seed_t)], input=inp, capture_output=True, text=True,
-                         check=True).stdout.split("\n")
-    checked = 0
-    for s, o in zip(seqs, out):
-        want = orc.sl(s, 0)
-        if want[0] >= max(4, seed_t):
-            assert tuple(map(int, o.split())) == want, len(s)
-            checked += 1
-    assert checked > 100
+def run(harness, seqs, seeds, lanes):
+    inp = "".join("".join(map(str, s)) + f" {k}\n" for s, k in zip(seqs, seeds))
+    out = subprocess.run([harness, str(lanes)], input=inp, capture_output=True, text=True, check=True).stdout
+    return [tuple(map(int, o.split())) for o in out.strip().split("\n")]
+
+
+@pytest.mark.parametrize("lanes", [256, 1, 7, 64, 1024])
+def test_stem_scan_matches_oracle(orc, harness, lanes):
+    seqs = [s for s in sequences(lanes, 200, 1600) if len(s) > 0]
+    want = [orc.sl(s, 0) for s in seqs]
+    rng = np.random.default_rng(lanes + 7)
+    for seeds in ([0] * len(seqs),
+                  [w[0] for w in want],                                     # the exact length
+                  [int(rng.integers(0, w[0] + 1)) for w in want]):          # any lower bound
+        got = run(harness, seqs, seeds, lanes)
+        for s, g, w, k in zip(seqs, got, want, seeds):
+            assert g == w, (len(s), k)
+
+
+def test_stem_scan_edge_lengths(orc, harness):
+    # every length around the 32-position chunk boundaries and the hairpin limit
+    rng = np.random.default_rng(3)
+    seqs = []
+    for n in list(range(1, 80)) + [95, 96, 97, 127, 128, 129, 1503, 4500]:
+        seqs.append(np.asarray(rng.integers(0, 4, n), np.uint8))
+        seqs.append(np.full(n, 0, np.uint8) if n % 2 else np.asarray([0, 3] * (n // 2), np.uint8))
+    want = [orc.sl(s, 0) for s in seqs]
+    got = run(harness, seqs, [0] * len(seqs), 256)
+    for s, g, w in zip(seqs, got, want):
+        assert g == w, len(s)
