@@ -110,9 +110,10 @@ int cd_job_read(cd_job *job, int8_t *genome, float *obj, int8_t *idx, int32_t *p
 /* Run `generations` generations launching every kernel separately and
  * report the mean duration in milliseconds of each kernel, measured with HIP
  * events on the job's stream: kernel_ms[0] offspring (survivor copy,
- * mutation, all six objectives), [1] dominance matrix, [2] fronts +
- * normalisation, [3] association, [4] niching. */
-#define CD_PROFILE_KERNELS 5
+ * mutation, objectives 0-4), [1] stem scan of the changed CDS, [2] stem
+ * objective, [3] dominance matrix, [4] fronts + normalisation,
+ * [5] association, [6] niching. */
+#define CD_PROFILE_KERNELS 7
 int cd_job_profile(cd_job *job, int32_t generations, float *kernel_ms);
 
 /* Number of CDS whose objectives were recomputed since cd_job_init (children

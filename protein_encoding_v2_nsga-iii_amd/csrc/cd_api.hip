@@ -156,7 +156,7 @@ struct cd_job {
     int4 *d_dom = nullptr;
     int2 *d_asc = nullptr;
     int n_dom = 0, n_asc = 0;
-    size_t lds = 0;
+    size_t lds = 0, stem_lds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool inited = false;
@@ -170,7 +170,8 @@ struct cd_batch {
     int4 *d_dom = nullptr;
     int2 *d_asc = nullptr;
     int n_off = 0, n_dom = 0, n_asc = 0;
-    size_t lds = 0;
+    size_t lds = 0, stem_lds = 0;
+    int mg = 0;  // largest m of the batch's jobs
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -245,6 +246,8 @@ int validate(const cd_job_desc *d) {
         if ((d->aa[k] == 20) != (k == d->L - 1)) return fail(CD_E_INVALID, "the stop (20) must be exactly the last codon");
     }
     if (3LL * d->L >= (1 << 20)) return fail(CD_E_INVALID, "sequence too long for stem positions");
+    if (cd::design_lds_bytes(d->L, d->m) > cd::LDS_BYTES || cd::stem_lds_bytes(d->L) > cd::LDS_BYTES)
+        return fail(CD_E_INVALID, "m*L too large for one workgroup's LDS (160 KiB)");
     return CD_OK;
 }
 
@@ -315,6 +318,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
         for (int c = 0; c < J->N; ++c) off[c] = make_int2(0, c);
         J->d_off = (int2 *)(base + o.off);
         J->lds = cd::design_lds_bytes(J->L, J->m);
+        J->stem_lds = cd::stem_lds_bytes(J->L);
     }
     auto ref = das_dennis(d->H);
     std::memcpy(&host[o.ref], ref.data(), ref.size() * sizeof(float));
@@ -344,6 +348,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     if (e == hipSuccess) e = hipMemcpy(J->mem, host.data(), o.total, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamCreate(&J->stream);
     if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
+    if (e == hipSuccess && J->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(J->stem_lds);
     if (e != hipSuccess) {
         (void)hipFree(J->mem);
         delete J;
@@ -362,12 +367,17 @@ struct Launch {
     int n_dom;
     const int2 *asc;
     int n_asc;
-    size_t lds;
+    size_t lds, stem_lds;
+    int mg;  // largest m
 };
 
-// Evaluate the designs of `items` (design kernel in `mode`).
+// Evaluate the designs of `items`: mutation (children) and objectives 0-4,
+// the stem of every changed CDS, objective 5.
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
-    return cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
+    hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
+    if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_lds, s);
+    if (e == hipSuccess) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
+    return e;
 }
 
 hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
@@ -398,7 +408,7 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 }
 
 Launch job_launch(cd_job *J) {
-    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds};
+    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds, J->stem_lds, J->m};
 }
 
 }  // namespace
@@ -516,15 +526,19 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(hipEventRecord(ev[0], J->stream));
         HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream));
         HIPCHK(hipEventRecord(ev[1], J->stream));
-        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, J->stream));
+        HIPCHK(cd::launch_stem(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->m, J->stem_lds, J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
-        HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
+        HIPCHK(cd::launch_stem_finish(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));
-        HIPCHK(cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream));
+        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, J->stream));
         HIPCHK(hipEventRecord(ev[4], J->stream));
-        HIPCHK(cd::launch_niche(J->d_self, 1, J->stream));
+        HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
         HIPCHK(hipEventRecord(ev[5], J->stream));
-        HIPCHK(hipEventSynchronize(ev[5]));
+        HIPCHK(cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream));
+        HIPCHK(hipEventRecord(ev[6], J->stream));
+        HIPCHK(cd::launch_niche(J->d_self, 1, J->stream));
+        HIPCHK(hipEventRecord(ev[7], J->stream));
+        HIPCHK(hipEventSynchronize(ev[7]));
         for (int k = 0; k < NK; ++k) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -688,6 +702,8 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
             for (int w = 0; w < J->W; ++w) dom.push_back(make_int4(j, t * 256, w, 0));
         }
         B->lds = std::max(B->lds, J->lds);
+        B->stem_lds = std::max(B->stem_lds, J->stem_lds);
+        B->mg = std::max(B->mg, J->m);
     }
     B->n_off = (int)off.size();
     B->n_dom = (int)dom.size();
@@ -702,6 +718,7 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     if (e == hipSuccess) e = hipMemcpy(B->d_asc, asc.data(), sizeof(int2) * asc.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamCreate(&B->stream);
     if (e == hipSuccess && B->lds > 48 * 1024) e = cd::set_design_lds_limit(B->lds);
+    if (e == hipSuccess && B->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(B->stem_lds);
     if (e != hipSuccess) {
         cd_batch_destroy(B);
         return fail(CD_E_DEVICE, std::string("batch setup: ") + hipGetErrorString(e));
@@ -714,7 +731,7 @@ int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
         const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom, B->n_dom, B->d_asc, B->n_asc,
-                       B->lds};
+                       B->lds, B->stem_lds, B->mg};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }

@@ -1,8 +1,11 @@
 // cd_kernels.hip -- HIP kernels of the codon-design engine for gfx950.
 //
-// One generation is five launches over every resident job:
+// One generation is seven launches over every resident job:
 //   k_design<OFFSPRING>  one workgroup per child: survivor copy, mutation,
-//                        evaluation (mutationKernel, main.cu:121-240)
+//                        objectives 0-4 (mutationKernel, main.cu:121-240)
+//   k_stem               one workgroup per (child, CDS): longest stem of each
+//                        CDS the mutation changed (calMaximumSL)
+//   k_stem_finish        one thread per child: objective 5
 //   k_dominance          bit-packed eps-domination matrix (sorting.cuh:994-1023)
 //   k_fronts             one workgroup per job: front peeling, ideal/worst,
 //                        extreme points, hyperplane/MNDF nadir
@@ -24,12 +27,14 @@ namespace cd {
 __host__ __device__ constexpr int align16(int x) { return (x + 15) & ~15; }
 
 // dynamic LDS of k_design: codons [m*L], amino acids [L], nucleotide codes
-// [3L], CDS statistics [m], bit planes of the sequence and its reverse
-// complement [2][scan_words(3L)]
+// [3L], CDS statistics [m]
 size_t design_lds_bytes(int L, int m) {
-    return (size_t)align16(m * L) + align16(L) + align16(3 * L) + align16(m * (int)sizeof(CdsStat)) +
-           (size_t)2 * scan_words(3 * L) * sizeof(uint2) + 16;
+    return (size_t)align16(m * L) + align16(L) + align16(3 * L) + align16(m * (int)sizeof(CdsStat)) + 16;
 }
+
+// dynamic LDS of k_stem: nucleotide codes [3L], bit planes of the sequence and
+// its reverse complement [2][scan_words(3L)]
+size_t stem_lds_bytes(int L) { return (size_t)align16(3 * L) + (size_t)2 * scan_words(3 * L) * sizeof(uint2); }
 
 struct DesignShared {
     float pobj[NOBJ];
@@ -60,7 +65,8 @@ __device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p
     return k;
 }
 
-__device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, DesignShared &sh) {
+__device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, int32_t *bcast,
+                             uint64_t *redu) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     const int W = scan_words(n);
     uint2 *sp = planes, *rp = planes + W;
@@ -77,20 +83,22 @@ __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const Cds
             rp[w + 1] = make_uint2((uint32_t)(rl >> 32), (uint32_t)(rh >> 32));
         }
     }
-    if (tid == 0) sh.misc[0] = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q) : 0;
+    if (tid == 0) *bcast = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q) : 0;
     __syncthreads();
-    int K = max(sh.misc[0], 1);
+    int K = max(*bcast, 1);
     uint64_t best = 0;
     for (int S = MIN_HAIRPIN + 1 + tid; S <= 2 * n - 3; S += NT) scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
-    return block_max_u64(best, sh.redu);
+    return block_max_u64(best, redu);
 }
 
 // ---------------------------------------------------------------------------
-// Evaluation of the CDS flagged dirty, then the six objectives
-// (calMinimumCAI/CBP/HSC/HD, calMaximumGC3/SL, utils.cuh:271-888).
+// Evaluation of the CDS flagged dirty, then objectives 0-4
+// (calMinimumCAI/CBP/HSC/HD, calMaximumGC3, utils.cuh:271-626).  A changed
+// CDS is marked for k_stem (pad = 1); its stem fields keep the parent's stem,
+// which seeds the scan, or are cleared when there is no parent.
 // ---------------------------------------------------------------------------
-__device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, uint2 *planes, CdsStat *cst,
-                         bool have_parent, DesignShared &sh, float *obj, int32_t *oidx, int32_t *opql) {
+__device__ void evaluate(const DevJob &J, const uint8_t *cod, CdsStat *cst, bool have_parent, DesignShared &sh,
+                         float *obj, int32_t *oidx) {
     const int L = J.L, m = J.m, n = 3 * L;
     const int tid = threadIdx.x, NT = blockDim.x;
     if (tid == 0) {
@@ -111,24 +119,16 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, uint
                 v[2] += J.hs[a * NCODON + b];
                 if (k < L - 2) v[1] += J.cpfx[a * NSENSE + b];
             }
-            ntb[3 * k] = J.nt[3 * a];
-            ntb[3 * k + 1] = J.nt[3 * a + 1];
-            ntb[3 * k + 2] = J.nt[3 * a + 2];
         }
-        block_sum_i64<4>(v, sh.red);  // also publishes ntb
-        const uint64_t key = stem_cds(ntb, n, planes, have_parent ? &cst[c] : nullptr, sh);
+        block_sum_i64<4>(v, sh.red);
         if (tid == 0) {
-            int l, p, q;
-            stem_unkey(key, l, p, q);
-            CdsStat s;
+            CdsStat s = cst[c];
+            if (!have_parent) s.l = s.p = s.q = 0;
             s.cai = v[0];
             s.cpb = v[1];
             s.hsc = (int32_t)v[2];
             s.gc3 = (int32_t)v[3];
-            s.l = l;
-            s.p = p;
-            s.q = q;
-            s.pad = 0;
+            s.pad = 1;
             cst[c] = s;
         }
         __syncthreads();
@@ -163,7 +163,6 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, uint
     if (tid == 0) {
         float cai_v = 0.f, cpb_v = 0.f, hsc_v = 0.f, gc_v = 0.f, hd_v = 0.f;
         for (int i = 0; i < 2 * NOBJ; ++i) oidx[i] = 0;
-        int bl = 0, bp = 0, bq = 0, bc = m - 1;
         for (int c = 0; c < m; ++c) {
             const CdsStat s = cst[c];
             const float cai = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
@@ -188,12 +187,6 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, uint
                 oidx[8] = c;
                 oidx[9] = (frac - J.gc3_ref) < 0 ? GC_UP : GC_DOWN;
             }
-            if (s.l >= bl) {
-                bl = s.l;
-                bp = s.p;
-                bq = s.q;
-                bc = c;
-            }
         }
         int q = 0;
         for (int i = 0; i < m - 1; ++i)
@@ -210,12 +203,6 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, uint8_t *ntb, uint
         obj[2] = hsc_v;
         obj[3] = hd_v;
         obj[4] = gc_v;
-        obj[5] = (float)bl / (float)J.den_sl;
-        oidx[10] = bc;
-        opql[0] = bp;
-        opql[1] = bq;
-        opql[2] = bl;
-        opql[3] = 0;
     }
     __syncthreads();
 }
@@ -548,7 +535,6 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     int8_t *aa = (int8_t *)(dyn + align16(mL));
     uint8_t *ntb = (uint8_t *)aa + align16(L);
     CdsStat *cst = (CdsStat *)(ntb + align16(3 * L));
-    uint2 *planes = (uint2 *)((uint8_t *)cst + align16(m * (int)sizeof(CdsStat)));
 
     uint32_t gen = 0;
     int dst = 0, out_slot = it.y;
@@ -604,7 +590,8 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
             sh.ppql[tid] = v;
         }
         if (tid < m) {
-            const CdsStat s = J.cst[src][(size_t)parent * m + tid];
+            CdsStat s = J.cst[src][(size_t)parent * m + tid];
+            s.pad = 0;
             J.cst[dst][(size_t)c * m + tid] = s;
             cst[tid] = s;
         }
@@ -633,14 +620,98 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
 
     float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
     __shared__ float s_obj[NOBJ];
-    __shared__ int32_t s_idx[2 * NOBJ], s_pql[4];
-    evaluate(J, cod, ntb, planes, cst, MODE == 2, sh, s_obj, s_idx, s_pql);
+    __shared__ int32_t s_idx[2 * NOBJ];
+    evaluate(J, cod, cst, MODE == 2, sh, s_obj, s_idx);
     int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
     for (int t = tid; t < mL; t += NT) og[t] = (int8_t)(cod[t] - c_syn_start[aa[t % L]]);
-    if (tid < NOBJ) oobj[tid] = s_obj[tid];
-    if (tid < 2 * NOBJ) J.oidx[dst][(size_t)out_slot * 2 * NOBJ + tid] = (int8_t)s_idx[tid];
-    if (tid < 4) J.pql[dst][(size_t)out_slot * 4 + tid] = s_pql[tid];
+    if (tid < NOBJ - 1) oobj[tid] = s_obj[tid];  // objective 5: k_stem_finish
+    if (tid < 2 * NOBJ && tid != 10) J.oidx[dst][(size_t)out_slot * 2 * NOBJ + tid] = (int8_t)s_idx[tid];
     if (tid < m) J.cst[dst][(size_t)out_slot * m + tid] = cst[tid];
+}
+
+// Buffer and slot of the design an evaluation work item refers to: children
+// of the current generation sit at N + child in the next buffer.
+__device__ __forceinline__ void item_slot(const DevJob &J, int mode, int y, int &buf, int &slot) {
+    if (mode == MODE_OFFSPRING) {
+        buf = (int)((J.st->gen + 1u) & 1u);
+        slot = J.N + y;
+    } else {
+        buf = 0;
+        slot = y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_stem: one workgroup per (work item, CDS); blockIdx.x = item * mg + c with
+// mg the largest m of the jobs launched.  CDS not marked by k_design (pad 0)
+// keep their record.  The nucleotide codes are rebuilt from the genome row,
+// the parent's stem seeds the scan (children only), and l, p, q are stored.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, const int2 *__restrict__ items, int mode,
+                                              int mg) {
+    extern __shared__ __align__(16) uint8_t dyn[];
+    __shared__ uint64_t redu[16];
+    __shared__ int32_t bcast;
+    const int2 it = items[blockIdx.x / mg];
+    const int c = (int)(blockIdx.x % mg);
+    const DevJob &J = jobs[it.x];
+    const int L = J.L, m = J.m, n = 3 * L;
+    if (c >= m) return;
+    int buf, slot;
+    item_slot(J, mode, it.y, buf, slot);
+    CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
+    if (cs->pad == 0) return;  // same word for every thread; k_stem never writes it
+    uint8_t *ntb = dyn;
+    uint2 *planes = (uint2 *)(dyn + align16(n));
+    const int8_t *g = J.genome[buf] + ((size_t)slot * m + c) * L;
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        const int gl = c_syn_start[J.aa[k]] + g[k];
+        ntb[3 * k] = J.nt[3 * gl];
+        ntb[3 * k + 1] = J.nt[3 * gl + 1];
+        ntb[3 * k + 2] = J.nt[3 * gl + 2];
+    }
+    __syncthreads();
+    const uint64_t key = stem_cds(ntb, n, planes, mode == MODE_OFFSPRING ? cs : nullptr, &bcast, redu);
+    if (threadIdx.x == 0) {
+        int l, p, q;
+        stem_unkey(key, l, p, q);
+        cs->l = l;
+        cs->p = p;
+        cs->q = q;
+    }
+}
+
+// Objective 5 of each work item's design: the longest stem over its CDS,
+// ties to the last CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880);
+// clears the marks k_design left for k_stem.
+__global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
+                                                     int n_items, int mode) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    const int2 it = items[i];
+    const DevJob &J = jobs[it.x];
+    int buf, slot;
+    item_slot(J, mode, it.y, buf, slot);
+    const int m = J.m;
+    CdsStat *cs = J.cst[buf] + (size_t)slot * m;
+    int bl = 0, bp = 0, bq = 0, bc = m - 1;
+    for (int c = 0; c < m; ++c) {
+        const CdsStat s = cs[c];
+        if (s.l >= bl) {
+            bl = s.l;
+            bp = s.p;
+            bq = s.q;
+            bc = c;
+        }
+        cs[c].pad = 0;
+    }
+    J.obj[buf][(size_t)slot * NOBJ + 5] = (float)bl / (float)J.den_sl;
+    J.oidx[buf][(size_t)slot * 2 * NOBJ + 10] = (int8_t)bc;
+    int32_t *pq = J.pql[buf] + (size_t)slot * 4;
+    pq[0] = bp;
+    pq[1] = bq;
+    pq[2] = bl;
+    pq[3] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,6 +1215,16 @@ hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n,
     }
     return hipGetLastError();
 }
+hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, size_t lds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stem, dim3(n * mg), dim3(256), lds, s, jobs, items, mode, mg);
+    return hipGetLastError();
+}
+hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stem_finish, dim3((n + 255) / 256), dim3(256), 0, s, jobs, items, n, mode);
+    return hipGetLastError();
+}
 hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items);
@@ -1178,6 +1259,9 @@ hipError_t set_design_lds_limit(size_t bytes) {
     e = hipFuncSetAttribute((const void *)k_design<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)k_design<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+hipError_t set_stem_lds_limit(size_t bytes) {
+    return hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace cd
