@@ -156,7 +156,7 @@ struct cd_job {
     int4 *d_dom = nullptr;
     int2 *d_asc = nullptr;
     int n_dom = 0, n_asc = 0;
-    size_t lds = 0, stem_lds = 0;
+    size_t lds = 0, stem_lds = 0, niche_lds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool inited = false;
@@ -170,7 +170,7 @@ struct cd_batch {
     int4 *d_dom = nullptr;
     int2 *d_asc = nullptr;
     int n_off = 0, n_dom = 0, n_asc = 0;
-    size_t lds = 0, stem_lds = 0;
+    size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0;  // largest m of the batch's jobs
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -182,7 +182,7 @@ namespace {
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
     size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], sorted, front, np, dom,
-        ref_of, dist, bucket, niche, flag, st, self, off, domi, asc;
+        ref_of, dist, niche, flag, st, self, off, domi, asc;
     size_t total;
 };
 
@@ -215,8 +215,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
     o.dom = s.add((size_t)P * W * 8);
     o.ref_of = s.add((size_t)P * 4);
     o.dist = s.add((size_t)P * 4);
-    o.bucket = s.add((size_t)P * 4);
-    o.niche = s.add((size_t)(5 * R + P) * 4);
+    o.niche = s.add((size_t)6 * R * 4);
     o.flag = s.add((size_t)P);
     o.st = s.add(sizeof(cd::SelState));
     o.self = s.add(sizeof(cd::DevJob));
@@ -228,7 +227,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
 
 void work_counts(int P, int W, int *n_dom, int *n_asc) {
     *n_dom = ((P + 255) / 256) * W;
-    *n_asc = (P + 255) / 256;
+    *n_asc = (P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS;
 }
 
 int validate(const cd_job_desc *d) {
@@ -331,7 +330,6 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.dom = (uint64_t *)(base + o.dom);
     h.ref_of = (int32_t *)(base + o.ref_of);
     h.dist = (float *)(base + o.dist);
-    h.bucket = (int32_t *)(base + o.bucket);
     h.niche = (int32_t *)(base + o.niche);
     h.flag = (uint8_t *)(base + o.flag);
     h.st = (cd::SelState *)(base + o.st);
@@ -342,13 +340,15 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     for (int t = 0; t < (J->P + 255) / 256; ++t)
         for (int w = 0; w < J->W; ++w) dom[q++] = make_int4(0, t * 256, w, 0);
     int2 *asc = (int2 *)&host[o.asc];
-    for (int t = 0; t < J->n_asc; ++t) asc[t] = make_int2(0, t * 256);
+    for (int t = 0; t < J->n_asc; ++t) asc[t] = make_int2(0, t * cd::ASSOC_DESIGNS);
     J->d_dom = (int4 *)(base + o.domi);
     J->d_asc = (int2 *)(base + o.asc);
     if (e == hipSuccess) e = hipMemcpy(J->mem, host.data(), o.total, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamCreate(&J->stream);
     if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
     if (e == hipSuccess && J->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(J->stem_lds);
+    J->niche_lds = cd::niche_lds_bytes(J->P);
+    if (e == hipSuccess && J->niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(J->niche_lds);
     if (e != hipSuccess) {
         (void)hipFree(J->mem);
         delete J;
@@ -367,7 +367,7 @@ struct Launch {
     int n_dom;
     const int2 *asc;
     int n_asc;
-    size_t lds, stem_lds;
+    size_t lds, stem_lds, niche_lds;
     int mg;  // largest m
 };
 
@@ -391,7 +391,7 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
     if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, s);
     if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
     if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
-    if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, s);
+    if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, w.niche_lds, s);
     return e;
 }
 
@@ -408,7 +408,7 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 }
 
 Launch job_launch(cd_job *J) {
-    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds, J->stem_lds, J->m};
+    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds, J->stem_lds, J->niche_lds, J->m};
 }
 
 }  // namespace
@@ -536,7 +536,7 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(hipEventRecord(ev[5], J->stream));
         HIPCHK(cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream));
         HIPCHK(hipEventRecord(ev[6], J->stream));
-        HIPCHK(cd::launch_niche(J->d_self, 1, J->stream));
+        HIPCHK(cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream));
         HIPCHK(hipEventRecord(ev[7], J->stream));
         HIPCHK(hipEventSynchronize(ev[7]));
         for (int k = 0; k < NK; ++k) {
@@ -624,7 +624,7 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, J->stream);
     if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->stream);
     if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream);
-    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->stream);
+    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
     if (e != hipSuccess) {
         cd_job_destroy(J);
@@ -697,13 +697,14 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
         B->jobs.push_back(J);
         descs.push_back(J->h);
         for (int c = 0; c < J->N; ++c) off.push_back(make_int2(j, c));
-        for (int t = 0; t < (J->P + 255) / 256; ++t) {
-            asc.push_back(make_int2(j, t * 256));
+        for (int t = 0; t < (J->P + 255) / 256; ++t)
             for (int w = 0; w < J->W; ++w) dom.push_back(make_int4(j, t * 256, w, 0));
-        }
+        for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
+            asc.push_back(make_int2(j, t * cd::ASSOC_DESIGNS));
         B->lds = std::max(B->lds, J->lds);
         B->stem_lds = std::max(B->stem_lds, J->stem_lds);
         B->mg = std::max(B->mg, J->m);
+        B->niche_lds = std::max(B->niche_lds, J->niche_lds);
     }
     B->n_off = (int)off.size();
     B->n_dom = (int)dom.size();
@@ -719,6 +720,7 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     if (e == hipSuccess) e = hipStreamCreate(&B->stream);
     if (e == hipSuccess && B->lds > 48 * 1024) e = cd::set_design_lds_limit(B->lds);
     if (e == hipSuccess && B->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(B->stem_lds);
+    if (e == hipSuccess && B->niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(B->niche_lds);
     if (e != hipSuccess) {
         cd_batch_destroy(B);
         return fail(CD_E_DEVICE, std::string("batch setup: ") + hipGetErrorString(e));
@@ -731,7 +733,7 @@ int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
         const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom, B->n_dom, B->d_asc, B->n_asc,
-                       B->lds, B->stem_lds, B->mg};
+                       B->lds, B->stem_lds, B->niche_lds, B->mg};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }

@@ -11,6 +11,7 @@ namespace cd {
 enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
 
 constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
+constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (one wave each)
 
 size_t design_lds_bytes(int L, int m);
 size_t stem_lds_bytes(int L);
@@ -30,7 +31,10 @@ hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);
 // items: (job, first design)
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s);
 // one workgroup per job
-hipError_t launch_niche(const DevJob *jobs, int njobs, hipStream_t s);
+// one workgroup per job; lds = niche_lds_bytes(largest P)
+size_t niche_lds_bytes(int P);
+hipError_t set_niche_lds_limit(size_t bytes);
+hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, hipStream_t s);
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s);
 

@@ -1010,15 +1010,20 @@ __device__ __forceinline__ float perp_dist(const float *f, const float *w) {  //
     return sqrtf(d);
 }
 
-// Nearest reference line of every ranked design (sorting.cuh:1105-1180).
-// item = (job, first design).
-__global__ __launch_bounds__(256) void k_associate(const DevJob *__restrict__ jobs, const int2 *__restrict__ items) {
+// Nearest reference line of every ranked design (sorting.cuh:1105-1180): one
+// wave per design, lanes take reference lines r = lane, lane + 64, ... and the
+// wave keeps the smallest distance, ties to the lowest index (the first
+// minimum of a sweep in index order).  item = (job, first of ASSOC_DESIGNS
+// designs).
+__global__ __launch_bounds__(64 * ASSOC_DESIGNS) void k_associate(const DevJob *__restrict__ jobs,
+                                                                  const int2 *__restrict__ items) {
     const int2 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
     const SelState *st = J.st;
     if (st->n_cut) return;
-    const int d = it.y + (int)threadIdx.x;
-    if (d >= J.P || J.front_of[d] < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int d = it.y + (int)(threadIdx.x >> 6);
+    if (d >= J.P || J.front_of[d] < 0) return;  // whole wave
     const float *obj = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
     float f[NOBJ];
 #pragma unroll
@@ -1028,15 +1033,26 @@ __global__ __launch_bounds__(256) void k_associate(const DevJob *__restrict__ jo
     }
     float best = FLT_MAX;
     int br = 0;
-    for (int r = 0; r < J.R; ++r) {
+    for (int r = lane; r < J.R; r += 64) {
         const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
         if (dd < best) {
             best = dd;
             br = r;
         }
     }
-    J.ref_of[d] = br;
-    J.dist[d] = best;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o, 64);
+        const int orr = __shfl_xor(br, o, 64);
+        if (ob < best || (ob == best && orr < br)) {
+            best = ob;
+            br = orr;
+        }
+    }
+    if (lane == 0) {
+        J.ref_of[d] = br;
+        J.dist[d] = best;
+    }
 }
 
 __device__ __forceinline__ uint64_t niche_prio(const DevJob &J, uint32_t gen, int d) {
@@ -1054,8 +1070,11 @@ __device__ __forceinline__ int block_sum_int(int v, int *red) {
 
 // One workgroup per job: niche counts, the level sweeps of
 // referenceBasedSorting (sorting.cuh:1182-1279) in closed form, and the
-// survivor list in ascending pool index.  Ends the generation.
+// survivor list in ascending pool index.  Ends the generation.  Dynamic LDS
+// holds the last front bucketed by reference point: design indices [P] and
+// the high words of their priorities [P].
 __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t dyn[];
     const DevJob &J = jobs[blockIdx.x];
     SelState *st = J.st;
     const int P = J.P, N = J.N, R = J.R, tid = threadIdx.x, NT = blockDim.x;
@@ -1064,8 +1083,9 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     __shared__ int red[16];
     __shared__ int s_carry;
     if (!st->n_cut) {
-        int *cnt = J.niche, *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R;
-        uint32_t *prio_hi = (uint32_t *)(kp + R);
+        int *cnt = J.niche, *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R, *lvl0 = kp + R;
+        int *lb = (int *)dyn;
+        uint32_t *lp = (uint32_t *)(lb + P);
         for (int r = tid; r < R; r += NT) {
             cnt[r] = 0;
             rem[r] = 0;
@@ -1090,28 +1110,49 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         for (int d = tid; d < P; d += NT) {
             if (J.front_of[d] == last) {
                 const int r = J.ref_of[d];
-                J.bucket[off[r] + atomicAdd(&fill[r], 1)] = d;
-                prio_hi[d] = (uint32_t)(niche_prio(J, gen, d) >> 32);
+                const int slot = off[r] + atomicAdd(&fill[r], 1);
+                lb[slot] = d;
+                lp[slot] = (uint32_t)(niche_prio(J, gen, d) >> 32);
             }
         }
         __syncthreads();
         const int need = N - st->base;
-        // picks made through level i: C(i) = sum_r clamp(i + 1 - cnt[r], 0, rem[r])
+        // picks made through level i: C(i) = sum_r clamp(i + 1 - cnt[r], 0, rem[r]),
+        // nondecreasing in i; the final level is the smallest i with C(i) >= need
+        // (or P), found by galloping from 0 (it is usually 0 or 1) then bisection
         auto picks_through = [&](int i) -> int {
-            int s = 0;
-            for (int r = tid; r < R; r += NT) s += min(max(i + 1 - cnt[r], 0), rem[r]);
-            return block_sum_int(s, red);
+            int sum = 0;
+            for (int r = tid; r < R; r += NT) sum += min(max(i + 1 - cnt[r], 0), rem[r]);
+            return block_sum_int(sum, red);
         };
-        int lo = 0, hi = P;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (picks_through(mid) >= need)
-                hi = mid;
-            else
-                lo = mid + 1;
+        int istar, prevC;
+        const int c0 = picks_through(0);
+        if (c0 >= need) {
+            istar = 0;
+            prevC = 0;
+        } else {
+            int lo = 0, clo = c0, hi, step = 1;
+            for (;;) {  // C(lo) = clo < need
+                hi = min(lo + step, P);
+                const int ch = picks_through(hi);
+                if (ch >= need || hi == P) break;
+                lo = hi;
+                clo = ch;
+                step <<= 1;
+            }
+            while (hi - lo > 1) {  // C(lo) < need, and C(hi) >= need or hi == P
+                const int mid = (lo + hi) >> 1;
+                const int cm = picks_through(mid);
+                if (cm >= need) {
+                    hi = mid;
+                } else {
+                    lo = mid;
+                    clo = cm;
+                }
+            }
+            istar = hi;
+            prevC = clo;
         }
-        const int istar = lo;
-        const int prevC = istar > 0 ? picks_through(istar - 1) : 0;
         const int need_final = need - prevC;
         if (tid == 0) s_carry = 0;
         __syncthreads();
@@ -1125,36 +1166,42 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
             if (tid == 0) s_carry += total;
             __syncthreads();
         }
-        for (int d = tid; d < P; d += NT) {
+        for (int r = tid; r < R; r += NT) {  // level 0 of an empty niche takes its nearest candidate
+            int l0 = -1;
+            if (kp[r] > 0 && cnt[r] == 0) {
+                float bd = FLT_MAX;
+                const int *b = lb + off[r];
+                for (int t = 0; t < rem[r]; ++t) {
+                    const int e = b[t];
+                    const float de = J.dist[e];
+                    if (l0 < 0 || de < bd || (de == bd && e < l0)) {
+                        bd = de;
+                        l0 = e;
+                    }
+                }
+            }
+            lvl0[r] = l0;
+        }
+        __syncthreads();
+        for (int d = tid; d < P; d += NT) {  // later levels: the kk smallest priorities of the bucket
             const int f = J.front_of[d];
             uint8_t sel = (f >= 0 && f < last);
             if (f == last) {
-                const int r = J.ref_of[d], k = kp[r];
-                const int *b = J.bucket + off[r];
-                const int nb = rem[r];
-                int lvl0 = -1;
-                if (k > 0 && cnt[r] == 0) {  // level 0 takes the nearest candidate
-                    float bd = FLT_MAX;
-                    for (int t = 0; t < nb; ++t) {
-                        const int e = b[t];
-                        const float de = J.dist[e];
-                        if (lvl0 < 0 || de < bd || (de == bd && e < lvl0)) {
-                            bd = de;
-                            lvl0 = e;
-                        }
-                    }
-                }
-                if (d == lvl0) {
+                const int r = J.ref_of[d], l0 = lvl0[r];
+                if (d == l0) {
                     sel = 1;
                 } else {
-                    const int kk = k - (lvl0 >= 0 ? 1 : 0);
+                    const int kk = kp[r] - (l0 >= 0 ? 1 : 0);
                     if (kk > 0) {
-                        const uint64_t my = ((uint64_t)prio_hi[d] << 32) | (uint32_t)d;
+                        const uint64_t my = niche_prio(J, gen, d);
+                        const int *b = lb + off[r];
+                        const uint32_t *bp = lp + off[r];
+                        const int nb = rem[r];
                         int rank = 0;
                         for (int t = 0; t < nb; ++t) {
                             const int e = b[t];
-                            if (e == d || e == lvl0) continue;
-                            rank += (((uint64_t)prio_hi[e] << 32) | (uint32_t)e) < my;
+                            if (e == d || e == l0) continue;
+                            rank += (((uint64_t)bp[t] << 32) | (uint32_t)e) < my;
                         }
                         sel = rank < kk;
                     }
@@ -1236,11 +1283,15 @@ hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {
 }
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_associate, dim3(n), dim3(256), 0, s, jobs, items);
+    hipLaunchKernelGGL(k_associate, dim3(n), dim3(64 * ASSOC_DESIGNS), 0, s, jobs, items);
     return hipGetLastError();
 }
-hipError_t launch_niche(const DevJob *jobs, int njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(1024), 0, s, jobs);
+size_t niche_lds_bytes(int P) { return (size_t)P * 8; }
+hipError_t set_niche_lds_limit(size_t bytes) {
+    return hipFuncSetAttribute((const void *)k_niche, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(1024), lds, s, jobs);
     return hipGetLastError();
 }
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s) {

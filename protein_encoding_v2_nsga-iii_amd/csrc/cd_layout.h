@@ -76,8 +76,7 @@ struct DevJob {
     uint64_t *dom;         // [P][W]: bit i of row j set when i dominates j
     int32_t *ref_of;       // [P]
     float *dist;           // [P]
-    int32_t *bucket;       // [P]
-    int32_t *niche;        // [6][R]: count, rem, off, fill, kpick, spare
+    int32_t *niche;        // [6][R]: count, rem, off, fill, kpick, level-0 pick
     uint8_t *flag;         // [P]
     SelState *st;
 };

@@ -32,17 +32,35 @@ __host__ __device__ inline int scan_words(int n) { return (((n + 31) >> 5) + 2 +
 
 // ({hi, lo} >> sh) truncated to 32 bits, 0 <= sh <= 31 (v_alignbit_b32).
 __host__ __device__ inline uint32_t scan_funnel(uint32_t hi, uint32_t lo, int sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)sh);
+#else
     return (uint32_t)(((((uint64_t)hi) << 32) | lo) >> sh);
-}
-
-// Stream X delayed by `off` positions (0 <= off <= 31): bit k of the result is
-// X at position k - off, where x is X's current chunk and xp its previous one.
-__host__ __device__ inline uint32_t scan_back(uint32_t x, uint32_t xp, int off) {
-    return off == 0 ? x : scan_funnel(x, xp, 32 - off);
+#endif
 }
 
 __host__ __device__ inline uint64_t scan_key(int l, int p, int q) {
     return ((uint64_t)l << 40) | ((uint64_t)(0xFFFFF - p) << 20) | (uint64_t)(0xFFFFF - q);
+}
+
+// Window of Kw = min(K, 16) matches: a = floor(log2 Kw), dw = Kw - 2^a.  A
+// run of Kw matches ends at k exactly when windows of 2^a matches end at k
+// and at k - dw (the two overlap and cover Kw positions).  The level is
+// chosen with masks (sel[i] all ones for i == a) so that the per-chunk
+// selection is branch-free; shift and dmask encode the delay by dw.
+struct ScanWindow {
+    uint32_t sel[5];
+    uint32_t dmask;  // all ones when dw == 0 (no second window)
+    int shift;       // 32 - dw, in [1, 31] when dw > 0
+};
+
+__host__ __device__ inline void scan_window(int K, ScanWindow &w) {
+    const int Kw = K < 16 ? K : 16;
+    const int a = Kw >= 16 ? 4 : Kw >= 8 ? 3 : Kw >= 4 ? 2 : Kw >= 2 ? 1 : 0;
+    const int dw = Kw - (1 << a);
+    for (int i = 0; i < 5; ++i) w.sel[i] = (i == a) ? ~0u : 0u;
+    w.dmask = dw ? 0u : ~0u;
+    w.shift = (32 - dw) & 31;
 }
 
 // Scan anti-diagonal S of a sequence of n bases whose planes are sp (sequence,
@@ -57,24 +75,23 @@ __host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, 
     const int len = i1 - i0 + 1;
     const int t0 = i0 + (n - 1 - S);
     const int nch = (len + 31) >> 5;
+    const uint32_t lastmask = (len & 31) ? (1u << (len & 31)) - 1u : ~0u;
     const int ss = i0 & 31, st = t0 & 31;
-    int ws = i0 >> 5, wt = t0 >> 5;
-    uint2 slo = sp[ws], rlo = rp[wt];
+    const uint2 *ps = sp + (i0 >> 5), *pr = rp + (t0 >> 5);
+    uint2 slo = ps[0], rlo = pr[0];
     uint32_t Mp = 0, A2p = 0, A4p = 0, A8p = 0, Wp = 0;
     int carry = 0;  // matches ending at the last position before the previous chunk
+    ScanWindow win;
+    scan_window(K, win);
     for (int c = 0; c <= nch; ++c) {
         uint32_t M = 0;
         if (c < nch) {
-            const uint2 shi = sp[ws + 1], rhi = rp[wt + 1];
+            const uint2 shi = ps[c + 1], rhi = pr[c + 1];
             const uint32_t xl = scan_funnel(shi.x, slo.x, ss) ^ scan_funnel(rhi.x, rlo.x, st);
             const uint32_t xh = scan_funnel(shi.y, slo.y, ss) ^ scan_funnel(rhi.y, rlo.y, st);
-            M = ~(xl | xh);
-            const int rem = len - 32 * c;
-            if (rem < 32) M &= (1u << rem) - 1u;
+            M = ~(xl | xh) & (c == nch - 1 ? lastmask : ~0u);
             slo = shi;
             rlo = rhi;
-            ++ws;
-            ++wt;
         }
         if (c > 0) {
             // run ends of the previous chunk: a window of K ends there and the
@@ -88,38 +105,22 @@ __host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, 
                 const int ie = i0 + 32 * (c - 1) + k;
                 const uint64_t key = scan_key(l, ie - l + 1, S - ie);
                 if (key > best) best = key;
-                if (l > K) K = l;
+                if (l > K) {
+                    K = l;
+                    scan_window(K, win);
+                }
             }
             carry = (Mp == ~0u) ? carry + 32 : __builtin_clz(~Mp);
         }
+        // windows of 2, 4, 8, 16 matches ending at each position
         const uint32_t A2 = M & scan_funnel(M, Mp, 31);
         const uint32_t A4 = A2 & scan_funnel(A2, A2p, 30);
         const uint32_t A8 = A4 & scan_funnel(A4, A4p, 28);
-        // window of Kw = min(K, 16) matches ending at each position
-        const int Kw = K < 16 ? K : 16;
-        uint32_t W;
-        int off;
-        if (Kw == 16) {
-            W = A8 & scan_funnel(A8, A8p, 24);
-            off = 16;
-        } else {
-            W = ~0u;
-            off = 0;
-        }
-        if (Kw & 8) {
-            W &= scan_back(A8, A8p, off);
-            off += 8;
-        }
-        if (Kw & 4) {
-            W &= scan_back(A4, A4p, off);
-            off += 4;
-        }
-        if (Kw & 2) {
-            W &= scan_back(A2, A2p, off);
-            off += 2;
-        }
-        if (Kw & 1) W &= scan_back(M, Mp, off);
-        Wp = W;
+        const uint32_t A16 = A8 & scan_funnel(A8, A8p, 24);
+        const uint32_t Am = (M & win.sel[0]) | (A2 & win.sel[1]) | (A4 & win.sel[2]) | (A8 & win.sel[3]) |
+                            (A16 & win.sel[4]);
+        const uint32_t Ap = (Mp & win.sel[0]) | (A2p & win.sel[1]) | (A4p & win.sel[2]) | (A8p & win.sel[3]);
+        Wp = Am & (scan_funnel(Am, Ap, win.shift) | win.dmask);  // level 4 has dw == 0
         Mp = M;
         A2p = A2;
         A4p = A4;
