@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-gens", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB three times on the GPU (a plain copy "
+                         "kernel of known size, to calibrate hardware byte counters); not part of the measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -135,6 +138,27 @@ def main():
         log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} designs/s")
     job.close()
     print(json.dumps(out), flush=True)
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
+
+
+def copy_calibration(mib):
+    """Three device-to-device copies of `mib` MiB (HIP runs them as its copy
+    kernel): a kernel of known traffic for scaling hardware byte counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), n) or hip.hipMalloc(ctypes.byref(dst), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, n)
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, n, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copied {mib} MiB three times")
 
 
 if __name__ == "__main__":

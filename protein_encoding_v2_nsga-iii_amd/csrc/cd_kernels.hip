@@ -3,8 +3,8 @@
 // One generation is seven launches over every resident job:
 //   k_design<OFFSPRING>  one workgroup per child: survivor copy, mutation,
 //                        objectives 0-4 (mutationKernel, main.cu:121-240)
-//   k_stem               one workgroup per (child, CDS): longest stem of each
-//                        CDS the mutation changed (calMaximumSL)
+//   k_stem               STEM_PARTS workgroups per (child, CDS): longest stem
+//                        of each CDS the mutation changed (calMaximumSL)
 //   k_stem_finish        one thread per child: objective 5
 //   k_dominance          bit-packed eps-domination matrix (sorting.cuh:994-1023)
 //   k_fronts             one workgroup per job: front peeling, ideal/worst,
@@ -50,8 +50,8 @@ struct DesignShared {
 // ---------------------------------------------------------------------------
 // Longest stem of one CDS held as nucleotide codes in LDS (calMaximumSL,
 // utils.cuh:712-888): the sequence and its reverse complement are packed into
-// bit planes with wave ballots, then each thread scans anti-diagonals
-// S = 5 + tid, 5 + tid + blockDim, ... bit-parallel (cd_scan.h) and the
+// bit planes with wave ballots, then each thread scans the anti-diagonals
+// S = 5 + (part + parts * j) * blockDim + tid bit-parallel (cd_scan.h) and the
 // workgroup takes the maximum key (longest, then smallest p, then smallest q).
 // seed_from, when given, is the parent's record of this CDS: the length of
 // the parent's stem as it stands in the child is a stem the child has, so the
@@ -65,8 +65,8 @@ __device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p
     return k;
 }
 
-__device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, int32_t *bcast,
-                             uint64_t *redu) {
+__device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, int part, int parts,
+                             int32_t *bcast, uint64_t *redu) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     const int W = scan_words(n);
     uint2 *sp = planes, *rp = planes + W;
@@ -87,7 +87,8 @@ __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const Cds
     __syncthreads();
     int K = max(*bcast, 1);
     uint64_t best = 0;
-    for (int S = MIN_HAIRPIN + 1 + tid; S <= 2 * n - 3; S += NT) scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
+    for (int S = MIN_HAIRPIN + 1 + part * NT + tid; S <= 2 * n - 3; S += parts * NT)
+        scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
     return block_max_u64(best, redu);
 }
 
@@ -129,6 +130,7 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, CdsStat *cst, bool
             s.hsc = (int32_t)v[2];
             s.gc3 = (int32_t)v[3];
             s.pad = 1;
+            s.key = 0;
             cst[c] = s;
         }
         __syncthreads();
@@ -642,18 +644,21 @@ __device__ __forceinline__ void item_slot(const DevJob &J, int mode, int y, int 
 }
 
 // ---------------------------------------------------------------------------
-// k_stem: one workgroup per (work item, CDS); blockIdx.x = item * mg + c with
-// mg the largest m of the jobs launched.  CDS not marked by k_design (pad 0)
-// keep their record.  The nucleotide codes are rebuilt from the genome row,
-// the parent's stem seeds the scan (children only), and l, p, q are stored.
+// k_stem: STEM_PARTS workgroups per (work item, CDS); blockIdx.x =
+// (item * mg + c) * STEM_PARTS + part with mg the largest m of the jobs
+// launched.  CDS not marked by k_design (pad 0) are skipped.  The nucleotide
+// codes are rebuilt from the genome row, the parent's stem seeds the scan
+// (children only), and each part folds its best key into the record with an
+// atomic maximum; k_stem_finish decodes it.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, const int2 *__restrict__ items, int mode,
                                               int mg) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];
     __shared__ int32_t bcast;
-    const int2 it = items[blockIdx.x / mg];
-    const int c = (int)(blockIdx.x % mg);
+    const int part = (int)(blockIdx.x % STEM_PARTS), ic = (int)(blockIdx.x / STEM_PARTS);
+    const int2 it = items[ic / mg];
+    const int c = ic % mg;
     const DevJob &J = jobs[it.x];
     const int L = J.L, m = J.m, n = 3 * L;
     if (c >= m) return;
@@ -671,19 +676,14 @@ __global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, c
         ntb[3 * k + 2] = J.nt[3 * gl + 2];
     }
     __syncthreads();
-    const uint64_t key = stem_cds(ntb, n, planes, mode == MODE_OFFSPRING ? cs : nullptr, &bcast, redu);
-    if (threadIdx.x == 0) {
-        int l, p, q;
-        stem_unkey(key, l, p, q);
-        cs->l = l;
-        cs->p = p;
-        cs->q = q;
-    }
+    const uint64_t key =
+        stem_cds(ntb, n, planes, mode == MODE_OFFSPRING ? cs : nullptr, part, STEM_PARTS, &bcast, redu);
+    if (threadIdx.x == 0 && key) atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
 }
 
 // Objective 5 of each work item's design: the longest stem over its CDS,
-// ties to the last CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880);
-// clears the marks k_design left for k_stem.
+// ties to the last CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880).
+// Stores the stems k_stem found and clears the marks k_design left.
 __global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
                                                      int n_items, int mode) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -696,14 +696,18 @@ __global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ 
     CdsStat *cs = J.cst[buf] + (size_t)slot * m;
     int bl = 0, bp = 0, bq = 0, bc = m - 1;
     for (int c = 0; c < m; ++c) {
-        const CdsStat s = cs[c];
+        CdsStat s = cs[c];
+        if (s.pad) {
+            stem_unkey(s.key, s.l, s.p, s.q);
+            s.pad = 0;
+            cs[c] = s;
+        }
         if (s.l >= bl) {
             bl = s.l;
             bp = s.p;
             bq = s.q;
             bc = c;
         }
-        cs[c].pad = 0;
     }
     J.obj[buf][(size_t)slot * NOBJ + 5] = (float)bl / (float)J.den_sl;
     J.oidx[buf][(size_t)slot * 2 * NOBJ + 10] = (int8_t)bc;
@@ -1264,7 +1268,7 @@ hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n,
 }
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, size_t lds, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stem, dim3(n * mg), dim3(256), lds, s, jobs, items, mode, mg);
+    hipLaunchKernelGGL(k_stem, dim3(n * mg * STEM_PARTS), dim3(256), lds, s, jobs, items, mode, mg);
     return hipGetLastError();
 }
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s) {

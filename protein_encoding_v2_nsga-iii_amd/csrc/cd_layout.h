@@ -32,7 +32,8 @@ struct CdsStat {
     int32_t hsc;  // hidden stop codons over junctions k|k+1, k < L-1
     int32_t gc3;  // codons whose third base is G or C
     int32_t l, p, q;  // longest stem and its position (canonical tie rule)
-    int32_t pad;
+    int32_t pad;      // 1 while the stem fields await k_stem
+    uint64_t key;     // best stem key of the parts of k_stem (atomic max), valid while pad is 1
 };
 
 // Selection state of one job (the reference keeps it in __device__ globals,

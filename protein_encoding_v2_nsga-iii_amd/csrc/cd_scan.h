@@ -43,30 +43,72 @@ __host__ __device__ inline uint64_t scan_key(int l, int p, int q) {
     return ((uint64_t)l << 40) | ((uint64_t)(0xFFFFF - p) << 20) | (uint64_t)(0xFFFFF - q);
 }
 
-// Window of Kw = min(K, 16) matches: a = floor(log2 Kw), dw = Kw - 2^a.  A
+// Window of Kw = min(K, 15) matches: a = floor(log2 Kw), dw = Kw - 2^a.  A
 // run of Kw matches ends at k exactly when windows of 2^a matches end at k
 // and at k - dw (the two overlap and cover Kw positions).  The level is
 // chosen with masks (sel[i] all ones for i == a) so that the per-chunk
-// selection is branch-free; shift and dmask encode the delay by dw.
+// selection is branch-free; shift and dmask encode the delay by dw.  Capping
+// the window at 15 only flags more runs (all of them are measured exactly).
 struct ScanWindow {
-    uint32_t sel[5];
+    uint32_t sel[4];
     uint32_t dmask;  // all ones when dw == 0 (no second window)
     int shift;       // 32 - dw, in [1, 31] when dw > 0
 };
 
 __host__ __device__ inline void scan_window(int K, ScanWindow &w) {
-    const int Kw = K < 16 ? K : 16;
-    const int a = Kw >= 16 ? 4 : Kw >= 8 ? 3 : Kw >= 4 ? 2 : Kw >= 2 ? 1 : 0;
+    const int Kw = K < 15 ? K : 15;
+    const int a = Kw >= 8 ? 3 : Kw >= 4 ? 2 : Kw >= 2 ? 1 : 0;
     const int dw = Kw - (1 << a);
-    for (int i = 0; i < 5; ++i) w.sel[i] = (i == a) ? ~0u : 0u;
+    for (int i = 0; i < 4; ++i) w.sel[i] = (i == a) ? ~0u : 0u;
     w.dmask = dw ? 0u : ~0u;
     w.shift = (32 - dw) & 31;
 }
 
+// State of one lane along one anti-diagonal, carried from chunk to chunk.
+struct ScanState {
+    uint32_t Mp, A2p, A4p, A8p, Wp;  // previous chunk: matches, windows of 2/4/8, flagged window ends
+    int carry;                       // matches ending just before the previous chunk
+};
+
+// One 32-position chunk c with match bits M: measure the runs that end in
+// the previous chunk and carry a window of K, then build this chunk's
+// windows.  FLUSH: M is the (empty) chunk after the last one.
+template <bool FLUSH>
+__host__ __device__ inline void scan_step(ScanState &z, uint32_t M, int c, int i0, int S, int &K, ScanWindow &win,
+                                          uint64_t &best) {
+    uint32_t E = z.Wp & ~scan_funnel(M, z.Mp, 1);  // window ends whose next position does not match
+    while (E) {
+        const int k = __builtin_ctz(E);
+        E &= E - 1u;
+        const uint32_t zr = ~z.Mp & ((2u << k) - 1u);
+        const int l = zr ? k - (31 - __builtin_clz(zr)) : k + 1 + z.carry;
+        const int ie = i0 + 32 * (c - 1) + k;
+        const uint64_t key = scan_key(l, ie - l + 1, S - ie);
+        if (key > best) best = key;
+        if (l > K) {
+            K = l;
+            scan_window(K, win);
+        }
+    }
+    if (FLUSH) return;
+    z.carry = (z.Mp == ~0u) ? z.carry + 32 : __builtin_clz(~z.Mp);
+    const uint32_t A2 = M & scan_funnel(M, z.Mp, 31);
+    const uint32_t A4 = A2 & scan_funnel(A2, z.A2p, 30);
+    const uint32_t A8 = A4 & scan_funnel(A4, z.A4p, 28);
+    const uint32_t Am = (M & win.sel[0]) | (A2 & win.sel[1]) | (A4 & win.sel[2]) | (A8 & win.sel[3]);
+    const uint32_t Ap = (z.Mp & win.sel[0]) | (z.A2p & win.sel[1]) | (z.A4p & win.sel[2]) | (z.A8p & win.sel[3]);
+    z.Wp = Am & (scan_funnel(Am, Ap, win.shift) | win.dmask);
+    z.Mp = M;
+    z.A2p = A2;
+    z.A4p = A4;
+    z.A8p = A8;
+}
+
 // Scan anti-diagonal S of a sequence of n bases whose planes are sp (sequence,
-// .x low bits, .y high bits) and rp (reverse complement).  K is the lane's
-// window (at least 1; raised when a longer stem is found), best its best
-// key.  Pairs (i, S-i) with j - i > min_hairpin are valid.
+// .x low bits, .y high bits) and rp (reverse complement), each of
+// scan_words(n) words.  K is the lane's window (at least 1; raised when a
+// longer stem is found), best its best key.  Pairs (i, S-i) with
+// j - i > min_hairpin are valid.
 __host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, int n, int S, int min_hairpin, int &K,
                                               uint64_t &best) {
     const int i0 = S > n - 1 ? S - (n - 1) : 0;
@@ -79,53 +121,22 @@ __host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, 
     const int ss = i0 & 31, st = t0 & 31;
     const uint2 *ps = sp + (i0 >> 5), *pr = rp + (t0 >> 5);
     uint2 slo = ps[0], rlo = pr[0];
-    uint32_t Mp = 0, A2p = 0, A4p = 0, A8p = 0, Wp = 0;
-    int carry = 0;  // matches ending at the last position before the previous chunk
+    ScanState z{0u, 0u, 0u, 0u, 0u, 0};
     ScanWindow win;
     scan_window(K, win);
-    for (int c = 0; c <= nch; ++c) {
-        uint32_t M = 0;
-        if (c < nch) {
-            const uint2 shi = ps[c + 1], rhi = pr[c + 1];
-            const uint32_t xl = scan_funnel(shi.x, slo.x, ss) ^ scan_funnel(rhi.x, rlo.x, st);
-            const uint32_t xh = scan_funnel(shi.y, slo.y, ss) ^ scan_funnel(rhi.y, rlo.y, st);
-            M = ~(xl | xh) & (c == nch - 1 ? lastmask : ~0u);
-            slo = shi;
-            rlo = rhi;
-        }
-        if (c > 0) {
-            // run ends of the previous chunk: a window of K ends there and the
-            // next position does not match
-            uint32_t E = Wp & ~scan_funnel(M, Mp, 1);
-            while (E) {
-                const int k = __builtin_ctz(E);
-                E &= E - 1u;
-                const uint32_t z = ~Mp & ((2u << k) - 1u);
-                const int l = z ? k - (31 - __builtin_clz(z)) : k + 1 + carry;
-                const int ie = i0 + 32 * (c - 1) + k;
-                const uint64_t key = scan_key(l, ie - l + 1, S - ie);
-                if (key > best) best = key;
-                if (l > K) {
-                    K = l;
-                    scan_window(K, win);
-                }
-            }
-            carry = (Mp == ~0u) ? carry + 32 : __builtin_clz(~Mp);
-        }
-        // windows of 2, 4, 8, 16 matches ending at each position
-        const uint32_t A2 = M & scan_funnel(M, Mp, 31);
-        const uint32_t A4 = A2 & scan_funnel(A2, A2p, 30);
-        const uint32_t A8 = A4 & scan_funnel(A4, A4p, 28);
-        const uint32_t A16 = A8 & scan_funnel(A8, A8p, 24);
-        const uint32_t Am = (M & win.sel[0]) | (A2 & win.sel[1]) | (A4 & win.sel[2]) | (A8 & win.sel[3]) |
-                            (A16 & win.sel[4]);
-        const uint32_t Ap = (Mp & win.sel[0]) | (A2p & win.sel[1]) | (A4p & win.sel[2]) | (A8p & win.sel[3]);
-        Wp = Am & (scan_funnel(Am, Ap, win.shift) | win.dmask);  // level 4 has dw == 0
-        Mp = M;
-        A2p = A2;
-        A4p = A4;
-        A8p = A8;
-    }
+    auto match = [&](int c) -> uint32_t {  // match bits of chunk c (reads word c+1 of each plane)
+        const uint2 shi = ps[c + 1], rhi = pr[c + 1];
+        const uint32_t xl = scan_funnel(shi.x, slo.x, ss) ^ scan_funnel(rhi.x, rlo.x, st);
+        const uint32_t xh = scan_funnel(shi.y, slo.y, ss) ^ scan_funnel(rhi.y, rlo.y, st);
+        slo = shi;
+        rlo = rhi;
+        return ~(xl | xh);
+    };
+    int c = 0;
+#pragma unroll 2
+    for (; c < nch - 1; ++c) scan_step<false>(z, match(c), c, i0, S, K, win, best);
+    scan_step<false>(z, match(c) & lastmask, c, i0, S, K, win, best);
+    scan_step<true>(z, 0u, c + 1, i0, S, K, win, best);
 }
 
 }  // namespace cd
