@@ -81,6 +81,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-gens", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-gens", type=int, default=20,
+                    help="generations timed in the batched-vs-per-instance comparison (0 skips it)")
+    ap.add_argument("--batched-jobs", type=int, default=64)
+    ap.add_argument("--served", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the single-job line, run the served form: open-loop Poisson arrivals for "
+                         "this many seconds (the served workload's mix) and print its own JSON line")
+    ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --served, jobs/s")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times on the GPU (a plain copy "
                          "kernel of known size, to calibrate hardware byte counters); not part of the measurement")
@@ -90,6 +97,9 @@ def main():
                                    "(jobs shard across GPUs with no communication, see DESIGN.md)"}))
         sys.exit(2)
 
+    if args.served > 0:
+        print(json.dumps(bench_served(importlib.import_module(PKG), args)), flush=True)
+        return
     cdp, aa, tab = workload(args)
     L = len(aa)
     job = cdp.Job(aa, tab, args.m, args.N, args.H, args.Pm, args.seed).init()
@@ -133,6 +143,9 @@ def main():
                      "frac": round(achieved / PEAK_CELLS, 5), "traffic": None, "kernel": "k_stem",
                      "cells_per_launch": int(cells)},
     }
+    if args.batched_gens > 0:
+        out["batched"] = bench_batched(cdp, args)
+        log(f"batched: {out['batched']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(aa, tab, args)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} designs/s")
@@ -140,6 +153,117 @@ def main():
     print(json.dumps(out), flush=True)
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib)
+
+
+def bench_batched(cdp, args):
+    """Mixed-size batched workload (64 jobs: U{50..2000} aa, m in {2,4,8},
+    N in {256,512,1024}, H=4, master seed 4), all resident at once: the same
+    jobs timed advanced one at a time (each job's own graph on its own
+    stream, one after another) and advanced together (one graph per
+    generation over all of them).  Results are identical either way
+    (tests/test_gpu_batch.py); only the time differs."""
+    W = importlib.import_module(PKG + ".workloads")
+    specs = W.batched_mix(args.batched_jobs, 4)
+    jobs = [cdp.Job(*sp.build()).init() for sp in specs]
+    K = args.batched_gens
+    for j in jobs:
+        j.step(2)
+    for j in jobs:
+        j.sync()
+    log(f"batched mix: {len(jobs)} jobs created and initialised")
+    per_gen = []  # per-instance: latency of one generation of one job
+    t_alone = 0.0
+    for j in jobs:
+        ms = [j.step_timed(1) for _ in range(K)]
+        per_gen += ms
+        t_alone += sum(ms)
+    b = cdp.Batch(jobs)
+    b.step(2)
+    b.sync()
+    bat = [b.step_timed(1) for _ in range(K)]
+    t_batch = sum(bat)
+    b.close()
+    for j in jobs:
+        j.close()
+    designs = sum(sp.N for sp in specs) * K
+    return {"workload": f"{len(specs)} jobs, U{{50..2000}} aa, m in {{2,4,8}}, N in {{256,512,1024}}, H=4, seed 4; "
+                        f"{K} timed generations",
+            "designs": designs,
+            "per_instance_designs_s": round(designs / (t_alone / 1e3), 1),
+            "batched_designs_s": round(designs / (t_batch / 1e3), 1),
+            "speedup": round(t_alone / t_batch, 3),
+            "per_instance_generation_ms": {"p50": round(float(np.percentile(per_gen, 50)), 4),
+                                           "p99": round(float(np.percentile(per_gen, 99)), 4)},
+            "batched_generation_ms": {"p50": round(float(np.percentile(bat, 50)), 4),
+                                      "p99": round(float(np.percentile(bat, 99)), 4)}}
+
+
+def bench_served(cdp, args):
+    """Served form: Poisson arrivals at --rate jobs/s for --served seconds
+    (lognormal protein length, median 350 aa, clipped to 30..3000; m in
+    {2,4,8}; N in {256..2048}; 300..1000 generations; H=4; master seed 5)
+    submitted to one server with a 128 GiB device budget.  Reports designs/s
+    over the whole run and p50/p99 time to front (submission to result on
+    the host), beside the same jobs run sequentially one at a time through
+    the per-instance path (timed on every 10th job and scaled, stated)."""
+    import threading
+    W = importlib.import_module(PKG + ".workloads")
+    arrivals = W.served_arrivals(args.rate, args.served, 5)
+    built = [a.spec.build() for a in arrivals]
+    log(f"served: {len(arrivals)} arrivals over {args.served:.0f} s")
+    srv = cdp.Server(128 << 30, 512)
+    tickets = [None] * len(arrivals)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, a in enumerate(arrivals):
+            dt = a.t - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = srv.submit(*built[i], a.spec.generations)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    log("served: all submitted")
+    ttf, queued = [], []
+    for i, t in enumerate(tickets):
+        while srv.wait(t, 30.0) not in (cdp.SERVER_DONE, cdp.SERVER_FAILED):
+            log(f"served: waiting for job {i} of {len(tickets)}; {srv.stats()}")
+        r = srv.result(t)
+        ttf.append(r["times"]["time_to_front"])
+        queued.append(r["times"]["queued"])
+    wall = time.perf_counter() - t0
+    designs = sum(a.spec.N * a.spec.generations for a in arrivals)
+    srv.close()
+    log(f"served: done in {wall:.1f} s")
+    # the same jobs one at a time through the per-instance path (a sample)
+    seq_ms, seq_designs = 0.0, 0
+    for i in range(0, len(arrivals), 10):
+        j = cdp.Job(*built[i]).init()
+        seq_ms += j.step_timed(arrivals[i].spec.generations)
+        seq_designs += arrivals[i].spec.N * arrivals[i].spec.generations
+        j.close()
+    seq_rate = seq_designs / (seq_ms / 1e3)
+    log(f"served: sequential sample done")
+    return {"metric": "served form: designs/s and p50/p99 time-to-front under open-loop Poisson arrivals",
+            "value": round(designs / wall, 1), "unit": "designs/s", "n_gpus": 1, "higher_is_better": True,
+            "data": "synthetic",
+            "config": {"workload": f"Poisson arrivals at {args.rate:g} jobs/s for {args.served:g} s: lognormal "
+                                   f"protein length (median 350 aa, sigma 0.8, clipped 30..3000), m in {{2,4,8}}, "
+                                   f"N in {{256..2048}}, 300..1000 generations, H=4, master seed 5; 128 GiB budget",
+                       "jobs": len(arrivals), "designs": designs},
+            "wall_s": round(wall, 3),
+            "time_to_front_s": {"p50": round(float(np.percentile(ttf, 50)), 3),
+                                "p99": round(float(np.percentile(ttf, 99)), 3),
+                                "max": round(float(np.max(ttf)), 3)},
+            "queued_s": {"p50": round(float(np.percentile(queued, 50)), 3),
+                         "p99": round(float(np.percentile(queued, 99)), 3)},
+            "sequential_per_instance": {"designs_s": round(seq_rate, 1),
+                                        "sample": f"every 10th job ({len(range(0, len(arrivals), 10))} jobs) "
+                                                  f"run alone, GPU time by HIP events",
+                                        "estimated_wall_s": round(designs / seq_rate, 1)},
+            "speedup_vs_sequential": round((designs / wall) / seq_rate, 3)}
 
 
 def copy_calibration(mib):

@@ -147,6 +147,37 @@ int cd_batch_step_timed(cd_batch *batch, int32_t generations, float *ms);
 int cd_batch_sync(cd_batch *batch);
 int cd_batch_destroy(cd_batch *batch);
 
+/* Served form (no counterpart in the reference, which runs one job per
+ * process): continuous batching of jobs of different protein lengths,
+ * population sizes and CDS counts.  Requests queue in arrival order; a
+ * worker thread admits the oldest one whenever its device bytes
+ * (cd_job_device_bytes) fit the budget beside the resident jobs and fewer
+ * than max_jobs are resident, and advances every resident job one
+ * generation per launch sequence.  Populations, fronts and generator state
+ * stay on the device until the job's generation budget is spent; then its
+ * final pool is copied to host memory and its device memory is freed.  A
+ * job's result is the same alone and in any mix.  Thread-safe: submit, wait
+ * and result may be called from any thread. */
+typedef struct cd_server cd_server;
+enum { CD_SERVER_QUEUED = 0, CD_SERVER_RUNNING = 1, CD_SERVER_DONE = 2, CD_SERVER_FAILED = 3 };
+int cd_server_create(int64_t hbm_budget_bytes, int32_t max_jobs, cd_server **out);
+/* Copies the description and its arrays; CD_E_BUDGET if the job alone
+ * exceeds the budget.  *ticket identifies the request. */
+int cd_server_submit(cd_server *server, const cd_job_desc *desc, int32_t generations, int64_t *ticket);
+/* Wait up to timeout_s seconds (negative: no limit) for the request to end;
+ * *state = CD_SERVER_*. */
+int cd_server_wait(cd_server *server, int64_t ticket, double timeout_s, int32_t *state);
+/* Copy out a finished request's final pool (as cd_job_read; any pointer may
+ * be NULL) and release it.  times [3] = seconds queued, seconds resident,
+ * seconds from submission to result (time to front). */
+int cd_server_result(cd_server *server, int64_t ticket, int8_t *genome, float *obj, int8_t *idx, int32_t *pql,
+                     int32_t *sorted, int32_t *front_of, double *times);
+/* stats [5] = queued requests, resident jobs, device bytes in use,
+ * generations run (launch sequences), designs produced. */
+int cd_server_stats(cd_server *server, int64_t *stats);
+/* Stops the worker; unfinished requests are dropped. */
+int cd_server_destroy(cd_server *server);
+
 /* Quality indicators on the host (replace the reference's prebuilt `hv`
  * program, main.cu:839-858, and MinEuclid, utils.cuh:42-58).
  * cd_hypervolume: exact hypervolume (minimisation) of n points in d

@@ -29,7 +29,10 @@ ERRORS = {-1: "CD_E_INVALID", -2: "CD_E_NOMEM", -3: "CD_E_DEVICE", -4: "CD_E_STA
 SYMBOLS = ("cd_job_create", "cd_job_init", "cd_job_step", "cd_job_step_timed", "cd_job_sync", "cd_job_generation",
            "cd_job_read", "cd_job_profile", "cd_job_cds_scanned", "cd_job_destroy", "cd_eval_batch", "cd_nds", "cd_select",
            "cd_batch_create", "cd_batch_step", "cd_batch_step_timed", "cd_batch_sync", "cd_batch_destroy",
-           "cd_job_device_bytes", "cd_last_error", "cd_hypervolume", "cd_normalize", "cd_min_distance_spread")
+           "cd_job_device_bytes", "cd_last_error", "cd_hypervolume", "cd_normalize", "cd_min_distance_spread",
+           "cd_server_create", "cd_server_submit", "cd_server_wait", "cd_server_result", "cd_server_stats",
+           "cd_server_destroy")
+SERVER_QUEUED, SERVER_RUNNING, SERVER_DONE, SERVER_FAILED = 0, 1, 2, 3
 
 
 class CdError(RuntimeError):
@@ -95,6 +98,12 @@ def lib():
         L.cd_hypervolume.argtypes = [f64, C.c_int32, C.c_int32, f64, C.POINTER(C.c_double)]
         L.cd_normalize.argtypes = [f32, C.c_int32, f32]
         L.cd_min_distance_spread.argtypes = [f32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.cd_server_create.argtypes = [C.c_int64, C.c_int32, C.POINTER(vp)]
+        L.cd_server_submit.argtypes = [vp, C.POINTER(_Desc), C.c_int32, C.POINTER(C.c_int64)]
+        L.cd_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(C.c_int32)]
+        L.cd_server_result.argtypes = [vp, C.c_int64, i8, f32, i8, i32, i32, i32, f64]
+        L.cd_server_stats.argtypes = [vp, P(np.int64, flags="C_CONTIGUOUS")]
+        L.cd_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -206,6 +215,57 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().cd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Continuous batching of jobs under a device-memory budget (cd_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, max_jobs=256):
+        h = C.c_void_p()
+        _check(lib().cd_server_create(int(hbm_budget_bytes), int(max_jobs), C.byref(h)))
+        self.h = h
+        self._shapes = {}
+
+    def submit(self, aa, tab: Tables, m, N, H, Pm, seed, generations):
+        d, keep = _desc(aa, tab, m, N, H, Pm, seed)
+        t = C.c_int64()
+        _check(lib().cd_server_submit(self.h, C.byref(d), int(generations), C.byref(t)))
+        del keep  # the server copied the description
+        self._shapes[t.value] = (len(aa), m, N)
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        st = C.c_int32()
+        _check(lib().cd_server_wait(self.h, ticket, float(timeout), C.byref(st)))
+        return st.value
+
+    def result(self, ticket):
+        """Final pool of a finished request (as Job.read) and its times
+        (queued, resident, time to front) in seconds; releases the request."""
+        L, m, N = self._shapes.pop(ticket)
+        P = 2 * N
+        out = {"genome": np.zeros((P, m, L), np.int8), "obj": np.zeros((P, 6), np.float32),
+               "idx": np.zeros((P, 12), np.int8), "pql": np.zeros((P, 3), np.int32),
+               "sorted": np.zeros(N, np.int32), "front_of": np.zeros(P, np.int32)}
+        times = np.zeros(3, np.float64)
+        _check(lib().cd_server_result(self.h, ticket, out["genome"], out["obj"], out["idx"], out["pql"],
+                                      out["sorted"], out["front_of"], times))
+        out["times"] = {"queued": times[0], "resident": times[1], "time_to_front": times[2]}
+        return out
+
+    def stats(self):
+        s = np.zeros(5, np.int64)
+        _check(lib().cd_server_stats(self.h, s))
+        return {"queued": int(s[0]), "resident": int(s[1]), "bytes": int(s[2]), "generations": int(s[3]),
+                "designs": int(s[4])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cd_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

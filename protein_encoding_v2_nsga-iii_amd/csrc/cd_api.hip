@@ -9,23 +9,12 @@
 #include <string>
 #include <vector>
 
-#include "cd_kernels.h"
-#include "codon_nsga3.h"
+#include "cd_host.h"
+
+using cdhost::fail;
+using cdhost::Launch;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return fail(CD_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 const int8_t SYN_START[21] = {0, 4, 6, 8, 10, 12, 16, 18, 21, 23, 29, 30, 32, 36, 38, 44, 50, 54, 58, 59, 61};
 const int8_t SYN_NUM[21] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4, 1, 2, 3};
@@ -144,38 +133,6 @@ struct Sizes {
 };
 
 }  // namespace
-
-struct cd_job {
-    cd::DevJob h{};
-    cd::DevJob *d_self = nullptr;
-    int L = 0, m = 0, N = 0, H = 0, R = 0, P = 0, W = 0;
-    void *mem = nullptr;
-    size_t mem_bytes = 0;
-    hipStream_t stream = nullptr;
-    int2 *d_off = nullptr;
-    int4 *d_dom = nullptr;
-    int2 *d_asc = nullptr;
-    int n_dom = 0, n_asc = 0;
-    size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    bool inited = false;
-    bool selection_only = false;
-};
-
-struct cd_batch {
-    std::vector<cd_job *> jobs;
-    cd::DevJob *d_jobs = nullptr;
-    int2 *d_off = nullptr;
-    int4 *d_dom = nullptr;
-    int2 *d_asc = nullptr;
-    int n_off = 0, n_dom = 0, n_asc = 0;
-    size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    int mg = 0;  // largest m of the batch's jobs
-    hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-};
 
 namespace {
 
@@ -358,19 +315,6 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     return CD_OK;
 }
 
-struct Launch {
-    const cd::DevJob *jobs;
-    int njobs;
-    const int2 *off;
-    int n_off;
-    const int4 *dom;
-    int n_dom;
-    const int2 *asc;
-    int n_asc;
-    size_t lds, stem_lds, niche_lds;
-    int mg;  // largest m
-};
-
 // Evaluate the designs of `items`: mutation (children) and objectives 0-4,
 // the stem of every changed CDS, objective 5.
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
@@ -379,6 +323,10 @@ hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_i
     if (e == hipSuccess) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
     return e;
 }
+
+}  // namespace
+
+namespace cdhost {
 
 hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
     hipError_t e = enqueue_designs(w, cd::MODE_OFFSPRING, w.off, w.n_off, s);
@@ -394,6 +342,37 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
     if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, w.niche_lds, s);
     return e;
 }
+
+void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
+    wl = WorkLists();
+    for (int j = 0; j < count; ++j) {
+        cd_job *J = jobs[j];
+        wl.descs.push_back(J->h);
+        for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
+        for (int t = 0; t < (J->P + 255) / 256; ++t)
+            for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * 256, w, 0));
+        for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
+            wl.asc.push_back(make_int2(j, t * cd::ASSOC_DESIGNS));
+        wl.lds = std::max(wl.lds, J->lds);
+        wl.stem_lds = std::max(wl.stem_lds, J->stem_lds);
+        wl.niche_lds = std::max(wl.niche_lds, J->niche_lds);
+        wl.mg = std::max(wl.mg, J->m);
+    }
+}
+
+hipError_t set_lds_limits(const WorkLists &wl) {
+    hipError_t e = hipSuccess;
+    if (wl.lds > 48 * 1024) e = cd::set_design_lds_limit(wl.lds);
+    if (e == hipSuccess && wl.stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(wl.stem_lds);
+    if (e == hipSuccess && wl.niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(wl.niche_lds);
+    return e;
+}
+
+}  // namespace cdhost
+
+using cdhost::enqueue_generation;
+
+namespace {
 
 int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -415,7 +394,7 @@ Launch job_launch(cd_job *J) {
 
 extern "C" {
 
-const char *cd_last_error(void) { return g_err.c_str(); }
+const char *cd_last_error(void) { return cdhost::g_err.c_str(); }
 
 int64_t cd_job_device_bytes(int32_t L, int32_t m, int32_t N, int32_t H) {
     if (L < 3 || m < 2 || N < 1 || H < 1) return -1;
@@ -678,49 +657,36 @@ int cd_select(const float *obj, int32_t N, int32_t H, uint32_t gen, uint64_t see
 // ---------------------------------------------------------------------------
 int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     if (!jobs || count <= 0 || !out) return fail(CD_E_INVALID, "bad batch arguments");
-    cd_batch *B = new cd_batch();
-    std::vector<cd::DevJob> descs;
-    std::vector<int2> off, asc;
-    std::vector<int4> dom;
     for (int j = 0; j < count; ++j) {
         cd_job *J = jobs[j];
-        if (!J || J->selection_only || !J->inited) {
-            delete B;
-            return fail(CD_E_STATE, "batch jobs must be created and initialised");
-        }
+        if (!J || J->selection_only || !J->inited) return fail(CD_E_STATE, "batch jobs must be created and initialised");
         for (int k = 0; k < j; ++k)
-            if (jobs[k] == J) {
-                delete B;
-                return fail(CD_E_INVALID, "a job appears twice in the batch");
-            }
-        (void)hipStreamSynchronize(J->stream);
-        B->jobs.push_back(J);
-        descs.push_back(J->h);
-        for (int c = 0; c < J->N; ++c) off.push_back(make_int2(j, c));
-        for (int t = 0; t < (J->P + 255) / 256; ++t)
-            for (int w = 0; w < J->W; ++w) dom.push_back(make_int4(j, t * 256, w, 0));
-        for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
-            asc.push_back(make_int2(j, t * cd::ASSOC_DESIGNS));
-        B->lds = std::max(B->lds, J->lds);
-        B->stem_lds = std::max(B->stem_lds, J->stem_lds);
-        B->mg = std::max(B->mg, J->m);
-        B->niche_lds = std::max(B->niche_lds, J->niche_lds);
+            if (jobs[k] == J) return fail(CD_E_INVALID, "a job appears twice in the batch");
     }
-    B->n_off = (int)off.size();
-    B->n_dom = (int)dom.size();
-    B->n_asc = (int)asc.size();
+    cd_batch *B = new cd_batch();
+    for (int j = 0; j < count; ++j) {
+        (void)hipStreamSynchronize(jobs[j]->stream);
+        B->jobs.push_back(jobs[j]);
+    }
+    cdhost::WorkLists wl;
+    cdhost::build_lists(jobs, count, wl);
+    B->lds = wl.lds;
+    B->stem_lds = wl.stem_lds;
+    B->niche_lds = wl.niche_lds;
+    B->mg = wl.mg;
+    B->n_off = (int)wl.off.size();
+    B->n_dom = (int)wl.dom.size();
+    B->n_asc = (int)wl.asc.size();
     hipError_t e = hipMalloc(&B->d_jobs, sizeof(cd::DevJob) * count);
-    if (e == hipSuccess) e = hipMalloc(&B->d_off, sizeof(int2) * off.size());
-    if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * dom.size());
-    if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * asc.size());
-    if (e == hipSuccess) e = hipMemcpy(B->d_jobs, descs.data(), sizeof(cd::DevJob) * count, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(B->d_off, off.data(), sizeof(int2) * off.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(B->d_dom, dom.data(), sizeof(int4) * dom.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(B->d_asc, asc.data(), sizeof(int2) * asc.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&B->d_off, sizeof(int2) * wl.off.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * wl.dom.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * wl.asc.size());
+    if (e == hipSuccess) e = hipMemcpy(B->d_jobs, wl.descs.data(), sizeof(cd::DevJob) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_off, wl.off.data(), sizeof(int2) * wl.off.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_dom, wl.dom.data(), sizeof(int4) * wl.dom.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_asc, wl.asc.data(), sizeof(int2) * wl.asc.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamCreate(&B->stream);
-    if (e == hipSuccess && B->lds > 48 * 1024) e = cd::set_design_lds_limit(B->lds);
-    if (e == hipSuccess && B->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(B->stem_lds);
-    if (e == hipSuccess && B->niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(B->niche_lds);
+    if (e == hipSuccess) e = cdhost::set_lds_limits(wl);
     if (e != hipSuccess) {
         cd_batch_destroy(B);
         return fail(CD_E_DEVICE, std::string("batch setup: ") + hipGetErrorString(e));

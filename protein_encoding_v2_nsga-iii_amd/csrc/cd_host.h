@@ -1,0 +1,94 @@
+// cd_host.h -- host-side internals shared by the C ABI (cd_api.hip) and the
+// serving loop (cd_server.hip): the job and batch objects behind the opaque
+// handles, error reporting, and the launch sequence of one generation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "cd_kernels.h"
+#include "codon_nsga3.h"
+
+// One job: tables, pools and selection state in one device allocation.
+struct cd_job {
+    cd::DevJob h{};
+    cd::DevJob *d_self = nullptr;
+    int L = 0, m = 0, N = 0, H = 0, R = 0, P = 0, W = 0;
+    void *mem = nullptr;
+    size_t mem_bytes = 0;
+    hipStream_t stream = nullptr;
+    int2 *d_off = nullptr;
+    int4 *d_dom = nullptr;
+    int2 *d_asc = nullptr;
+    int n_dom = 0, n_asc = 0;
+    size_t lds = 0, stem_lds = 0, niche_lds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool inited = false;
+    bool selection_only = false;
+};
+
+struct cd_batch {
+    std::vector<cd_job *> jobs;
+    cd::DevJob *d_jobs = nullptr;
+    int2 *d_off = nullptr;
+    int4 *d_dom = nullptr;
+    int2 *d_asc = nullptr;
+    int n_off = 0, n_dom = 0, n_asc = 0;
+    size_t lds = 0, stem_lds = 0, niche_lds = 0;
+    int mg = 0;  // largest m of the batch's jobs
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace cdhost {
+
+inline thread_local std::string g_err;  // cd_last_error of the calling thread
+
+inline int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                          \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess) return ::cdhost::fail(CD_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device work lists and launch sizes of one set of jobs advanced together.
+struct Launch {
+    const cd::DevJob *jobs;
+    int njobs;
+    const int2 *off;  // (job, child) of every child
+    int n_off;
+    const int4 *dom;  // (job, first row, word, 0) domination tiles
+    int n_dom;
+    const int2 *asc;  // (job, first design) association groups
+    int n_asc;
+    size_t lds, stem_lds, niche_lds;
+    int mg;  // largest m
+};
+
+// Host images of the work lists of `jobs` (job k = index k of the list)
+// and the largest per-job launch sizes.
+struct WorkLists {
+    std::vector<cd::DevJob> descs;
+    std::vector<int2> off, asc;
+    std::vector<int4> dom;
+    size_t lds = 0, stem_lds = 0, niche_lds = 0;
+    int mg = 0;
+};
+
+void build_lists(cd_job *const *jobs, int count, WorkLists &wl);
+
+// Mutation, evaluation and selection of one generation of every job of `w`.
+hipError_t enqueue_generation(const Launch &w, hipStream_t s);
+
+// Raise the dynamic LDS limits of the kernels to what `wl` needs.
+hipError_t set_lds_limits(const WorkLists &wl);
+
+}  // namespace cdhost
