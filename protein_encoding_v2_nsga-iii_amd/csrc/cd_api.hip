@@ -304,7 +304,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     if (e == hipSuccess) e = hipStreamCreate(&J->stream);
     if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
     if (e == hipSuccess && J->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(J->stem_lds);
-    J->niche_lds = cd::niche_lds_bytes(J->P);
+    J->niche_lds = cd::niche_lds_bytes(J->P, J->R);
     if (e == hipSuccess && J->niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(J->niche_lds);
     if (e != hipSuccess) {
         (void)hipFree(J->mem);

@@ -32,8 +32,8 @@ hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);
 // items: (job, first design)
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s);
 // one workgroup per job
-// one workgroup per job; lds = niche_lds_bytes(largest P)
-size_t niche_lds_bytes(int P);
+// one workgroup per job; lds = the largest niche_lds_bytes of the jobs
+size_t niche_lds_bytes(int P, int R);
 hipError_t set_niche_lds_limit(size_t bytes);
 hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, hipStream_t s);
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s);

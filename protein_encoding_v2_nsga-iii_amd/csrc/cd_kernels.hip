@@ -87,8 +87,19 @@ __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const Cds
     __syncthreads();
     int K = max(*bcast, 1);
     uint64_t best = 0;
-    for (int S = MIN_HAIRPIN + 1 + part * NT + tid; S <= 2 * n - 3; S += parts * NT)
-        scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
+    // Every lane's K is a length some stem of this CDS has, so the wave may
+    // share the largest after each anti-diagonal.  The trip count is the
+    // wave's first lane's, so every lane takes part in the exchange (lanes
+    // past the last anti-diagonal scan empty ones, which return at once).
+    const int stride = parts * NT, S0 = MIN_HAIRPIN + 1 + part * NT + tid;
+    const int Sw = MIN_HAIRPIN + 1 + part * NT + (tid & ~63), last = 2 * n - 3;
+    const int trips = Sw <= last ? (last - Sw) / stride + 1 : 0;
+    for (int j = 0; j < trips; ++j) {
+        const int S = S0 + j * stride;
+        if (S <= last) scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) K = max(K, __shfl_xor(K, o, 64));
+    }
     return block_max_u64(best, redu);
 }
 
@@ -735,13 +746,16 @@ __device__ __forceinline__ bool dominates(const float *a, const float *b) {
     return any && !bad;
 }
 
-// Bit i of dom[j][w] (i = 64w + bit) is set when design i dominates design j;
-// np[j] counts the dominators.  item = (job, first row, word, 0).
+// Bit i of word w of row j (i = 64w + bit) is set when design i dominates
+// design j; the matrix is stored word-major, dom[w * P + j], so that a
+// workgroup's rows are one contiguous store and the front peeling reads one
+// word of consecutive rows per wave.  np[j] counts the dominators.
+// item = (job, first row, word, 0).
 __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items) {
     __shared__ float cand[64 * NOBJ];
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
-    const int P = J.P, W = J.W, word = it.z;
+    const int P = J.P, word = it.z;
     const float *obj = J.obj[pool_buffer(J)];
     for (int t = threadIdx.x; t < 64 * NOBJ; t += blockDim.x) {
         const int i = word * 64 + t / NOBJ;
@@ -759,7 +773,7 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         const int i = word * 64 + b;
         if (i != r && dominates(cand + b * NOBJ, mine)) bits |= 1ull << b;
     }
-    J.dom[(size_t)r * W + word] = bits;
+    J.dom[(size_t)word * P + r] = bits;
     if (bits) atomicAdd(&J.np[r], (int)__popcll(bits));
 }
 
@@ -817,7 +831,9 @@ __device__ __forceinline__ void argmin_pair(float &v, int &id, float ov, int oid
 
 // One workgroup per job: nonDominatedSorting's peeling (sorting.cuh:1026-1086),
 // updateIdealValue/updateWorstValue, findExtremePoints, updateNadirValue_HYP
-// with the MNDF fallback (sorting.cuh:132-972).
+// with the MNDF fallback (sorting.cuh:132-972).  Three passes over the pool:
+// F0 with ideal/worst/F0 maximum, the peeling of further fronts (only while
+// fewer than N are ranked), and the ASF minima of F0 for all six axes.
 __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs) {
     const DevJob &J = jobs[blockIdx.x];
     const int P = J.P, W = J.W, N = J.N;
@@ -826,9 +842,8 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
     __shared__ uint64_t ranked[MAX_W], newm[MAX_W];
     __shared__ int s_cnt, s_new;
     __shared__ float red_v[16 * 18];
-    __shared__ int red_i[16];
+    __shared__ int red_i[16 * NOBJ];
     __shared__ float ext[NOBJ * NOBJ], ideal[NOBJ], worst[NOBJ], f0max[NOBJ];
-    __shared__ float win[NOBJ];
 
     for (int w = tid; w < W; w += NT) {
         ranked[w] = 0;
@@ -840,41 +855,76 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
     }
     if (tid < NOBJ * NOBJ) ext[tid] = J.st->ext[tid];
     __syncthreads();
-    for (int r = tid; r < P; r += NT) {
-        const bool f0 = J.np[r] == 0;
-        J.front_of[r] = f0 ? 0 : -1;
-        if (f0) atomicOr((unsigned long long *)&ranked[r >> 6], 1ull << (r & 63));
-    }
-    __syncthreads();
+    // F0 = designs nobody dominates; ideal / worst over the pool, maximum over F0
     {
+        float mn[NOBJ], mx[NOBJ], fm[NOBJ];
+        for (int k = 0; k < NOBJ; ++k) {
+            mn[k] = FLT_MAX;
+            mx[k] = -FLT_MAX;
+            fm[k] = -FLT_MAX;
+        }
         int c = 0;
-        for (int w = tid; w < W; w += NT) c += __popcll(ranked[w]);
-        atomicAdd(&s_cnt, c);
+        for (int r = tid; r < P; r += NT) {
+            const bool f0 = J.np[r] == 0;
+            J.front_of[r] = f0 ? 0 : -1;
+            if (f0) {
+                atomicOr((unsigned long long *)&ranked[r >> 6], 1ull << (r & 63));
+                ++c;
+            }
+            for (int k = 0; k < NOBJ; ++k) {
+                const float v = obj[(size_t)r * NOBJ + k];
+                mn[k] = fminf(mn[k], v);
+                mx[k] = fmaxf(mx[k], v);
+                if (f0) fm[k] = fmaxf(fm[k], v);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            c += __shfl_xor(c, o, 64);
+            for (int k = 0; k < NOBJ; ++k) {
+                mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+                mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+                fm[k] = fmaxf(fm[k], __shfl_xor(fm[k], o, 64));
+            }
+        }
+        if (lane == 0) {
+            atomicAdd(&s_cnt, c);
+            for (int k = 0; k < NOBJ; ++k) {
+                red_v[wid * 18 + k] = mn[k];
+                red_v[wid * 18 + 6 + k] = mx[k];
+                red_v[wid * 18 + 12 + k] = fm[k];
+            }
+        }
+        __syncthreads();
+        if (tid < NOBJ) {
+            float a = FLT_MAX, b = -FLT_MAX, d = -FLT_MAX;
+            for (int w = 0; w < nw; ++w) {
+                a = fminf(a, red_v[w * 18 + tid]);
+                b = fmaxf(b, red_v[w * 18 + 6 + tid]);
+                d = fmaxf(d, red_v[w * 18 + 12 + tid]);
+            }
+            ideal[tid] = a;
+            worst[tid] = b;
+            f0max[tid] = d;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     int f = 0, last_size = s_cnt;
     while (s_cnt < N) {
         ++f;
         // a design joins front f when all its dominators are ranked
-        for (int r = wid; r < P; r += nw) {
+        int c = 0;
+        for (int r = tid; r < P; r += NT) {
             if ((ranked[r >> 6] >> (r & 63)) & 1ull) continue;
-            bool any = false;
-            for (int w0 = 0; w0 < W && !any; w0 += 64) {
-                const int w = w0 + lane;
-                const uint64_t x = (w < W) ? (J.dom[(size_t)r * W + w] & ~ranked[w]) : 0ull;
-                any = __any(x != 0ull);
-            }
-            if (!any && lane == 0) {
+            uint64_t any = 0;
+            for (int w = 0; w < W; ++w) any |= J.dom[(size_t)w * P + r] & ~ranked[w];
+            if (!any) {
                 atomicOr((unsigned long long *)&newm[r >> 6], 1ull << (r & 63));
                 J.front_of[r] = f;
+                ++c;
             }
         }
-        __syncthreads();
-        {
-            int c = 0;
-            for (int w = tid; w < W; w += NT) c += __popcll(newm[w]);
-            atomicAdd(&s_new, c);
-        }
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(&s_new, c);
         __syncthreads();
         if (s_new == 0) {  // a dominance cycle: every remaining design forms the front
             for (int r = tid; r < P; r += NT)
@@ -899,77 +949,47 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
         __syncthreads();
     }
     const int ranked_n = s_cnt;
-    // ideal / worst over the pool, maximum over F0
-    float mn[NOBJ], mx[NOBJ], fm[NOBJ];
-    for (int k = 0; k < NOBJ; ++k) {
-        mn[k] = FLT_MAX;
-        mx[k] = -FLT_MAX;
-        fm[k] = -FLT_MAX;
-    }
-    for (int r = tid; r < P; r += NT) {
-        const bool f0 = J.front_of[r] == 0;
-        for (int k = 0; k < NOBJ; ++k) {
-            const float v = obj[(size_t)r * NOBJ + k];
-            mn[k] = fminf(mn[k], v);
-            mx[k] = fmaxf(mx[k], v);
-            if (f0) fm[k] = fmaxf(fm[k], v);
+    // findExtremePoints: per axis the ASF minimum over F0 (ascending index),
+    // then against the previous extremes, axis by axis
+    {
+        float bv[NOBJ];
+        int bi[NOBJ];
+        for (int a = 0; a < NOBJ; ++a) {
+            bv[a] = FLT_MAX;
+            bi[a] = 0x7FFFFFFF;
         }
-    }
-    for (int o = 32; o > 0; o >>= 1)
-        for (int k = 0; k < NOBJ; ++k) {
-            mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
-            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
-            fm[k] = fmaxf(fm[k], __shfl_xor(fm[k], o, 64));
+        for (int r = tid; r < P; r += NT) {
+            if (J.front_of[r] != 0) continue;
+            float x[NOBJ];
+            for (int k = 0; k < NOBJ; ++k) x[k] = obj[(size_t)r * NOBJ + k];
+            for (int a = 0; a < NOBJ; ++a) argmin_pair(bv[a], bi[a], asf(x, ideal, a), r);
         }
-    if (lane == 0)
-        for (int k = 0; k < NOBJ; ++k) {
-            red_v[wid * 18 + k] = mn[k];
-            red_v[wid * 18 + 6 + k] = mx[k];
-            red_v[wid * 18 + 12 + k] = fm[k];
-        }
-    __syncthreads();
-    if (tid < NOBJ) {
-        float a = FLT_MAX, b = -FLT_MAX, c = -FLT_MAX;
-        for (int w = 0; w < nw; ++w) {
-            a = fminf(a, red_v[w * 18 + tid]);
-            b = fmaxf(b, red_v[w * 18 + 6 + tid]);
-            c = fmaxf(c, red_v[w * 18 + 12 + tid]);
-        }
-        ideal[tid] = a;
-        worst[tid] = b;
-        f0max[tid] = c;
-    }
-    __syncthreads();
-    // findExtremePoints: candidates are F0 (ascending index) then the previous extremes
-    for (int axis = 0; axis < NOBJ; ++axis) {
-        float bv = FLT_MAX;
-        int bi = 0x7FFFFFFF;
-        for (int r = tid; r < P; r += NT)
-            if (J.front_of[r] == 0) argmin_pair(bv, bi, asf(obj + (size_t)r * NOBJ, ideal, axis), r);
-        if (tid < NOBJ) argmin_pair(bv, bi, asf(ext + tid * NOBJ, ideal, axis), P + tid);
-        for (int o = 32; o > 0; o >>= 1) {
-            const float ov = __shfl_xor(bv, o, 64);
-            const int oi = __shfl_xor(bi, o, 64);
-            argmin_pair(bv, bi, ov, oi);
-        }
-        if (lane == 0) {
-            red_v[wid] = bv;
-            red_i[wid] = bi;
-        }
+        for (int o = 32; o > 0; o >>= 1)
+            for (int a = 0; a < NOBJ; ++a) {
+                const float ov = __shfl_xor(bv[a], o, 64);
+                const int oi = __shfl_xor(bi[a], o, 64);
+                argmin_pair(bv[a], bi[a], ov, oi);
+            }
+        if (lane == 0)
+            for (int a = 0; a < NOBJ; ++a) {
+                red_v[wid * NOBJ + a] = bv[a];
+                red_i[wid * NOBJ + a] = bi[a];
+            }
         __syncthreads();
-        if (tid == 0) {
-            float v = red_v[0];
-            int id = red_i[0];
-            for (int w = 1; w < nw; ++w) argmin_pair(v, id, red_v[w], red_i[w]);
-            if (id != 0x7FFFFFFF && v < FLT_MAX)
-                for (int k = 0; k < NOBJ; ++k)
-                    win[k] = (id >= P) ? ext[(id - P) * NOBJ + k] : obj[(size_t)id * NOBJ + k];
-            else
-                for (int k = 0; k < NOBJ; ++k) win[k] = ext[axis * NOBJ + k];
-        }
-        __syncthreads();
-        if (tid < NOBJ) ext[axis * NOBJ + tid] = win[tid];
-        __syncthreads();
+        if (tid == 0)
+            for (int axis = 0; axis < NOBJ; ++axis) {
+                float v = red_v[axis];
+                int id = red_i[axis];
+                for (int w = 1; w < nw; ++w) argmin_pair(v, id, red_v[w * NOBJ + axis], red_i[w * NOBJ + axis]);
+                for (int t = 0; t < NOBJ; ++t) argmin_pair(v, id, asf(ext + t * NOBJ, ideal, axis), P + t);
+                float win[NOBJ];
+                if (id != 0x7FFFFFFF && v < FLT_MAX)
+                    for (int k = 0; k < NOBJ; ++k)
+                        win[k] = (id >= P) ? ext[(id - P) * NOBJ + k] : obj[(size_t)id * NOBJ + k];
+                else
+                    for (int k = 0; k < NOBJ; ++k) win[k] = ext[axis * NOBJ + k];
+                for (int k = 0; k < NOBJ; ++k) ext[axis * NOBJ + k] = win[k];
+            }
     }
     if (tid == 0) {
         SelState *st = J.st;
@@ -1075,8 +1095,10 @@ __device__ __forceinline__ int block_sum_int(int v, int *red) {
 // One workgroup per job: niche counts, the level sweeps of
 // referenceBasedSorting (sorting.cuh:1182-1279) in closed form, and the
 // survivor list in ascending pool index.  Ends the generation.  Dynamic LDS
-// holds the last front bucketed by reference point: design indices [P] and
-// the high words of their priorities [P].
+// (niche_lds_need) holds the last front bucketed by reference point (design
+// indices [P], high words of their priorities [P]) and, when they fit, the
+// per-reference-point arrays and copies of front_of, ref_of and the flags;
+// otherwise those stay in the job's global scratch.
 __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t dyn[];
     const DevJob &J = jobs[blockIdx.x];
@@ -1086,10 +1108,25 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     const int last = st->last;
     __shared__ int red[16];
     __shared__ int s_carry;
+    const int tier = niche_tier(P, R);
+    int *lb = (int *)dyn;
+    uint32_t *lp = (uint32_t *)(lb + P);
+    int *cnt = tier >= 1 ? (int *)(lp + P) : J.niche;
+    int *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R, *lvl0 = kp + R;
+    const int *fo = J.front_of, *ro = J.ref_of;
+    uint8_t *flag = J.flag;
+    if (tier >= 2) {
+        int *fo2 = (int *)(lp + P) + 6 * R, *ro2 = fo2 + P;
+        uint8_t *fl2 = (uint8_t *)(ro2 + P);
+        for (int d = tid; d < P; d += NT) {
+            fo2[d] = J.front_of[d];
+            ro2[d] = J.ref_of[d];
+        }
+        fo = fo2;
+        ro = ro2;
+        flag = fl2;
+    }
     if (!st->n_cut) {
-        int *cnt = J.niche, *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R, *lvl0 = kp + R;
-        int *lb = (int *)dyn;
-        uint32_t *lp = (uint32_t *)(lb + P);
         for (int r = tid; r < R; r += NT) {
             cnt[r] = 0;
             rem[r] = 0;
@@ -1097,8 +1134,8 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         }
         __syncthreads();
         for (int d = tid; d < P; d += NT) {
-            const int f = J.front_of[d];
-            if (f >= 0) atomicAdd(f < last ? &cnt[J.ref_of[d]] : &rem[J.ref_of[d]], 1);
+            const int f = fo[d];
+            if (f >= 0) atomicAdd(f < last ? &cnt[ro[d]] : &rem[ro[d]], 1);
         }
         if (tid == 0) s_carry = 0;
         __syncthreads();
@@ -1112,8 +1149,8 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
             __syncthreads();
         }
         for (int d = tid; d < P; d += NT) {
-            if (J.front_of[d] == last) {
-                const int r = J.ref_of[d];
+            if (fo[d] == last) {
+                const int r = ro[d];
                 const int slot = off[r] + atomicAdd(&fill[r], 1);
                 lb[slot] = d;
                 lp[slot] = (uint32_t)(niche_prio(J, gen, d) >> 32);
@@ -1188,10 +1225,10 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         }
         __syncthreads();
         for (int d = tid; d < P; d += NT) {  // later levels: the kk smallest priorities of the bucket
-            const int f = J.front_of[d];
+            const int f = fo[d];
             uint8_t sel = (f >= 0 && f < last);
             if (f == last) {
-                const int r = J.ref_of[d], l0 = lvl0[r];
+                const int r = ro[d], l0 = lvl0[r];
                 if (d == l0) {
                     sel = 1;
                 } else {
@@ -1211,20 +1248,21 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
                     }
                 }
             }
-            J.flag[d] = sel;
+            flag[d] = sel;
         }
     } else {
-        for (int d = tid; d < P; d += NT) J.flag[d] = J.front_of[d] >= 0;
+        __syncthreads();
+        for (int d = tid; d < P; d += NT) flag[d] = fo[d] >= 0;
     }
     __syncthreads();
     // survivors in ascending pool index
     const int ch = (P + NT - 1) / NT, d0 = tid * ch, d1 = min(P, d0 + ch);
     int c = 0;
-    for (int d = d0; d < d1; ++d) c += J.flag[d];
+    for (int d = d0; d < d1; ++d) c += flag[d];
     int total;
     int pos = block_scan_excl(c, red, &total);
     for (int d = d0; d < d1; ++d)
-        if (J.flag[d] && pos < N) J.sorted[pos++] = d;
+        if (flag[d] && pos < N) J.sorted[pos++] = d;
     __syncthreads();
     if (tid == 0) st->gen = gen + 1;
 }
@@ -1290,7 +1328,7 @@ hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStr
     hipLaunchKernelGGL(k_associate, dim3(n), dim3(64 * ASSOC_DESIGNS), 0, s, jobs, items);
     return hipGetLastError();
 }
-size_t niche_lds_bytes(int P) { return (size_t)P * 8; }
+size_t niche_lds_bytes(int P, int R) { return niche_lds_need(P, R); }
 hipError_t set_niche_lds_limit(size_t bytes) {
     return hipFuncSetAttribute((const void *)k_niche, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }

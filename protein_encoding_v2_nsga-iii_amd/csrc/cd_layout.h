@@ -51,6 +51,22 @@ struct SelState {
     float ideal[NOBJ], worst[NOBJ], nadir[NOBJ];
 };
 
+// Dynamic LDS of k_niche: the bucketed last front (8P bytes) always; the six
+// per-reference-point arrays (24R bytes) when they fit beside it; and copies
+// of front_of, ref_of and the survivor flags (9P bytes) when all fit.
+constexpr size_t NICHE_LDS_CAP = 150 * 1024;
+
+__host__ __device__ inline int niche_tier(int P, int R) {
+    if ((size_t)17 * P + (size_t)24 * R + 16 <= NICHE_LDS_CAP) return 2;
+    if ((size_t)8 * P + (size_t)24 * R <= NICHE_LDS_CAP) return 1;
+    return 0;
+}
+
+__host__ __device__ inline size_t niche_lds_need(int P, int R) {
+    const int t = niche_tier(P, R);
+    return (size_t)8 * P + (t >= 1 ? (size_t)24 * R : 0) + (t >= 2 ? (size_t)9 * P + 16 : 0);
+}
+
 // Device view of one job.  Pools ping-pong on generation parity: generation
 // g reads buffer g&1 and writes buffer (g+1)&1.
 struct DevJob {

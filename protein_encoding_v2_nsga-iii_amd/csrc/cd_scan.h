@@ -64,44 +64,8 @@ __host__ __device__ inline void scan_window(int K, ScanWindow &w) {
     w.shift = (32 - dw) & 31;
 }
 
-// State of one lane along one anti-diagonal, carried from chunk to chunk.
-struct ScanState {
-    uint32_t Mp, A2p, A4p, A8p, Wp;  // previous chunk: matches, windows of 2/4/8, flagged window ends
-    int carry;                       // matches ending just before the previous chunk
-};
-
-// One 32-position chunk c with match bits M: measure the runs that end in
-// the previous chunk and carry a window of K, then build this chunk's
-// windows.  FLUSH: M is the (empty) chunk after the last one.
-template <bool FLUSH>
-__host__ __device__ inline void scan_step(ScanState &z, uint32_t M, int c, int i0, int S, int &K, ScanWindow &win,
-                                          uint64_t &best) {
-    uint32_t E = z.Wp & ~scan_funnel(M, z.Mp, 1);  // window ends whose next position does not match
-    while (E) {
-        const int k = __builtin_ctz(E);
-        E &= E - 1u;
-        const uint32_t zr = ~z.Mp & ((2u << k) - 1u);
-        const int l = zr ? k - (31 - __builtin_clz(zr)) : k + 1 + z.carry;
-        const int ie = i0 + 32 * (c - 1) + k;
-        const uint64_t key = scan_key(l, ie - l + 1, S - ie);
-        if (key > best) best = key;
-        if (l > K) {
-            K = l;
-            scan_window(K, win);
-        }
-    }
-    if (FLUSH) return;
-    z.carry = (z.Mp == ~0u) ? z.carry + 32 : __builtin_clz(~z.Mp);
-    const uint32_t A2 = M & scan_funnel(M, z.Mp, 31);
-    const uint32_t A4 = A2 & scan_funnel(A2, z.A2p, 30);
-    const uint32_t A8 = A4 & scan_funnel(A4, z.A4p, 28);
-    const uint32_t Am = (M & win.sel[0]) | (A2 & win.sel[1]) | (A4 & win.sel[2]) | (A8 & win.sel[3]);
-    const uint32_t Ap = (z.Mp & win.sel[0]) | (z.A2p & win.sel[1]) | (z.A4p & win.sel[2]) | (z.A8p & win.sel[3]);
-    z.Wp = Am & (scan_funnel(Am, Ap, win.shift) | win.dmask);
-    z.Mp = M;
-    z.A2p = A2;
-    z.A4p = A4;
-    z.A8p = A8;
+__host__ __device__ inline uint32_t scan_level(const ScanWindow &w, uint32_t m, uint32_t a2, uint32_t a4, uint32_t a8) {
+    return (m & w.sel[0]) | (a2 & w.sel[1]) | (a4 & w.sel[2]) | (a8 & w.sel[3]);
 }
 
 // Scan anti-diagonal S of a sequence of n bases whose planes are sp (sequence,
@@ -109,6 +73,12 @@ __host__ __device__ inline void scan_step(ScanState &z, uint32_t M, int c, int i
 // scan_words(n) words.  K is the lane's window (at least 1; raised when a
 // longer stem is found), best its best key.  Pairs (i, S-i) with
 // j - i > min_hairpin are valid.
+//
+// Per chunk the steady state computes the match bits, the windows of 2, 4, 8
+// matches and the K-window ends.  Only a chunk with a K-window end (rare once
+// K is near the longest stem) measures its runs exactly; a run that reaches
+// back past its chunk is followed backwards through earlier chunks, rebuilt
+// from the planes.
 __host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, int n, int S, int min_hairpin, int &K,
                                               uint64_t &best) {
     const int i0 = S > n - 1 ? S - (n - 1) : 0;
@@ -120,23 +90,73 @@ __host__ __device__ inline void scan_diagonal(const uint2 *sp, const uint2 *rp, 
     const uint32_t lastmask = (len & 31) ? (1u << (len & 31)) - 1u : ~0u;
     const int ss = i0 & 31, st = t0 & 31;
     const uint2 *ps = sp + (i0 >> 5), *pr = rp + (t0 >> 5);
-    uint2 slo = ps[0], rlo = pr[0];
-    ScanState z{0u, 0u, 0u, 0u, 0u, 0};
     ScanWindow win;
     scan_window(K, win);
-    auto match = [&](int c) -> uint32_t {  // match bits of chunk c (reads word c+1 of each plane)
+    // match bits of chunk c from words c and c+1 (the last chunk is masked)
+    auto chunk_at = [&](int c) -> uint32_t {
+        const uint2 a0 = ps[c], a1 = ps[c + 1], b0 = pr[c], b1 = pr[c + 1];
+        const uint32_t xl = scan_funnel(a1.x, a0.x, ss) ^ scan_funnel(b1.x, b0.x, st);
+        const uint32_t xh = scan_funnel(a1.y, a0.y, ss) ^ scan_funnel(b1.y, b0.y, st);
+        return ~(xl | xh) & (c == nch - 1 ? lastmask : ~0u);
+    };
+    uint32_t Mp = 0, A2p = 0, A4p = 0, A8p = 0, Lp = 0, Wp = 0;
+    // runs ending in chunk c-1 (bits Wp) given the next chunk's matches M
+    auto measure = [&](uint32_t M, int c) {
+        uint32_t E = Wp & ~scan_funnel(M, Mp, 1);
+        while (E) {
+            const int k = __builtin_ctz(E);
+            E &= E - 1u;
+            const uint32_t zr = ~Mp & ((2u << k) - 1u);
+            int l;
+            if (zr) {
+                l = k - (31 - __builtin_clz(zr));
+            } else {  // the run reaches back past chunk c-1
+                l = k + 1;
+                for (int cc = c - 2; cc >= 0; --cc) {
+                    const uint32_t Mc = chunk_at(cc);
+                    if (Mc != ~0u) {
+                        l += __builtin_clz(~Mc);
+                        break;
+                    }
+                    l += 32;
+                }
+            }
+            const int ie = i0 + 32 * (c - 1) + k;
+            const uint64_t key = scan_key(l, ie - l + 1, S - ie);
+            if (key > best) best = key;
+            if (l > K) {
+                K = l;
+                scan_window(K, win);
+                Lp = scan_level(win, Mp, A2p, A4p, A8p);
+            }
+        }
+    };
+    auto step = [&](uint32_t M, int c) {
+        if (Wp) measure(M, c);
+        const uint32_t A2 = M & scan_funnel(M, Mp, 31);
+        const uint32_t A4 = A2 & scan_funnel(A2, A2p, 30);
+        const uint32_t A8 = A4 & scan_funnel(A4, A4p, 28);
+        const uint32_t Am = scan_level(win, M, A2, A4, A8);
+        Wp = Am & (scan_funnel(Am, Lp, win.shift) | win.dmask);
+        Mp = M;
+        A2p = A2;
+        A4p = A4;
+        A8p = A8;
+        Lp = Am;
+    };
+    uint2 slo = ps[0], rlo = pr[0];
+    int c = 0;
+#pragma unroll 2
+    for (; c < nch - 1; ++c) {  // every chunk but the last, streaming the plane words
         const uint2 shi = ps[c + 1], rhi = pr[c + 1];
         const uint32_t xl = scan_funnel(shi.x, slo.x, ss) ^ scan_funnel(rhi.x, rlo.x, st);
         const uint32_t xh = scan_funnel(shi.y, slo.y, ss) ^ scan_funnel(rhi.y, rlo.y, st);
         slo = shi;
         rlo = rhi;
-        return ~(xl | xh);
-    };
-    int c = 0;
-#pragma unroll 2
-    for (; c < nch - 1; ++c) scan_step<false>(z, match(c), c, i0, S, K, win, best);
-    scan_step<false>(z, match(c) & lastmask, c, i0, S, K, win, best);
-    scan_step<true>(z, 0u, c + 1, i0, S, K, win, best);
+        step(~(xl | xh), c);
+    }
+    step(chunk_at(c), c);
+    if (Wp) measure(0u, c + 1);
 }
 
 }  // namespace cd

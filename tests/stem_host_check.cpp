@@ -1,10 +1,13 @@
 // Host check of the engine's stem scan (csrc/cd_scan.h): reads sequences of
 // nucleotide codes from stdin (one per line, digits 0-3, optionally followed
-// by a space and a seed length), runs the scan for every lane of a workgroup
-// of NT lanes in turn (lane t takes anti-diagonals S = 5 + t, 5 + t + NT, ...,
-// as the kernel does) and prints "l p q" of the best stem per sequence.
-// Planes are allocated at exactly their documented size so that the address
-// sanitizer catches any read outside them.
+// by a space and a seed length) and runs the lane loop of the stem kernel
+// (stem_cds in cd_kernels.hip) for PARTS workgroups of NT lanes: lane t of
+// part p takes anti-diagonals S = 5 + p*NT + t + j*PARTS*NT for the trip
+// count of its wave's first lane, and after every trip the 64 lanes of a
+// wave share the largest K, as the kernel's shuffle does.  Prints "l p q" of
+// the best stem per sequence.  Planes are allocated at exactly their
+// documented size so that the address sanitizer catches any read outside
+// them.
 #include <cstdio>
 #include <iostream>
 #include <sstream>
@@ -14,7 +17,8 @@
 #include "cd_scan.h"
 
 int main(int argc, char **argv) {
-    const int NT = argc > 1 ? std::stoi(argv[1]) : 256;
+    const int NT = argc > 1 ? std::stoi(argv[1]) : 256;  // a multiple of 64, or fewer than 64
+    const int PARTS = argc > 2 ? std::stoi(argv[2]) : 1;
     const int MIN_HAIRPIN = 4;
     std::string line;
     while (std::getline(std::cin, line)) {
@@ -37,13 +41,25 @@ int main(int argc, char **argv) {
             rp[i >> 5].y |= (uint32_t)(rc >> 1) << (i & 31);
         }
         uint64_t best = 0;
-        for (int t = 0; t < NT; ++t) {
-            int K = seed > 1 ? seed : 1;
-            uint64_t b = 0;
-            for (int S = MIN_HAIRPIN + 1 + t; S <= 2 * n - 3; S += NT)
-                cd::scan_diagonal(sp.data(), rp.data(), n, S, MIN_HAIRPIN, K, b);
-            if (b > best) best = b;
-        }
+        const int stride = PARTS * NT, last = 2 * n - 3;
+        for (int part = 0; part < PARTS; ++part)
+            for (int w0 = 0; w0 < NT; w0 += 64) {
+                const int lanes = NT - w0 < 64 ? NT - w0 : 64;
+                std::vector<int> K(lanes, seed > 1 ? seed : 1);
+                std::vector<uint64_t> b(lanes, 0);
+                const int Sw = MIN_HAIRPIN + 1 + part * NT + w0;
+                const int trips = Sw <= last ? (last - Sw) / stride + 1 : 0;
+                for (int j = 0; j < trips; ++j) {
+                    int kmax = 0;
+                    for (int l = 0; l < lanes; ++l) {
+                        const int S = MIN_HAIRPIN + 1 + part * NT + w0 + l + j * stride;
+                        if (S <= last) cd::scan_diagonal(sp.data(), rp.data(), n, S, MIN_HAIRPIN, K[l], b[l]);
+                        kmax = K[l] > kmax ? K[l] : kmax;
+                    }
+                    for (int l = 0; l < lanes; ++l) K[l] = kmax;
+                }
+                for (int l = 0; l < lanes; ++l) best = b[l] > best ? b[l] : best;
+            }
         int l = (int)(best >> 40), p = 0, q = 0;
         if (l > 0) {
             p = 0xFFFFF - (int)((best >> 20) & 0xFFFFF);

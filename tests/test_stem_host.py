@@ -53,21 +53,22 @@ def sequences(seed, count, nmax):
     return out
 
 
-def run(harness, seqs, seeds, lanes):
+def run(harness, seqs, seeds, lanes, parts=1):
     inp = "".join("".join(map(str, s)) + f" {k}\n" for s, k in zip(seqs, seeds))
-    out = subprocess.run([harness, str(lanes)], input=inp, capture_output=True, text=True, check=True).stdout
+    out = subprocess.run([harness, str(lanes), str(parts)], input=inp, capture_output=True, text=True,
+                         check=True).stdout
     return [tuple(map(int, o.split())) for o in out.strip().split("\n")]
 
 
-@pytest.mark.parametrize("lanes", [256, 1, 7, 64, 1024])
-def test_stem_scan_matches_oracle(orc, harness, lanes):
-    seqs = [s for s in sequences(lanes, 200, 1600) if len(s) > 0]
+@pytest.mark.parametrize("lanes,parts", [(256, 4), (256, 1), (1, 1), (7, 3), (64, 2), (1024, 1)])
+def test_stem_scan_matches_oracle(orc, harness, lanes, parts):
+    seqs = [s for s in sequences(lanes + parts, 200, 1600) if len(s) > 0]
     want = [orc.sl(s, 0) for s in seqs]
     rng = np.random.default_rng(lanes + 7)
     for seeds in ([0] * len(seqs),
                   [w[0] for w in want],                                     # the exact length
                   [int(rng.integers(0, w[0] + 1)) for w in want]):          # any lower bound
-        got = run(harness, seqs, seeds, lanes)
+        got = run(harness, seqs, seeds, lanes, parts)
         for s, g, w, k in zip(seqs, got, want, seeds):
             assert g == w, (len(s), k)
 
@@ -80,6 +81,6 @@ def test_stem_scan_edge_lengths(orc, harness):
         seqs.append(np.asarray(rng.integers(0, 4, n), np.uint8))
         seqs.append(np.full(n, 0, np.uint8) if n % 2 else np.asarray([0, 3] * (n // 2), np.uint8))
     want = [orc.sl(s, 0) for s in seqs]
-    got = run(harness, seqs, [0] * len(seqs), 256)
+    got = run(harness, seqs, [0] * len(seqs), 256, 4)
     for s, g, w in zip(seqs, got, want):
         assert g == w, len(s)
