@@ -246,6 +246,8 @@ class Server:
     def result(self, ticket):
         """Final pool of a finished request (as Job.read) and its times
         (queued, resident, time to front) in seconds; releases the request."""
+        if ticket not in self._shapes:
+            raise CdError(-1, "unknown ticket")
         L, m, N = self._shapes.pop(ticket)
         P = 2 * N
         out = {"genome": np.zeros((P, m, L), np.int8), "obj": np.zeros((P, 6), np.float32),
