@@ -172,7 +172,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
     o.dom = s.add((size_t)P * W * 8);
     o.ref_of = s.add((size_t)P * 4);
     o.dist = s.add((size_t)P * 4);
-    o.niche = s.add((size_t)6 * R * 4);
+    o.niche = s.add((size_t)8 * R * 4);
     o.flag = s.add((size_t)P);
     o.st = s.add(sizeof(cd::SelState));
     o.self = s.add(sizeof(cd::DevJob));

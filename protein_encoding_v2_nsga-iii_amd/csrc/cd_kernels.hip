@@ -976,20 +976,42 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
                 red_i[wid * NOBJ + a] = bi[a];
             }
         __syncthreads();
-        if (tid == 0)
+        // The rows an extreme can end up holding are the previous extremes
+        // (candidates 0-5) and the pool minima of the six axes (6-11): their
+        // ASF values for every axis are computed in parallel, then the axes
+        // are settled in order with lookups.
+        __shared__ float pv[NOBJ], crow[2 * NOBJ][NOBJ], casf[2 * NOBJ][NOBJ];
+        __shared__ int pid[NOBJ];
+        if (tid < NOBJ) {
+            float v = red_v[tid];
+            int id = red_i[tid];
+            for (int w = 1; w < nw; ++w) argmin_pair(v, id, red_v[w * NOBJ + tid], red_i[w * NOBJ + tid]);
+            pv[tid] = v;
+            pid[tid] = id;
+        }
+        __syncthreads();
+        if (tid < 2 * NOBJ * NOBJ) {
+            const int c = tid / NOBJ, k = tid % NOBJ;
+            if (c < NOBJ)
+                crow[c][k] = ext[c * NOBJ + k];
+            else
+                crow[c][k] = pid[c - NOBJ] < P ? obj[(size_t)pid[c - NOBJ] * NOBJ + k] : 0.f;
+        }
+        __syncthreads();
+        if (tid < 2 * NOBJ * NOBJ) casf[tid / NOBJ][tid % NOBJ] = asf(crow[tid / NOBJ], ideal, tid % NOBJ);
+        __syncthreads();
+        if (tid == 0) {
+            int cur[NOBJ];  // candidate row now held by each extreme slot
+            for (int t = 0; t < NOBJ; ++t) cur[t] = t;
             for (int axis = 0; axis < NOBJ; ++axis) {
-                float v = red_v[axis];
-                int id = red_i[axis];
-                for (int w = 1; w < nw; ++w) argmin_pair(v, id, red_v[w * NOBJ + axis], red_i[w * NOBJ + axis]);
-                for (int t = 0; t < NOBJ; ++t) argmin_pair(v, id, asf(ext + t * NOBJ, ideal, axis), P + t);
-                float win[NOBJ];
-                if (id != 0x7FFFFFFF && v < FLT_MAX)
-                    for (int k = 0; k < NOBJ; ++k)
-                        win[k] = (id >= P) ? ext[(id - P) * NOBJ + k] : obj[(size_t)id * NOBJ + k];
-                else
-                    for (int k = 0; k < NOBJ; ++k) win[k] = ext[axis * NOBJ + k];
-                for (int k = 0; k < NOBJ; ++k) ext[axis * NOBJ + k] = win[k];
+                float v = pv[axis];
+                int id = pid[axis];
+                for (int t = 0; t < NOBJ; ++t) argmin_pair(v, id, casf[cur[t]][axis], P + t);
+                if (id != 0x7FFFFFFF && v < FLT_MAX) cur[axis] = (id >= P) ? cur[id - P] : NOBJ + axis;
             }
+            for (int t = 0; t < NOBJ; ++t)
+                for (int k = 0; k < NOBJ; ++k) ext[t * NOBJ + k] = crow[cur[t]][k];
+        }
     }
     if (tid == 0) {
         SelState *st = J.st;
@@ -1097,8 +1119,8 @@ __device__ __forceinline__ int block_sum_int(int v, int *red) {
 // survivor list in ascending pool index.  Ends the generation.  Dynamic LDS
 // (niche_lds_need) holds the last front bucketed by reference point (design
 // indices [P], high words of their priorities [P]) and, when they fit, the
-// per-reference-point arrays and copies of front_of, ref_of and the flags;
-// otherwise those stay in the job's global scratch.
+// bucket's distances, the per-reference-point arrays and copies of front_of,
+// ref_of and the flags; otherwise those stay in the job's global scratch.
 __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t dyn[];
     const DevJob &J = jobs[blockIdx.x];
@@ -1109,15 +1131,18 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     __shared__ int red[16];
     __shared__ int s_carry;
     const int tier = niche_tier(P, R);
+    const int Pa = niche_stride(P);
     int *lb = (int *)dyn;
-    uint32_t *lp = (uint32_t *)(lb + P);
-    int *cnt = tier >= 1 ? (int *)(lp + P) : J.niche;
+    uint32_t *lp = (uint32_t *)(lb + Pa);
+    float *ld = tier >= 1 ? (float *)(lp + Pa) : nullptr;  // bucket distances when staged
+    int *cnt = tier >= 1 ? (int *)(ld + Pa) : J.niche;
     int *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R, *lvl0 = kp + R;
+    uint64_t *thr = (uint64_t *)(lvl0 + R);  // spare u64 [R] (keeps the layout's alignment)
     const int *fo = J.front_of, *ro = J.ref_of;
     uint8_t *flag = J.flag;
     if (tier >= 2) {
-        int *fo2 = (int *)(lp + P) + 6 * R, *ro2 = fo2 + P;
-        uint8_t *fl2 = (uint8_t *)(ro2 + P);
+        int *fo2 = (int *)(thr + R), *ro2 = fo2 + Pa;
+        uint8_t *fl2 = (uint8_t *)(ro2 + Pa);
         for (int d = tid; d < P; d += NT) {
             fo2[d] = J.front_of[d];
             ro2[d] = J.ref_of[d];
@@ -1148,12 +1173,14 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
             if (tid == 0) s_carry += total;
             __syncthreads();
         }
+        const int total_last = s_carry;  // designs of the last front
         for (int d = tid; d < P; d += NT) {
             if (fo[d] == last) {
                 const int r = ro[d];
                 const int slot = off[r] + atomicAdd(&fill[r], 1);
                 lb[slot] = d;
                 lp[slot] = (uint32_t)(niche_prio(J, gen, d) >> 32);
+                if (ld) ld[slot] = J.dist[d];
             }
         }
         __syncthreads();
@@ -1207,48 +1234,60 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
             if (tid == 0) s_carry += total;
             __syncthreads();
         }
-        for (int r = tid; r < R; r += NT) {  // level 0 of an empty niche takes its nearest candidate
-            int l0 = -1;
-            if (kp[r] > 0 && cnt[r] == 0) {
-                float bd = FLT_MAX;
-                const int *b = lb + off[r];
-                for (int t = 0; t < rem[r]; ++t) {
-                    const int e = b[t];
-                    const float de = J.dist[e];
-                    if (l0 < 0 || de < bd || (de == bd && e < l0)) {
-                        bd = de;
-                        l0 = e;
+        // level 0 of an empty niche takes its nearest candidate (ties to the
+        // lowest index): one wave per reference point with picks
+        {
+            const int lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+            for (int r = wid; r < R; r += nw) {
+                int l0 = -1;
+                if (kp[r] > 0 && cnt[r] == 0) {
+                    const int *b = lb + off[r];
+                    const int nb = rem[r];
+                    float bd = FLT_MAX;
+                    int bi = 0x7FFFFFFF;
+                    for (int t = lane; t < nb; t += 64) argmin_pair(bd, bi, ld ? ld[off[r] + t] : J.dist[b[t]], b[t]);
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        const float od = __shfl_xor(bd, o, 64);
+                        const int oi = __shfl_xor(bi, o, 64);
+                        argmin_pair(bd, bi, od, oi);
                     }
+                    // every distance FLT_MAX gives the lowest index, as a sweep in order would
+                    l0 = bi != 0x7FFFFFFF ? bi : b[0];
                 }
+                if (lane == 0) lvl0[r] = l0;
             }
-            lvl0[r] = l0;
+        }
+        for (int d = tid; d < P; d += NT) {
+            const int f = fo[d];
+            flag[d] = (f >= 0 && f < last);
         }
         __syncthreads();
-        for (int d = tid; d < P; d += NT) {  // later levels: the kk smallest priorities of the bucket
-            const int f = fo[d];
-            uint8_t sel = (f >= 0 && f < last);
-            if (f == last) {
-                const int r = ro[d], l0 = lvl0[r];
-                if (d == l0) {
-                    sel = 1;
-                } else {
-                    const int kk = kp[r] - (l0 >= 0 ? 1 : 0);
-                    if (kk > 0) {
-                        const uint64_t my = niche_prio(J, gen, d);
-                        const int *b = lb + off[r];
-                        const uint32_t *bp = lp + off[r];
-                        const int nb = rem[r];
-                        int rank = 0;
-                        for (int t = 0; t < nb; ++t) {
-                            const int e = b[t];
-                            if (e == d || e == l0) continue;
-                            rank += (((uint64_t)bp[t] << 32) | (uint32_t)e) < my;
-                        }
-                        sel = rank < kk;
+        // later levels take the kk smallest priorities of the rest of the
+        // bucket: one thread per bucket slot (consecutive slots share a bucket,
+        // so a wave's loop is as long as its own buckets)
+        for (int sl = tid; sl < total_last; sl += NT) {
+            const int d = lb[sl], r = ro[d], l0 = lvl0[r];
+            uint8_t sel = (d == l0);
+            if (!sel) {
+                const int kk = kp[r] - (l0 >= 0 ? 1 : 0);
+                const int nb = rem[r] - (l0 >= 0 ? 1 : 0);
+                if (kk >= nb) {
+                    sel = kk > 0;
+                } else if (kk > 0) {
+                    const uint64_t my = ((uint64_t)lp[sl] << 32) | (uint32_t)d;
+                    const int *b = lb + off[r];
+                    const uint32_t *bp = lp + off[r];
+                    int rank = 0;
+                    for (int u = 0; u < rem[r] && rank < kk; ++u) {
+                        const int e = b[u];
+                        if (e == d || e == l0) continue;
+                        rank += (((uint64_t)bp[u] << 32) | (uint32_t)e) < my;
                     }
+                    sel = rank < kk;
                 }
             }
-            flag[d] = sel;
+            if (sel) flag[d] = 1;
         }
     } else {
         __syncthreads();

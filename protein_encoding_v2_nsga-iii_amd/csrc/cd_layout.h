@@ -51,20 +51,28 @@ struct SelState {
     float ideal[NOBJ], worst[NOBJ], nadir[NOBJ];
 };
 
-// Dynamic LDS of k_niche: the bucketed last front (8P bytes) always; the six
-// per-reference-point arrays (24R bytes) when they fit beside it; and copies
-// of front_of, ref_of and the survivor flags (9P bytes) when all fit.
+// Dynamic LDS of k_niche: the bucketed last front (design indices and
+// priority high words, 8P bytes) always; with tier >= 1 also the bucket's
+// distances (4P) and the per-reference-point arrays (six int arrays and one
+// u64 array, 32R); with tier 2 also copies of front_of, ref_of and the
+// survivor flags (9P).  What is not in LDS lives in the job's global scratch.
 constexpr size_t NICHE_LDS_CAP = 150 * 1024;
 
+// P-sized arrays are laid out with an even stride so that the u64 array that
+// follows them stays 8-byte aligned.
+__host__ __device__ inline int niche_stride(int P) { return (P + 1) & ~1; }
+
 __host__ __device__ inline int niche_tier(int P, int R) {
-    if ((size_t)17 * P + (size_t)24 * R + 16 <= NICHE_LDS_CAP) return 2;
-    if ((size_t)8 * P + (size_t)24 * R <= NICHE_LDS_CAP) return 1;
+    const size_t Pa = niche_stride(P);
+    if (21 * Pa + (size_t)32 * R + 16 <= NICHE_LDS_CAP) return 2;
+    if (12 * Pa + (size_t)32 * R <= NICHE_LDS_CAP) return 1;
     return 0;
 }
 
 __host__ __device__ inline size_t niche_lds_need(int P, int R) {
     const int t = niche_tier(P, R);
-    return (size_t)8 * P + (t >= 1 ? (size_t)24 * R : 0) + (t >= 2 ? (size_t)9 * P + 16 : 0);
+    const size_t Pa = niche_stride(P);
+    return 8 * Pa + (t >= 1 ? 4 * Pa + (size_t)32 * R : 0) + (t >= 2 ? 9 * Pa + 16 : 0);
 }
 
 // Device view of one job.  Pools ping-pong on generation parity: generation
@@ -93,7 +101,7 @@ struct DevJob {
     uint64_t *dom;         // [P][W]: bit i of row j set when i dominates j
     int32_t *ref_of;       // [P]
     float *dist;           // [P]
-    int32_t *niche;        // [6][R]: count, rem, off, fill, kpick, level-0 pick
+    int32_t *niche;        // [8][R]: count, rem, off, fill, kpick, level-0 pick, then u64 [R] pick threshold
     uint8_t *flag;         // [P]
     SelState *st;
 };
