@@ -34,6 +34,10 @@ PKG = "protein_encoding_v2_nsga-iii_amd"
 VALU_INT32_OPS = 256 * 64 * 2.4e9
 PEAK_CELLS = 32 * VALU_INT32_OPS
 KERNELS = ("offspring", "stem", "stem_finish", "dominance", "fronts", "associate", "niche")
+# HBM bytes per k_stem launch at the default workload: FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000 (scaled by a 256 MiB device copy in the same run) from
+# the rocprofv3 --pmc runs summarised in profiles/pmc_single_job_L501_m4_N1024.md
+K_STEM_TRAFFIC_BYTES = int((2564.8 * 1.9998 + 111.6) * 1024)
 
 
 def log(msg):
@@ -102,6 +106,7 @@ def main():
         return
     cdp, aa, tab = workload(args)
     L = len(aa)
+    default_workload = (args.residues, args.m, args.N, args.H, args.Pm, args.seed) == (500, 4, 1024, 5, 0.1, 2)
     job = cdp.Job(aa, tab, args.m, args.N, args.H, args.Pm, args.seed).init()
     log(f"job created and initialised (L={L}, m={args.m}, N={args.N}, H={args.H})")
     job.step(args.warmup)
@@ -140,7 +145,8 @@ def main():
         "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in prof])),
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4), "peak": round(PEAK_CELLS / 1e12, 2),
                      "unit": "T stem cells/s (peak: 32 cells per int32 lane-op)",
-                     "frac": round(achieved / PEAK_CELLS, 5), "traffic": None, "kernel": "k_stem",
+                     "frac": round(achieved / PEAK_CELLS, 5),
+                     "traffic": K_STEM_TRAFFIC_BYTES if default_workload else None, "kernel": "k_stem",
                      "cells_per_launch": int(cells)},
     }
     if args.batched_gens > 0:
