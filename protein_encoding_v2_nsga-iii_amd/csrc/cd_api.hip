@@ -275,6 +275,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
         J->d_off = (int2 *)(base + o.off);
         J->lds = cd::design_lds_bytes(J->L, J->m);
         J->stem_lds = cd::stem_lds_bytes(J->L);
+        J->stem_parts = cd::stem_parts_of(J->L);
     }
     auto ref = das_dennis(d->H);
     std::memcpy(&host[o.ref], ref.data(), ref.size() * sizeof(float));
@@ -319,7 +320,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
 // the stem of every changed CDS, objective 5.
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
     hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
-    if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_lds, s);
+    if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
     if (e == hipSuccess) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
     return e;
 }
@@ -357,6 +358,7 @@ void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
         wl.stem_lds = std::max(wl.stem_lds, J->stem_lds);
         wl.niche_lds = std::max(wl.niche_lds, J->niche_lds);
         wl.mg = std::max(wl.mg, J->m);
+        wl.stem_parts = std::max(wl.stem_parts, J->stem_parts);
     }
 }
 
@@ -387,7 +389,8 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 }
 
 Launch job_launch(cd_job *J) {
-    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds, J->stem_lds, J->niche_lds, J->m};
+    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds, J->stem_lds, J->niche_lds, J->m,
+                  J->stem_parts};
 }
 
 }  // namespace
@@ -505,7 +508,8 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(hipEventRecord(ev[0], J->stream));
         HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream));
         HIPCHK(hipEventRecord(ev[1], J->stream));
-        HIPCHK(cd::launch_stem(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->m, J->stem_lds, J->stream));
+        HIPCHK(cd::launch_stem(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->m, J->stem_parts, J->stem_lds,
+                               J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
         HIPCHK(cd::launch_stem_finish(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));
@@ -674,6 +678,7 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     B->stem_lds = wl.stem_lds;
     B->niche_lds = wl.niche_lds;
     B->mg = wl.mg;
+    B->stem_parts = wl.stem_parts;
     B->n_off = (int)wl.off.size();
     B->n_dom = (int)wl.dom.size();
     B->n_asc = (int)wl.asc.size();
@@ -699,7 +704,7 @@ int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
         const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom, B->n_dom, B->d_asc, B->n_asc,
-                       B->lds, B->stem_lds, B->niche_lds, B->mg};
+                       B->lds, B->stem_lds, B->niche_lds, B->mg, B->stem_parts};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }

@@ -24,6 +24,7 @@ struct cd_job {
     int2 *d_asc = nullptr;
     int n_dom = 0, n_asc = 0;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
+    int stem_parts = 1;  // k_stem workgroups per CDS
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool inited = false;
@@ -38,7 +39,8 @@ struct cd_batch {
     int2 *d_asc = nullptr;
     int n_off = 0, n_dom = 0, n_asc = 0;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    int mg = 0;  // largest m of the batch's jobs
+    int mg = 0;          // largest m of the batch's jobs
+    int stem_parts = 0;  // largest k_stem workgroups per CDS of the batch's jobs
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -70,7 +72,8 @@ struct Launch {
     const int2 *asc;  // (job, first design) association groups
     int n_asc;
     size_t lds, stem_lds, niche_lds;
-    int mg;  // largest m
+    int mg;          // largest m
+    int stem_parts;  // largest k_stem workgroups per CDS
 };
 
 // Host images of the work lists of `jobs` (job k = index k of the list)
@@ -80,7 +83,7 @@ struct WorkLists {
     std::vector<int2> off, asc;
     std::vector<int4> dom;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    int mg = 0;
+    int mg = 0, stem_parts = 0;
 };
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl);

@@ -12,17 +12,20 @@ enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
 
 constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
 constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (one wave each)
-constexpr int STEM_PARTS = 4;              // k_stem workgroups per CDS (interleaved anti-diagonal blocks)
+constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
 
 size_t design_lds_bytes(int L, int m);
 size_t stem_lds_bytes(int L);
+int stem_parts_of(int L);  // k_stem workgroups per CDS of L codons (1 .. STEM_PARTS)
 hipError_t set_design_lds_limit(size_t bytes);
 hipError_t set_stem_lds_limit(size_t bytes);
 
 // items: (job, design or child)
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s);
-// STEM_PARTS workgroups per (item, CDS c < mg): stem of each CDS k_design marked
-hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, size_t lds, hipStream_t s);
+// parts workgroups per (item, CDS c < mg), parts the largest stem_parts_of of
+// the jobs: stem of each CDS k_design marked
+hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,
+                       hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
 // items: (job, first row, word, 0)

@@ -3,8 +3,8 @@
 // One generation is seven launches over every resident job:
 //   k_design<OFFSPRING>  one workgroup per child: survivor copy, mutation,
 //                        objectives 0-4 (mutationKernel, main.cu:121-240)
-//   k_stem               STEM_PARTS workgroups per (child, CDS): longest stem
-//                        of each CDS the mutation changed (calMaximumSL)
+//   k_stem               1..STEM_PARTS workgroups per (child, CDS): longest
+//                        stem of each CDS the mutation changed (calMaximumSL)
 //   k_stem_finish        one thread per child: objective 5
 //   k_dominance          bit-packed eps-domination matrix (sorting.cuh:994-1023)
 //   k_fronts             one workgroup per job: front peeling, ideal/worst,
@@ -21,6 +21,7 @@
 #include "cd_device.h"
 #include "cd_kernels.h"
 #include "cd_scan.h"
+#include "cd_tile.h"
 
 namespace cd {
 
@@ -50,9 +51,11 @@ struct DesignShared {
 // ---------------------------------------------------------------------------
 // Longest stem of one CDS held as nucleotide codes in LDS (calMaximumSL,
 // utils.cuh:712-888): the sequence and its reverse complement are packed into
-// bit planes with wave ballots, then each thread scans the anti-diagonals
-// S = 5 + (part + parts * j) * blockDim + tid bit-parallel (cd_scan.h) and the
-// workgroup takes the maximum key (longest, then smallest p, then smallest q).
+// bit planes with wave ballots; the lanes of the `parts` workgroups of the
+// CDS split its tiles (cd_tile.h) into contiguous ranges, and the workgroup
+// takes the maximum key (longest, then smallest p, then smallest q).  A CDS
+// with no run of 4 and no seed of 4 is scanned exactly along its
+// anti-diagonals (cd_scan.h).
 // seed_from, when given, is the parent's record of this CDS: the length of
 // the parent's stem as it stands in the child is a stem the child has, so the
 // scan starts with that window.
@@ -85,22 +88,51 @@ __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const Cds
     }
     if (tid == 0) *bcast = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q) : 0;
     __syncthreads();
-    int K = max(*bcast, 1);
-    uint64_t best = 0;
     // Every lane's K is a length some stem of this CDS has, so the wave may
-    // share the largest after each anti-diagonal.  The trip count is the
-    // wave's first lane's, so every lane takes part in the exchange (lanes
-    // past the last anti-diagonal scan empty ones, which return at once).
-    const int stride = parts * NT, S0 = MIN_HAIRPIN + 1 + part * NT + tid;
-    const int Sw = MIN_HAIRPIN + 1 + part * NT + (tid & ~63), last = 2 * n - 3;
-    const int trips = Sw <= last ? (last - Sw) / stride + 1 : 0;
-    for (int j = 0; j < trips; ++j) {
-        const int S = S0 + j * stride;
-        if (S <= last) scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
+    // share the largest after each step.  Trip counts are the same for every
+    // lane, so every lane takes part in the exchanges.
+    int K = max(*bcast, TILE_MIN_RUN);
+    uint64_t best = 0;
+    {
+        const int TT = 2 * tile_prefix(n, tile_blocks(n)), T = parts * NT, L = part * NT + tid;
+        const int t0 = (int)((int64_t)L * TT / T), t1 = (int)((int64_t)(L + 1) * TT / T);
+        const int trips = (TT + T - 1) / T;
+        TileCursor cur;
+        for (int j = 0; j < trips; ++j) {
+            if (t0 + j < t1) {
+                if (j == 0) tile_seek(sp, rp, n, t0, cur);
+                tile_step(sp, rp, n, cur, K, best);
+            }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) K = max(K, __shfl_xor(K, o, 64));
+            for (int o = 32; o > 0; o >>= 1) K = max(K, __shfl_xor(K, o, 64));
+        }
     }
-    return block_max_u64(best, redu);
+    best = block_max_u64(best, redu);
+    // With a seed of 4 or more the longest stem is at least 4 long, so the
+    // share holding it measures it; a share without a flag holds nothing
+    // longer.  Without one, a share with no run of 4 is scanned exactly.
+    if (*bcast < TILE_MIN_RUN && (int)(best >> 40) < TILE_MIN_RUN) {
+        best = 0;
+        K = 1;
+        const int stride = parts * NT, S0 = MIN_HAIRPIN + 1 + part * NT + tid;
+        const int Sw = MIN_HAIRPIN + 1 + part * NT + (tid & ~63), last = 2 * n - 3;
+        const int trips = Sw <= last ? (last - Sw) / stride + 1 : 0;
+        for (int j = 0; j < trips; ++j) {
+            const int S = S0 + j * stride;
+            if (S <= last) scan_diagonal(sp, rp, n, S, MIN_HAIRPIN, K, best);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) K = max(K, __shfl_xor(K, o, 64));
+        }
+        best = block_max_u64(best, redu);
+    }
+    return best;
+}
+
+// Workgroups of k_stem per CDS of n bases: about three tiles per lane or
+// more, at most STEM_PARTS.
+__host__ __device__ inline int stem_parts(int n) {
+    const int TT = 2 * tile_prefix(n, tile_blocks(n)), p = TT / (3 * 256);
+    return p < 1 ? 1 : (p > STEM_PARTS ? STEM_PARTS : p);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,24 +687,28 @@ __device__ __forceinline__ void item_slot(const DevJob &J, int mode, int y, int 
 }
 
 // ---------------------------------------------------------------------------
-// k_stem: STEM_PARTS workgroups per (work item, CDS); blockIdx.x =
-// (item * mg + c) * STEM_PARTS + part with mg the largest m of the jobs
-// launched.  CDS not marked by k_design (pad 0) are skipped.  The nucleotide
+// k_stem: up to STEM_PARTS workgroups per (work item, CDS); blockIdx.x =
+// part * nic + item * mg + c with mg the largest m of the jobs launched and
+// nic = items * mg.  Part-major order keeps the workgroups that do work
+// consecutive, so the round-robin placement of workgroups on the XCDs
+// spreads them over all eight (a CDS-major order with unused parts would
+// leave the working ones on the XCDs of one residue class).  CDS not marked
+// by k_design (pad 0) are skipped.  The nucleotide
 // codes are rebuilt from the genome row, the parent's stem seeds the scan
 // (children only), and each part folds its best key into the record with an
 // atomic maximum; k_stem_finish decodes it.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, const int2 *__restrict__ items, int mode,
-                                              int mg) {
+                                              int mg, int nic) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];
     __shared__ int32_t bcast;
-    const int part = (int)(blockIdx.x % STEM_PARTS), ic = (int)(blockIdx.x / STEM_PARTS);
+    const int part = (int)(blockIdx.x / (unsigned)nic), ic = (int)(blockIdx.x % (unsigned)nic);
     const int2 it = items[ic / mg];
     const int c = ic % mg;
     const DevJob &J = jobs[it.x];
-    const int L = J.L, m = J.m, n = 3 * L;
-    if (c >= m) return;
+    const int L = J.L, m = J.m, n = 3 * L, parts = stem_parts(n);
+    if (c >= m || part >= parts) return;
     int buf, slot;
     item_slot(J, mode, it.y, buf, slot);
     CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
@@ -688,7 +724,7 @@ __global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, c
     }
     __syncthreads();
     const uint64_t key =
-        stem_cds(ntb, n, planes, mode == MODE_OFFSPRING ? cs : nullptr, part, STEM_PARTS, &bcast, redu);
+        stem_cds(ntb, n, planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, &bcast, redu);
     if (threadIdx.x == 0 && key) atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
 }
 
@@ -1343,9 +1379,13 @@ hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n,
     }
     return hipGetLastError();
 }
-hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, size_t lds, hipStream_t s) {
+int stem_parts_of(int L) { return stem_parts(3 * L); }
+
+hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,
+                       hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stem, dim3(n * mg * STEM_PARTS), dim3(256), lds, s, jobs, items, mode, mg);
+    if (parts < 1 || parts > STEM_PARTS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stem, dim3(n * mg * parts), dim3(256), lds, s, jobs, items, mode, mg, n * mg);
     return hipGetLastError();
 }
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s) {

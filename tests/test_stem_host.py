@@ -1,7 +1,8 @@
-"""The engine's stem scan (csrc/cd_scan.h), compiled for the host with the
-address sanitizer and run lane by lane, against the oracle's closed form
-(with its tie rule) on random, repetitive, skewed and planted-hairpin
-sequences, with and without a seed (a stem length known to exist)."""
+"""The engine's stem scans (csrc/cd_scan.h along anti-diagonals, csrc/cd_tile.h
+across them), compiled for the host with the address sanitizer and run lane
+by lane as the stem kernel runs them, against the oracle's closed form (with
+its tie rule) on random, repetitive, skewed and planted-hairpin sequences,
+with and without a seed (a stem length known to exist)."""
 import os
 import shutil
 import subprocess
@@ -53,27 +54,31 @@ def sequences(seed, count, nmax):
     return out
 
 
-def run(harness, seqs, seeds, lanes, parts=1):
+def run(harness, seqs, seeds, lanes, parts=1, mode="diag"):
     inp = "".join("".join(map(str, s)) + f" {k}\n" for s, k in zip(seqs, seeds))
-    out = subprocess.run([harness, str(lanes), str(parts)], input=inp, capture_output=True, text=True,
+    out = subprocess.run([harness, str(lanes), str(parts), mode], input=inp, capture_output=True, text=True,
                          check=True).stdout
     return [tuple(map(int, o.split())) for o in out.strip().split("\n")]
 
 
-@pytest.mark.parametrize("lanes,parts", [(256, 4), (256, 1), (1, 1), (7, 3), (64, 2), (1024, 1)])
-def test_stem_scan_matches_oracle(orc, harness, lanes, parts):
+@pytest.mark.parametrize("mode,lanes,parts", [("diag", 256, 4), ("diag", 256, 1), ("diag", 1, 1), ("diag", 7, 3),
+                                              ("diag", 64, 2), ("diag", 1024, 1), ("tile", 256, 1), ("tile", 64, 1),
+                                              ("tile", 256, 4), ("tile", 7, 1), ("tile", 1, 1),
+                                              ("tile", 7, 3), ("tile", 64, 2)])
+def test_stem_scan_matches_oracle(orc, harness, mode, lanes, parts):
     seqs = [s for s in sequences(lanes + parts, 200, 1600) if len(s) > 0]
     want = [orc.sl(s, 0) for s in seqs]
     rng = np.random.default_rng(lanes + 7)
     for seeds in ([0] * len(seqs),
                   [w[0] for w in want],                                     # the exact length
                   [int(rng.integers(0, w[0] + 1)) for w in want]):          # any lower bound
-        got = run(harness, seqs, seeds, lanes, parts)
+        got = run(harness, seqs, seeds, lanes, parts, mode)
         for s, g, w, k in zip(seqs, got, want, seeds):
             assert g == w, (len(s), k)
 
 
-def test_stem_scan_edge_lengths(orc, harness):
+@pytest.mark.parametrize("mode", ["diag", "tile"])
+def test_stem_scan_edge_lengths(orc, harness, mode):
     # every length around the 32-position chunk boundaries and the hairpin limit
     rng = np.random.default_rng(3)
     seqs = []
@@ -81,6 +86,6 @@ def test_stem_scan_edge_lengths(orc, harness):
         seqs.append(np.asarray(rng.integers(0, 4, n), np.uint8))
         seqs.append(np.full(n, 0, np.uint8) if n % 2 else np.asarray([0, 3] * (n // 2), np.uint8))
     want = [orc.sl(s, 0) for s in seqs]
-    got = run(harness, seqs, [0] * len(seqs), 256, 4)
+    got = run(harness, seqs, [0] * len(seqs), 256, 4 if mode == "diag" else 2, mode)
     for s, g, w in zip(seqs, got, want):
         assert g == w, len(s)
