@@ -111,6 +111,20 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     return v;
 }
 
+// Largest v >= 0 over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8
+// leave each row's maximum in its lane 15, the row broadcasts of lanes 15
+// and 31 carry it across rows, and lane 63 ends with the wave's maximum.
+// Every lane of the wave must be active.
+__device__ __forceinline__ int wave_max_nonneg(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Sum of up to 4 int64 values over the workgroup; every thread gets the
 // totals.  red must hold 4*(blockDim/64) entries.
 template <int NV>

@@ -60,12 +60,21 @@ struct DesignShared {
 // the parent's stem as it stands in the child is a stem the child has, so the
 // scan starts with that window.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p, int q) {
-    if (l <= 0) return 0;
+// Wave 0: the longest run of pairs (i - k, j + k), k < 64, that still pair
+// in the child, with (i, j) = (p + l - 1, q) the parent's innermost pair.
+// Each such run is a stem of the child (its pairs lie on one anti-diagonal
+// and enclose at least the parent's loop), so its length bounds the longest
+// stem from below; only its value up to 8 matters to the scan.
+__device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p, int q, int lane) {
     const int i = p + l - 1, j = q;
-    int k = 0;
-    while (i - k >= 0 && j + k < n && ((ntb[i - k] ^ ntb[j + k]) == 3)) ++k;
-    return k;
+    const bool pair = l > 0 && lane <= i && j + lane < n && ((ntb[i - lane] ^ ntb[j + lane]) == 3);
+    uint64_t m = __ballot(pair);
+    int run = 0;
+    while (m) {  // each round removes the last pair of every run
+        ++run;
+        m &= m >> 1;
+    }
+    return run;
 }
 
 __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, int part, int parts,
@@ -86,7 +95,10 @@ __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const Cds
             rp[w + 1] = make_uint2((uint32_t)(rl >> 32), (uint32_t)(rh >> 32));
         }
     }
-    if (tid == 0) *bcast = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q) : 0;
+    if (wid == 0) {
+        const int seed = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q, lane) : 0;
+        if (lane == 0) *bcast = seed;
+    }
     __syncthreads();
     // Every lane's K is a length some stem of this CDS has, so the wave may
     // share the largest after each step.  Trip counts are the same for every
@@ -103,8 +115,7 @@ __device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const Cds
                 if (j == 0) tile_seek(sp, rp, n, t0, cur);
                 tile_step(sp, rp, n, cur, K, best);
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) K = max(K, __shfl_xor(K, o, 64));
+            K = wave_max_nonneg(K);
         }
     }
     best = block_max_u64(best, redu);

@@ -12,16 +12,19 @@
 // positions (rows).  Per row, the 32 match bits are two funnel shifts of the
 // sliding window, the row's own base broadcast to all bits, and three logic
 // operations on the 2-bit planes; a doubling chain over rows (windows of 2,
-// 4 matches, history in registers) marks the rows at which a window of
-// W = min(K, 8) matches ends, K the lane's best length so far (at least 4).
+// 4 matches, history in registers) marks the rows at which a window of W
+// matches ends: W = 8 once the lane's best length so far K is at least 8,
+// 7 at K = 7, else 4 (K is at least 4).
 // Diagonals so flagged in a tile are measured exactly from the planes, 64
 // rows at a time: every maximal run meeting the tile whose length is at least W,
 // extended past the tile as far as it goes.  So every run of length
 // >= min(K, 8) is measured, and with K never above the longest stem the
 // maximum of the keys is exact including the tie rule (longest, then
-// smallest p, then smallest q).  The window follows K exactly up to 8, so
-// flags stay rare once K nears the longest stem.  A CDS without any run of
-// 4 is left to an exact scan (the caller checks the maximum).
+// smallest p, then smallest q).  The windows 7 and 8 keep flags rare once K
+// nears the longest stem; the windows 5 and 6 would need longer histories
+// of the match and 2-window rows (more registers than they save).  A CDS
+// without any run of 4 is left to an exact scan (the caller checks the
+// maximum).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -152,14 +155,13 @@ __host__ __device__ inline void tile_hist_reset(TileHist &t) {
 
 // Rows u0 + J0 .. u0 + 31 of block (h, base): X is the plane read along the
 // diagonals (rc for half A, s for half B), Y the plane of the row's own base.
-// Sets f[e] to the diagonals with a window of 4 + e matches (e = 0..4)
-// ending in these rows and updates the history.  A window of 4 + e (e >= 1)
-// is a 4-window and the 1-, 2-, 3- (a 4-window one row later) or 4-window
-// ending four rows earlier.  All five are kept in one pass over the rows
-// (one three-input logic operation each per row).
+// Sets f[0], f[1], f[2] to the diagonals with a window of 4, 7, 8 matches
+// ending in these rows and updates the history.  A 7-window is a 4-window
+// and the 4-window ending three rows earlier, an 8-window the one ending
+// four rows earlier (one three-input logic operation each per row).
 template <int J0>
 __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int base, int u0, TileHist &t,
-                                          uint32_t f[5]) {
+                                          uint32_t f[3]) {
     const int w = (u0 + base) >> 5;
     const uint2 x0 = X[w], x1 = X[w + 1], y = Y[u0 >> 5];
     uint32_t M[36], A2[36], A4[36];  // index r + 4 holds row u0 + J0 + r; 0..3 the history
@@ -169,7 +171,7 @@ __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int ba
         A2[i] = t.a2[i];
         A4[i] = t.a4[i];
     }
-    uint32_t f4 = 0u, f5 = 0u, f6 = 0u, f7 = 0u, f8 = 0u;
+    uint32_t f4 = 0u, f7 = 0u, f8 = 0u;
 #pragma unroll
     for (int j = J0; j < 32; ++j) {
         const int a = j - J0 + 4;
@@ -180,8 +182,6 @@ __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int ba
         A2[a] = m & M[a - 1];
         A4[a] = A2[a] & A2[a - 2];
         f4 |= A4[a];
-        f5 |= A4[a] & M[a - 4];
-        f6 |= A4[a] & A2[a - 4];
         f7 |= A4[a] & A4[a - 3];
         f8 |= A4[a] & A4[a - 4];
     }
@@ -193,10 +193,8 @@ __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int ba
         t.a4[i] = A4[cnt + i];
     }
     f[0] = f4;
-    f[1] = f5;
-    f[2] = f6;
-    f[3] = f7;
-    f[4] = f8;
+    f[1] = f7;
+    f[2] = f8;
 }
 
 // One lane's share of the tiled pass: the flat tile range [t0, t1) over half
@@ -227,7 +225,7 @@ __host__ __device__ inline void tile_seek(const uint2 *sp, const uint2 *rp, int 
     tile_hist_reset(c.hist);
     if (c.g > 0) {  // the last 8 rows of the previous tile of the block
         const uint2 *X = c.h ? sp : rp, *Y = c.h ? rp : sp;
-        uint32_t f[5];
+        uint32_t f[3];
         tile_rows<24>(X, Y, 32 * c.b, 32 * (c.g - 1), c.hist, f);
     }
 }
@@ -237,10 +235,10 @@ __host__ __device__ inline void tile_step(const uint2 *sp, const uint2 *rp, int 
                                           uint64_t &best) {
     const uint2 *X = c.h ? sp : rp, *Y = c.h ? rp : sp;
     const int base = 32 * c.b, u0 = 32 * c.g;
-    const int W = K < 8 ? K : 8;  // K >= TILE_MIN_RUN
-    uint32_t fw[5];
+    const int W = K >= 8 ? 8 : (K == 7 ? 7 : TILE_MIN_RUN);  // K >= TILE_MIN_RUN
+    uint32_t fw[3];
     tile_rows<0>(X, Y, base, u0, c.hist, fw);
-    uint32_t f = W == 4 ? fw[0] : W == 5 ? fw[1] : W == 6 ? fw[2] : W == 7 ? fw[3] : fw[4];
+    uint32_t f = W == 8 ? fw[2] : (W == 7 ? fw[1] : fw[0]);
     while (f) {
         const int k = __builtin_ctz(f);
         f &= f - 1u;
