@@ -111,6 +111,37 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     return v;
 }
 
+// Sums over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8 give each
+// lane the sum of its row up to itself (lane 15 the row's sum), the row
+// broadcasts of lanes 15 and 31 carry the sums across rows, and lane 63 ends
+// with the wave's sum.  Every lane of the wave must be active.
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ int64_t dpp_i64(int64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWS, 0xf, BOUND);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)((uint64_t)v >> 32), CTRL, ROWS, 0xf, BOUND);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ int wave_sum_i32(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+    v += dpp_i64<0x111, 0xf, true>(v);
+    v += dpp_i64<0x112, 0xf, true>(v);
+    v += dpp_i64<0x114, 0xf, true>(v);
+    v += dpp_i64<0x118, 0xf, true>(v);
+    v += dpp_i64<0x142, 0xa, false>(v);
+    v += dpp_i64<0x143, 0xc, false>(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), 63);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Largest v >= 0 over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8
 // leave each row's maximum in its lane 15, the row broadcasts of lanes 15
 // and 31 carry it across rows, and lane 63 ends with the wave's maximum.

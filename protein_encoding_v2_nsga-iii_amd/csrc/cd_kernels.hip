@@ -27,10 +27,20 @@ namespace cd {
 
 __host__ __device__ constexpr int align16(int x) { return (x + 15) & ~15; }
 
-// dynamic LDS of k_design: codons [m*L], amino acids [L], nucleotide codes
-// [3L], CDS statistics [m]
+// Table block at the start of k_design's dynamic LDS, staged from the job's
+// tables in 16-byte words: log weights lw [64] (int64), nucleotide
+// mismatches nd [64*64], hidden stops hs [64*64], nucleotide codes nt [64*3],
+// GC3 flags gc3 [64].
+constexpr int DT_LW = 0, DT_ND = 512, DT_HS = DT_ND + 4096, DT_NT = DT_HS + 4096, DT_GC3 = DT_NT + 192,
+              DT_BYTES = DT_GC3 + 64;
+static_assert(DT_BYTES % 16 == 0, "table block");
+
+// dynamic LDS of k_design: the table block, codons [m*L], per position the
+// first global codon and the number of synonyms of its amino acid [L] each,
+// nucleotide codes [3L], CDS statistics [m]
 size_t design_lds_bytes(int L, int m) {
-    return (size_t)align16(m * L) + align16(L) + align16(3 * L) + align16(m * (int)sizeof(CdsStat)) + 16;
+    return (size_t)DT_BYTES + align16(m * L) + 2 * (size_t)align16(L) + align16(3 * L) +
+           align16(m * (int)sizeof(CdsStat)) + 16;
 }
 
 // dynamic LDS of k_stem: nucleotide codes [3L], bit planes of the sequence and
@@ -44,8 +54,21 @@ struct DesignShared {
     int32_t dirty[MAX_M];
     int32_t hd[MAX_M * (MAX_M - 1) / 2];
     int64_t red[64];
+    int32_t red32[64];
+    float ov[4][MAX_M];  // per CDS: CAI, CPB, HSC quotient, GC3 fraction
     uint64_t redu[16];
     int32_t misc[16];
+};
+
+// k_design's LDS copies of the tables and the per-position synonym ranges.
+struct DesignLds {
+    const int64_t *lw;     // [64]
+    const uint8_t *nd;     // [64*64]
+    const uint8_t *hs;     // [64*64]
+    const uint8_t *nt;     // [64*3]
+    const uint8_t *gc3;    // [64]
+    const uint8_t *pbase;  // [L] first global codon of position k's amino acid
+    const uint8_t *pnum;   // [L] synonyms of position k's amino acid
 };
 
 // ---------------------------------------------------------------------------
@@ -150,93 +173,148 @@ __host__ __device__ inline int stem_parts(int n) {
 // Evaluation of the CDS flagged dirty, then objectives 0-4
 // (calMinimumCAI/CBP/HSC/HD, calMaximumGC3, utils.cuh:271-626).  A changed
 // CDS is marked for k_stem (pad = 1); its stem fields keep the parent's stem,
-// which seeds the scan, or are cleared when there is no parent.
+// which seeds the scan, or are cleared when there is no parent.  Up to four
+// dirty CDS share one pass over the positions, and up to eight CDS pairs one
+// pass of the Hamming distances, so the table reads of a pass overlap; the
+// sums are wave sums (DPP) added over the waves by one thread per value.
 // ---------------------------------------------------------------------------
-__device__ void evaluate(const DevJob &J, const uint8_t *cod, CdsStat *cst, bool have_parent, DesignShared &sh,
-                         float *obj, int32_t *oidx) {
+__device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod, CdsStat *cst, bool have_parent,
+                         DesignShared &sh, float *obj, int32_t *oidx) {
     const int L = J.L, m = J.m, n = 3 * L;
-    const int tid = threadIdx.x, NT = blockDim.x;
-    if (tid == 0) {
-        unsigned long long nd = 0;
-        for (int c = 0; c < m; ++c) nd += sh.dirty[c] != 0;
-        atomicAdd(&J.st->cds_scanned, nd);
-    }
-    for (int c = 0; c < m; ++c) {
-        if (!sh.dirty[c]) continue;
-        const uint8_t *g = cod + c * L;
-        int64_t v[4] = {0, 0, 0, 0};
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+    uint32_t dmask = 0;  // the same in every thread
+    for (int c = 0; c < m; ++c) dmask |= (uint32_t)(sh.dirty[c] != 0) << c;
+    if (tid == 0) atomicAdd(&J.st->cds_scanned, (unsigned long long)__popc(dmask));
+    while (dmask) {
+        int cs[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            cs[u] = dmask ? __builtin_ctz(dmask) : -1;
+            dmask &= dmask - 1u;
+        }
+        int64_t cai[4] = {0, 0, 0, 0}, cpb[4] = {0, 0, 0, 0};
+        int hsc[4] = {0, 0, 0, 0}, gcn[4] = {0, 0, 0, 0};
         for (int k = tid; k < L; k += NT) {
-            const int a = g[k];
-            v[0] += J.lw[a];
-            v[3] += J.gc3[a];
-            if (k < L - 1) {
-                const int b = g[k + 1];
-                v[2] += J.hs[a * NCODON + b];
-                if (k < L - 2) v[1] += J.cpfx[a * NSENSE + b];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (cs[u] < 0) continue;
+                const uint8_t *g = cod + cs[u] * L;
+                const int a = g[k];
+                cai[u] += D.lw[a];
+                gcn[u] += D.gc3[a];
+                if (k < L - 1) {
+                    const int b = g[k + 1];
+                    hsc[u] += D.hs[a * NCODON + b];
+                    if (k < L - 2) cpb[u] += J.cpfx[a * NSENSE + b];
+                }
             }
         }
-        block_sum_i64<4>(v, sh.red);
-        if (tid == 0) {
-            CdsStat s = cst[c];
-            if (!have_parent) s.l = s.p = s.q = 0;
-            s.cai = v[0];
-            s.cpb = v[1];
-            s.hsc = (int32_t)v[2];
-            s.gc3 = (int32_t)v[3];
-            s.pad = 1;
-            s.key = 0;
-            cst[c] = s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (cs[u] < 0) continue;
+            cai[u] = wave_sum_i64(cai[u]);
+            cpb[u] = wave_sum_i64(cpb[u]);
+            hsc[u] = wave_sum_i32(hsc[u]);
+            gcn[u] = wave_sum_i32(gcn[u]);
+            if (lane == 0) {
+                sh.red[wid * 8 + 2 * u] = cai[u];
+                sh.red[wid * 8 + 2 * u + 1] = cpb[u];
+                sh.red32[wid * 8 + 2 * u] = hsc[u];
+                sh.red32[wid * 8 + 2 * u + 1] = gcn[u];
+            }
+        }
+        __syncthreads();
+        if (tid < 4) {
+            const int c = tid == 0 ? cs[0] : tid == 1 ? cs[1] : tid == 2 ? cs[2] : cs[3];
+            if (c >= 0) {
+                int64_t a = 0, b = 0;
+                int h = 0, g = 0;
+                for (int w = 0; w < nw; ++w) {
+                    a += sh.red[w * 8 + 2 * tid];
+                    b += sh.red[w * 8 + 2 * tid + 1];
+                    h += sh.red32[w * 8 + 2 * tid];
+                    g += sh.red32[w * 8 + 2 * tid + 1];
+                }
+                CdsStat s = cst[c];
+                if (!have_parent) s.l = s.p = s.q = 0;
+                s.cai = a;
+                s.cpb = b;
+                s.hsc = h;
+                s.gc3 = g;
+                s.pad = 1;
+                s.key = 0;
+                cst[c] = s;
+            }
         }
         __syncthreads();
     }
-    // pairwise Hamming distances, four pairs per reduction
+    // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order
     const int npair = m * (m - 1) / 2;
-    for (int p0 = 0; p0 < npair; p0 += 4) {
-        int64_t v[4] = {0, 0, 0, 0};
-        int pi[4], pj[4];
-        for (int u = 0; u < 4; ++u) {
-            int q = p0 + u, i = 0;
-            if (q >= npair) {
-                pi[u] = pj[u] = -1;
-                continue;
+    int pi = 0, pj = 1;  // first pair of the pass
+    for (int p0 = 0; p0 < npair; p0 += 8) {
+        int ai[8], aj[8];
+        {
+            int i = pi, j = pj;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool ok = p0 + u < npair;
+                ai[u] = ok ? i : -1;
+                aj[u] = j;
+                if (ok && ++j == m) {
+                    ++i;
+                    j = i + 1;
+                }
             }
-            while (q >= m - 1 - i) {
-                q -= m - 1 - i;
-                ++i;
-            }
-            pi[u] = i;
-            pj[u] = i + 1 + q;
+            pi = i;
+            pj = j;
         }
+        int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int k = tid; k < L; k += NT)
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (pi[u] >= 0) v[u] += J.nd[cod[pi[u] * L + k] * NCODON + cod[pj[u] * L + k]];
-        block_sum_i64<4>(v, sh.red);
-        if (tid == 0)
-            for (int u = 0; u < 4 && p0 + u < npair; ++u) sh.hd[p0 + u] = (int32_t)v[u];
+            for (int u = 0; u < 8; ++u)
+                if (ai[u] >= 0) acc[u] += D.nd[cod[ai[u] * L + k] * NCODON + cod[aj[u] * L + k]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (ai[u] < 0) continue;
+            acc[u] = wave_sum_i32(acc[u]);
+            if (lane == 0) sh.red32[wid * 8 + u] = acc[u];
+        }
+        __syncthreads();
+        if (tid < 8 && p0 + tid < npair) {
+            int v = 0;
+            for (int w = 0; w < nw; ++w) v += sh.red32[w * 8 + tid];
+            sh.hd[p0 + tid] = v;
+        }
+        __syncthreads();
+    }
+    if (tid < m) {  // per-CDS values, one lane each
+        const CdsStat s = cst[tid];
+        sh.ov[0][tid] = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
+        sh.ov[1][tid] = (float)(((double)s.cpb * 0x1p-40) / (double)(L - 2));
+        sh.ov[2][tid] = (float)s.hsc / (float)(L - 1);
+        sh.ov[3][tid] = (float)s.gc3 / (float)L;
     }
     __syncthreads();
     if (tid == 0) {
         float cai_v = 0.f, cpb_v = 0.f, hsc_v = 0.f, gc_v = 0.f, hd_v = 0.f;
         for (int i = 0; i < 2 * NOBJ; ++i) oidx[i] = 0;
         for (int c = 0; c < m; ++c) {
-            const CdsStat s = cst[c];
-            const float cai = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
+            const float cai = sh.ov[0][c];
             if (c == 0 || -cai >= cai_v) {  // ties -> last CDS, utils.cuh:327
                 cai_v = -cai;
                 oidx[0] = c;
             }
-            const float cpb = (float)(((double)s.cpb * 0x1p-40) / (double)(L - 2));
+            const float cpb = sh.ov[1][c];
             if (c == 0 || -cpb >= cpb_v) {
                 cpb_v = -cpb;
                 oidx[2] = c;
             }
-            const float hq = (float)s.hsc / (float)(L - 1);
+            const float hq = sh.ov[2][c];
             if (c == 0 || -hq >= hsc_v) {
                 hsc_v = -hq;
                 oidx[4] = c;
             }
-            const float frac = (float)s.gc3 / (float)L;
+            const float frac = sh.ov[3][c];
             const float dev = fabsf(frac - J.gc3_ref);
             if (c == 0 || dev >= gc_v) {
                 gc_v = dev;
@@ -270,8 +348,8 @@ __device__ void evaluate(const DevJob &J, const uint8_t *cod, CdsStat *cst, bool
 // ---------------------------------------------------------------------------
 struct MutArgs {
     const DevJob *J;
+    const DesignLds *D;
     uint8_t *cod;
-    const int8_t *aa;
     uint32_t gen, child;
 };
 
@@ -283,9 +361,9 @@ __device__ void mut_random(const MutArgs &A, int32_t *dirty) {
         const int c = t / L, k = t - c * L;
         uint32_t r[4];
         philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int a = A.aa[k], ns = c_syn_num[a];
+        const int ns = A.D->pnum[k];
         if (!bern(r[0], J.thr_pm) || ns == 1) continue;
-        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        const int base = A.D->pbase[k], cur = A.cod[t] - base;
         int nw = (int)uniform_int(r[1], (uint32_t)(ns - 1));
         if (nw >= cur) ++nw;
         A.cod[t] = (uint8_t)(base + nw);
@@ -301,9 +379,9 @@ __device__ void mut_cai(const MutArgs &A, int cds, int32_t *dirty) {
         const int t = cds * L + k;
         uint32_t r[4];
         philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int a = A.aa[k], ns = c_syn_num[a];
+        const int ns = A.D->pnum[k];
         if (!bern(r[0], J.thr_pm) || ns == 1) continue;
-        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        const int base = A.D->pbase[k], cur = A.cod[t] - base;
         if (cur != ns - 1) {
             A.cod[t] = (uint8_t)(base + cur + 1 + (int)uniform_int(r[1], (uint32_t)(ns - 1 - cur)));
             dirty[cds] = 1;
@@ -320,9 +398,9 @@ __device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty) {
         const int t = cds * L + k;
         uint32_t r[4];
         philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int a = A.aa[k], ns = c_syn_num[a];
+        const int ns = A.D->pnum[k];
         if (!bern(r[0], J.thr_2pm) || ns == 1) continue;
-        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        const int base = A.D->pbase[k], cur = A.cod[t] - base;
         const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
         const int right = A.cod[t + 1];
         auto score = [&](int g) -> float {
@@ -351,15 +429,15 @@ __device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty) {
         const int t = cds * L + k;
         uint32_t r[4];
         philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int a = A.aa[k], ns = c_syn_num[a];
+        const int ns = A.D->pnum[k];
         if (!bern(r[0], J.thr_2pm) || ns == 1) continue;
-        const int base = c_syn_start[a], cur = A.cod[t] - base;
+        const int base = A.D->pbase[k], cur = A.cod[t] - base;
         const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
         const int right = (k < L - 1) ? (int)A.cod[t + 1] : -1;
         auto count = [&](int g) -> int {
             int v = 0;
-            if (left >= 0) v += J.hs[left * NCODON + g];
-            if (right >= 0) v += J.hs[g * NCODON + right];
+            if (left >= 0) v += A.D->hs[left * NCODON + g];
+            if (right >= 0) v += A.D->hs[g * NCODON + right];
             return v;
         };
         const int curv = count(base + cur);
@@ -404,28 +482,28 @@ __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
     for (int k = threadIdx.x; k < L; k += blockDim.x) {
         uint32_t d[6];
         slot_draws(A, (uint32_t)(cds1 * L + k), d);
-        const int a = A.aa[k], ns = c_syn_num[a];
+        const int ns = A.D->pnum[k];
         if (!bern(d[0], J.thr_pm) || ns == 1) continue;
         int8_t order[6];
         shuffle(d + 1, ns, order);
-        const int base = c_syn_start[a], cur = A.cod[cds1 * L + k] - base;
+        const int base = A.D->pbase[k], cur = A.cod[cds1 * L + k] - base;
         const int g2 = A.cod[cds2 * L + k];
         auto min_hd = [&](int g) -> int {
             int mn = 127;
             for (int o = 0; o < m; ++o) {
                 if (o == cds1) continue;
-                const int t = J.nd[g * NCODON + A.cod[o * L + k]];
+                const int t = A.D->nd[g * NCODON + A.cod[o * L + k]];
                 mn = t < mn ? t : mn;
             }
             return mn;
         };
-        const int cur_pair = J.nd[(base + cur) * NCODON + g2];
+        const int cur_pair = A.D->nd[(base + cur) * NCODON + g2];
         const int cur_mhd = min_hd(base + cur);
         int best_pair = -1, best_mhd = -1, best = -1;
         for (int j = 0; j < ns; ++j) {
             const int sh = order[j];
             if (sh == cur) continue;
-            const int np_ = J.nd[(base + sh) * NCODON + g2];
+            const int np_ = A.D->nd[(base + sh) * NCODON + g2];
             const int nm = min_hd(base + sh);
             if (nm > cur_mhd && nm > best_mhd) {
                 best_mhd = nm;
@@ -450,16 +528,16 @@ __device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *
     for (int k = threadIdx.x; k < L; k += blockDim.x) {
         uint32_t d[6];
         slot_draws(A, (uint32_t)(cds * L + k), d);
-        const int a = A.aa[k], ns = c_syn_num[a];
+        const int ns = A.D->pnum[k];
         if (!bern(d[0], thr) || ns == 1) continue;
         int8_t order[6];
         shuffle(d + 1, ns, order);
-        const int base = c_syn_start[a], cur = A.cod[cds * L + k] - base;
-        const int curg = J.gc3[base + cur];
+        const int base = A.D->pbase[k], cur = A.cod[cds * L + k] - base;
+        const int curg = A.D->gc3[base + cur];
         for (int j = 0; j < ns; ++j) {
             const int sh = order[j];
             if (sh == cur) continue;
-            const int ng = J.gc3[base + sh];
+            const int ng = A.D->gc3[base + sh];
             if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
                 A.cod[cds * L + k] = (uint8_t)(base + sh);
                 dirty[cds] = 1;
@@ -504,7 +582,7 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
         if (tid == 0) {
             uint32_t r[4];
             philox(A.gen, A.child, (uint32_t)t, TAG_SL, J.seed, r);
-            const int a = A.aa[pos], ns = c_syn_num[a], base = c_syn_start[a];
+            const int ns = A.D->pnum[pos], base = A.D->pbase[pos];
             ctl[0] = 0;
             ctl[1] = A.cod[cds * L + pos];
             if (bern(r[0], J.thr_pm) && ns > 1) {
@@ -523,9 +601,9 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
         __syncthreads();
         for (int k = tid; k < L; k += NT) {
             const int g = A.cod[cds * L + k];
-            ntb[3 * k] = J.nt[3 * g];
-            ntb[3 * k + 1] = J.nt[3 * g + 1];
-            ntb[3 * k + 2] = J.nt[3 * g + 2];
+            ntb[3 * k] = A.D->nt[3 * g];
+            ntb[3 * k + 1] = A.D->nt[3 * g + 1];
+            ntb[3 * k + 2] = A.D->nt[3 * g + 2];
         }
         __syncthreads();
         int f[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, l4[4] = {-1, -1, -1, -1};
@@ -587,48 +665,76 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     const DevJob &J = jobs[it.x];
     const int L = J.L, m = J.m, mL = m * L, N = J.N;
     const int tid = threadIdx.x, NT = blockDim.x;
-    uint8_t *cod = dyn;
-    int8_t *aa = (int8_t *)(dyn + align16(mL));
-    uint8_t *ntb = (uint8_t *)aa + align16(L);
+    uint8_t *cod = dyn + DT_BYTES;
+    uint8_t *pbase = cod + align16(mL);
+    uint8_t *pnum = pbase + align16(L);
+    uint8_t *ntb = pnum + align16(L);
     CdsStat *cst = (CdsStat *)(ntb + align16(3 * L));
+    const DesignLds D{(const int64_t *)(dyn + DT_LW), dyn + DT_ND, dyn + DT_HS, dyn + DT_NT, dyn + DT_GC3, pbase, pnum};
 
-    uint32_t gen = 0;
-    int dst = 0, out_slot = it.y;
-    for (int k = tid; k < L; k += NT) aa[k] = J.aa[k];
+    // stage the tables (the layout of the table block matches the job's
+    // tables, each a multiple of 16 bytes) and the synonym ranges
+    for (int i = tid; i < DT_BYTES / 16; i += NT) {
+        int o = i;
+        const uint4 *src;
+        if (o < DT_ND / 16)
+            src = (const uint4 *)J.lw + o;
+        else if ((o -= DT_ND / 16) < 4096 / 16)
+            src = (const uint4 *)J.nd + o;
+        else if ((o -= 4096 / 16) < 4096 / 16)
+            src = (const uint4 *)J.hs + o;
+        else if ((o -= 4096 / 16) < 192 / 16)
+            src = (const uint4 *)J.nt + o;
+        else
+            src = (const uint4 *)J.gc3 + (o - 192 / 16);
+        ((uint4 *)dyn)[i] = *src;
+    }
+    for (int k = tid; k < L; k += NT) {
+        const int a = J.aa[k];
+        pbase[k] = (uint8_t)c_syn_start[a];
+        pnum[k] = (uint8_t)c_syn_num[a];
+    }
     if (tid < MAX_M) sh.dirty[tid] = (MODE != 2);
+    uint32_t gen = 0;
+    int dst = 0, out_slot = it.y, src = 0, parent = 0;
+    if (MODE == 2) {
+        gen = J.st->gen;
+        src = (int)(gen & 1u);
+        dst = src ^ 1;
+        out_slot = N + it.y;
+        parent = J.sorted[it.y];
+    }
     __syncthreads();
 
     if (MODE == 0) {
         const int d = it.y;
-        for (int t = tid; t < mL; t += NT) {
-            const int k = t % L, a = aa[k], ns = c_syn_num[a];
-            int s;
-            if (d == 0) {
-                s = ns - 1;
-            } else {
-                uint32_t r[4];
-                philox(0u, (uint32_t)d, (uint32_t)t, TAG_INIT, J.seed, r);
-                s = (int)uniform_int(r[0], (uint32_t)ns);
+        for (int c = 0; c < m; ++c)
+            for (int k = tid; k < L; k += NT) {
+                const int t = c * L + k, ns = pnum[k];
+                int s;
+                if (d == 0) {
+                    s = ns - 1;
+                } else {
+                    uint32_t r[4];
+                    philox(0u, (uint32_t)d, (uint32_t)t, TAG_INIT, J.seed, r);
+                    s = (int)uniform_int(r[0], (uint32_t)ns);
+                }
+                cod[t] = (uint8_t)(pbase[k] + s);
             }
-            cod[t] = (uint8_t)(c_syn_start[a] + s);
-        }
     } else if (MODE == 1) {
-        const int8_t *src = J.genome[0] + (size_t)it.y * mL;
-        for (int t = tid; t < mL; t += NT) cod[t] = (uint8_t)(c_syn_start[aa[t % L]] + src[t]);
+        const int8_t *sg = J.genome[0] + (size_t)it.y * mL;
+        for (int c = 0; c < m; ++c)
+            for (int k = tid; k < L; k += NT) cod[c * L + k] = (uint8_t)(pbase[k] + sg[c * L + k]);
     } else {
         const int c = it.y;
-        gen = J.st->gen;
-        const int src = (int)(gen & 1u);
-        dst = src ^ 1;
-        out_slot = N + c;
-        const int parent = J.sorted[c];
         const int8_t *pg = J.genome[src] + (size_t)parent * mL;
         int8_t *sg = J.genome[dst] + (size_t)c * mL;
-        for (int t = tid; t < mL; t += NT) {
-            const int8_t v = pg[t];
-            sg[t] = v;
-            cod[t] = (uint8_t)(c_syn_start[aa[t % L]] + v);
-        }
+        for (int q = 0; q < m; ++q)
+            for (int k = tid; k < L; k += NT) {
+                const int8_t v = pg[q * L + k];
+                sg[q * L + k] = v;
+                cod[q * L + k] = (uint8_t)(pbase[k] + v);
+            }
         // survivor records (copySolution, utils.cuh:1106-1147)
         if (tid < NOBJ) {
             const float v = J.obj[src][(size_t)parent * NOBJ + tid];
@@ -659,7 +765,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         uint32_t r[4];
         philox(gen, (uint32_t)c, 0u, TAG_TYPE, J.seed, r);
         const int type = (int)uniform_int(r[0], (uint32_t)(NOBJ + 1));  // main.cu:172-178
-        const MutArgs A{&J, cod, aa, gen, (uint32_t)c};
+        const MutArgs A{&J, &D, cod, gen, (uint32_t)c};
         switch (type) {  // main.cu:181-211
         case 0: mut_random(A, sh.dirty); break;
         case 1: mut_cai(A, sh.pidx[0], sh.dirty); break;
@@ -677,9 +783,10 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
     __shared__ float s_obj[NOBJ];
     __shared__ int32_t s_idx[2 * NOBJ];
-    evaluate(J, cod, cst, MODE == 2, sh, s_obj, s_idx);
+    evaluate(J, D, cod, cst, MODE == 2, sh, s_obj, s_idx);
     int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
-    for (int t = tid; t < mL; t += NT) og[t] = (int8_t)(cod[t] - c_syn_start[aa[t % L]]);
+    for (int c = 0; c < m; ++c)
+        for (int k = tid; k < L; k += NT) og[c * L + k] = (int8_t)(cod[c * L + k] - pbase[k]);
     if (tid < NOBJ - 1) oobj[tid] = s_obj[tid];  // objective 5: k_stem_finish
     if (tid < 2 * NOBJ && tid != 10) J.oidx[dst][(size_t)out_slot * 2 * NOBJ + tid] = (int8_t)s_idx[tid];
     if (tid < m) J.cst[dst][(size_t)out_slot * m + tid] = cst[tid];
