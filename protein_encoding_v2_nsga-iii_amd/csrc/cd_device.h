@@ -111,6 +111,29 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     return v;
 }
 
+// Global-memory access through the global address space.  Pointers held in
+// the job descriptor are generic, and generic (flat) accesses also count
+// against the LDS wait counter, so an LDS-only barrier would still wait for
+// them; these compile to global loads and stores.
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads also waits for the
+// workgroup's outstanding global stores; code whose threads never read what
+// other threads of the workgroup stored to global memory can let those
+// stores drain while it goes on.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Sums over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8 give each
 // lane the sum of its row up to itself (lane 15 the row's sum), the row
 // broadcasts of lanes 15 and 31 carry the sums across rows, and lane 63 ends
@@ -140,6 +163,29 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), 63);
     return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Smallest and largest v over a full wave, wave-uniform (lanes outside a
+// shifted row or a masked row contribute the identity).
+__device__ __forceinline__ int wave_min_i32(int v) {
+    constexpr int I = 0x7FFFFFFF;
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x118, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x142, 0xa, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(I, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+    constexpr int I = (int)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(I, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Largest v >= 0 over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8

@@ -58,7 +58,35 @@ struct DesignShared {
     float ov[4][MAX_M];  // per CDS: CAI, CPB, HSC quotient, GC3 fraction
     uint64_t redu[16];
     int32_t misc[16];
+    int32_t slw[4 * 11];  // mut_sl: per wave, first [4], last [4], full [3]
 };
+
+// CdsStat records in global memory, field by field through the global
+// address space (an aggregate copy would go through flat accesses).
+__device__ __forceinline__ CdsStat gload_stat(const CdsStat *p) {
+    CdsStat s;
+    s.cai = gload(&p->cai);
+    s.cpb = gload(&p->cpb);
+    s.hsc = gload(&p->hsc);
+    s.gc3 = gload(&p->gc3);
+    s.l = gload(&p->l);
+    s.p = gload(&p->p);
+    s.q = gload(&p->q);
+    s.pad = gload(&p->pad);
+    s.key = gload(&p->key);
+    return s;
+}
+__device__ __forceinline__ void gstore_stat(CdsStat *p, const CdsStat &s) {
+    gstore(&p->cai, s.cai);
+    gstore(&p->cpb, s.cpb);
+    gstore(&p->hsc, s.hsc);
+    gstore(&p->gc3, s.gc3);
+    gstore(&p->l, s.l);
+    gstore(&p->p, s.p);
+    gstore(&p->q, s.q);
+    gstore(&p->pad, s.pad);
+    gstore(&p->key, s.key);
+}
 
 // k_design's LDS copies of the tables and the per-position synonym ranges.
 struct DesignLds {
@@ -184,7 +212,6 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     uint32_t dmask = 0;  // the same in every thread
     for (int c = 0; c < m; ++c) dmask |= (uint32_t)(sh.dirty[c] != 0) << c;
-    if (tid == 0) atomicAdd(&J.st->cds_scanned, (unsigned long long)__popc(dmask));
     while (dmask) {
         int cs[4];
 #pragma unroll
@@ -205,7 +232,7 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
                 if (k < L - 1) {
                     const int b = g[k + 1];
                     hsc[u] += D.hs[a * NCODON + b];
-                    if (k < L - 2) cpb[u] += J.cpfx[a * NSENSE + b];
+                    if (k < L - 2) cpb[u] += gload(J.cpfx + a * NSENSE + b);
                 }
             }
         }
@@ -223,7 +250,7 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
                 sh.red32[wid * 8 + 2 * u + 1] = gcn[u];
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (tid < 4) {
             const int c = tid == 0 ? cs[0] : tid == 1 ? cs[1] : tid == 2 ? cs[2] : cs[3];
             if (c >= 0) {
@@ -246,7 +273,7 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
                 cst[c] = s;
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
     // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order
     const int npair = m * (m - 1) / 2;
@@ -279,13 +306,13 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
             acc[u] = wave_sum_i32(acc[u]);
             if (lane == 0) sh.red32[wid * 8 + u] = acc[u];
         }
-        __syncthreads();
+        lds_barrier();
         if (tid < 8 && p0 + tid < npair) {
             int v = 0;
             for (int w = 0; w < nw; ++w) v += sh.red32[w * 8 + tid];
             sh.hd[p0 + tid] = v;
         }
-        __syncthreads();
+        lds_barrier();
     }
     if (tid < m) {  // per-CDS values, one lane each
         const CdsStat s = cst[tid];
@@ -294,7 +321,7 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
         sh.ov[2][tid] = (float)s.hsc / (float)(L - 1);
         sh.ov[3][tid] = (float)s.gc3 / (float)L;
     }
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) {
         float cai_v = 0.f, cpb_v = 0.f, hsc_v = 0.f, gc_v = 0.f, hd_v = 0.f;
         for (int i = 0; i < 2 * NOBJ; ++i) oidx[i] = 0;
@@ -338,7 +365,7 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
         obj[3] = hd_v;
         obj[4] = gc_v;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // ---------------------------------------------------------------------------
@@ -346,6 +373,12 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
 // Each thread owns whole codons; operators that read neighbours only touch
 // codons of the other parity or of other CDS, as in the reference.
 // ---------------------------------------------------------------------------
+// The pick-th (0-based) set bit of a candidate mask.
+__device__ __forceinline__ int nth_bit(uint32_t mask, uint32_t pick) {
+    for (uint32_t q = 0; q < pick; ++q) mask &= mask - 1u;
+    return __builtin_ctz(mask);
+}
+
 struct MutArgs {
     const DevJob *J;
     const DesignLds *D;
@@ -410,11 +443,11 @@ __device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty) {
             return v;
         };
         const float curv = score(base + cur);
-        int cand[6], nc = 0;
+        uint32_t cand = 0;  // synonyms scoring above the current codon, in order
         for (int j = 0; j < ns; ++j)
-            if (j != cur && score(base + j) > curv) cand[nc++] = j;
-        if (nc > 0) {
-            A.cod[t] = (uint8_t)(base + cand[uniform_int(r[1], (uint32_t)nc)]);
+            if (j != cur && score(base + j) > curv) cand |= 1u << j;
+        if (cand) {
+            A.cod[t] = (uint8_t)(base + nth_bit(cand, uniform_int(r[1], (uint32_t)__popc(cand))));
             dirty[cds] = 1;
         }
     }
@@ -441,11 +474,11 @@ __device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty) {
             return v;
         };
         const int curv = count(base + cur);
-        int cand[6], nc = 0;
+        uint32_t cand = 0;
         for (int j = 0; j < ns; ++j)
-            if (j != cur && count(base + j) > curv) cand[nc++] = j;
-        if (nc > 0) {
-            A.cod[t] = (uint8_t)(base + cand[uniform_int(r[1], (uint32_t)nc)]);
+            if (j != cur && count(base + j) > curv) cand |= 1u << j;
+        if (cand) {
+            A.cod[t] = (uint8_t)(base + nth_bit(cand, uniform_int(r[1], (uint32_t)__popc(cand))));
             dirty[cds] = 1;
         }
     }
@@ -463,17 +496,24 @@ __device__ __forceinline__ void slot_draws(const MutArgs &A, uint32_t slot, uint
     d[5] = b[1];
 }
 
-// indexArrayShuffling, utils.cuh:203-218
-__device__ __forceinline__ void shuffle(const uint32_t *draws, int n, int8_t *arr) {
-    for (int i = 0; i < 6; ++i) arr[i] = (int8_t)(i < n ? i : -1);
-    int d = 0;
-    for (int i = n - 1; i > 0; --i) {
-        const int j = (int)uniform_int(draws[d++], (uint32_t)(i + 1));
-        const int8_t tmp = arr[i];
-        arr[i] = arr[j];
-        arr[j] = tmp;
+// indexArrayShuffling, utils.cuh:203-218: Fisher-Yates over the n <= 6
+// synonyms, draw d swapping entries n-1-d and uniform_int(draws[d], n-d).
+// The order is packed in 4-bit fields (entry i at bits 4i) so that it stays
+// in registers.
+__device__ __forceinline__ uint32_t shuffle(const uint32_t *draws, int n) {
+    uint32_t perm = 0x543210u;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        const int i = n - 1 - d;
+        if (i <= 0) break;
+        const int j = (int)uniform_int(draws[d], (uint32_t)(i + 1));
+        const uint32_t vi = (perm >> (4 * i)) & 0xFu, vj = (perm >> (4 * j)) & 0xFu;
+        perm &= ~((0xFu << (4 * i)) | (0xFu << (4 * j)));
+        perm |= (vj << (4 * i)) | (vi << (4 * j));
     }
+    return perm;
 }
+__device__ __forceinline__ int perm_at(uint32_t perm, int j) { return (int)((perm >> (4 * j)) & 0xFu); }
 
 // mutationHD, mutation.cuh:390-534
 __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
@@ -484,8 +524,7 @@ __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
         slot_draws(A, (uint32_t)(cds1 * L + k), d);
         const int ns = A.D->pnum[k];
         if (!bern(d[0], J.thr_pm) || ns == 1) continue;
-        int8_t order[6];
-        shuffle(d + 1, ns, order);
+        const uint32_t order = shuffle(d + 1, ns);
         const int base = A.D->pbase[k], cur = A.cod[cds1 * L + k] - base;
         const int g2 = A.cod[cds2 * L + k];
         auto min_hd = [&](int g) -> int {
@@ -501,7 +540,7 @@ __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
         const int cur_mhd = min_hd(base + cur);
         int best_pair = -1, best_mhd = -1, best = -1;
         for (int j = 0; j < ns; ++j) {
-            const int sh = order[j];
+            const int sh = perm_at(order, j);
             if (sh == cur) continue;
             const int np_ = A.D->nd[(base + sh) * NCODON + g2];
             const int nm = min_hd(base + sh);
@@ -530,12 +569,11 @@ __device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *
         slot_draws(A, (uint32_t)(cds * L + k), d);
         const int ns = A.D->pnum[k];
         if (!bern(d[0], thr) || ns == 1) continue;
-        int8_t order[6];
-        shuffle(d + 1, ns, order);
+        const uint32_t order = shuffle(d + 1, ns);
         const int base = A.D->pbase[k], cur = A.cod[cds * L + k] - base;
         const int curg = A.D->gc3[base + cur];
         for (int j = 0; j < ns; ++j) {
-            const int sh = order[j];
+            const int sh = perm_at(order, j);
             if (sh == cur) continue;
             const int ng = A.D->gc3[base + sh];
             if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
@@ -565,12 +603,10 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
     const int den = J.den_sl;
     const float parent_sl = sh.pobj[5];
     const int trials = (odd ? 1 : 0) + 2 * half;
-    int *first = sh.misc;      // [4]
-    int *last = sh.misc + 4;   // [4]
-    int *full = sh.misc + 8;   // [3]
-    int *ctl = sh.misc + 11;   // [0] changed, [1] saved, [2] terminate, [3] cur_L
+    const int lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+    int *ctl = sh.misc + 11;  // [0] changed, [1] saved, [2] terminate, [3] cur_L
     if (tid == 0) ctl[3] = Lst;
-    __syncthreads();
+    lds_barrier();
     for (int t = 0; t < trials; ++t) {
         int pos;
         if (odd && t == 0) {
@@ -593,25 +629,26 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
                 ctl[0] = 1;
             }
         }
-        if (tid < 4) {
-            first[tid] = 0x7FFFFFFF;
-            last[tid] = -1;
-        }
-        if (tid < 3) full[tid] = 1;
-        __syncthreads();
+        lds_barrier();
         for (int k = tid; k < L; k += NT) {
             const int g = A.cod[cds * L + k];
             ntb[3 * k] = A.D->nt[3 * g];
             ntb[3 * k + 1] = A.D->nt[3 * g + 1];
             ntb[3 * k + 2] = A.D->nt[3 * g + 2];
         }
-        __syncthreads();
+        lds_barrier();
+        // first and last position of each base seen by this thread (positions
+        // rise, so the first hit is the first); kept in registers, one per base
         int f[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, l4[4] = {-1, -1, -1, -1};
         int okA = 1, okB = 1, okC = 1;
         for (int i = tid; i < n; i += NT) {
             const int b = ntb[i];
-            f[b] = min(f[b], i);
-            l4[b] = max(l4[b], i);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (b == q) {
+                    f[q] = min(f[q], i);
+                    l4[q] = i;
+                }
         }
         for (int i = tid; i < den; i += NT) {
             okA &= ((ntb[i] ^ ntb[n - 1 - i]) == 3);
@@ -620,16 +657,41 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
                 okC &= ((ntb[1 + i] ^ ntb[n - 1 - i]) == 3);
             }
         }
-        for (int b = 0; b < 4; ++b) {
-            if (f[b] != 0x7FFFFFFF) atomicMin(&first[b], f[b]);
-            if (l4[b] >= 0) atomicMax(&last[b], l4[b]);
+        // wave minima, maxima and conjunctions (DPP, ballots), combined over
+        // the waves by thread 0
+        const bool allA = __ballot(!okA) == 0, allB = __ballot(!okB) == 0, allC = __ballot(!okC) == 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f[q] = wave_min_i32(f[q]);
+            l4[q] = wave_max_i32(l4[q]);
         }
-        if (!okA) atomicAnd(&full[0], 0);
-        if (!okB) atomicAnd(&full[1], 0);
-        if (!okC) atomicAnd(&full[2], 0);
-        __syncthreads();
+        if (lane == 0) {
+            int *w = sh.slw + wid * 11;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                w[q] = f[q];
+                w[4 + q] = l4[q];
+            }
+            w[8] = allA;
+            w[9] = allB;
+            w[10] = allC;
+        }
+        lds_barrier();
         if (tid == 0) {
+            int first[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, last[4] = {-1, -1, -1, -1};
+            int full[3] = {1, 1, 1};
+            for (int w = 0; w < nw; ++w) {
+                const int *v = sh.slw + w * 11;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    first[q] = min(first[q], v[q]);
+                    last[q] = max(last[q], v[4 + q]);
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) full[q] &= v[8 + q];
+            }
             bool any = false;
+#pragma unroll
             for (int a = 0; a < 4; ++a) {
                 const int b = a ^ 3;
                 if (first[a] != 0x7FFFFFFF && last[b] >= 0 && last[b] - first[a] > MIN_HAIRPIN) any = true;
@@ -645,7 +707,7 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
             if (!ctl[2] && ctl[0]) A.cod[cds * L + pos] = (uint8_t)ctl[1];
             if (ctl[2] && ctl[0]) dirty[cds] = 1;
         }
-        __syncthreads();
+        lds_barrier();
         if (ctl[2]) break;
     }
 }
@@ -673,24 +735,24 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     const DesignLds D{(const int64_t *)(dyn + DT_LW), dyn + DT_ND, dyn + DT_HS, dyn + DT_NT, dyn + DT_GC3, pbase, pnum};
 
     // stage the tables (the layout of the table block matches the job's
-    // tables, each a multiple of 16 bytes) and the synonym ranges
-    for (int i = tid; i < DT_BYTES / 16; i += NT) {
+    // tables, each a multiple of 8 bytes) and the synonym ranges
+    for (int i = tid; i < DT_BYTES / 8; i += NT) {
         int o = i;
-        const uint4 *src;
-        if (o < DT_ND / 16)
-            src = (const uint4 *)J.lw + o;
-        else if ((o -= DT_ND / 16) < 4096 / 16)
-            src = (const uint4 *)J.nd + o;
-        else if ((o -= 4096 / 16) < 4096 / 16)
-            src = (const uint4 *)J.hs + o;
-        else if ((o -= 4096 / 16) < 192 / 16)
-            src = (const uint4 *)J.nt + o;
+        const uint64_t *src;
+        if (o < DT_ND / 8)
+            src = (const uint64_t *)J.lw + o;
+        else if ((o -= DT_ND / 8) < 4096 / 8)
+            src = (const uint64_t *)J.nd + o;
+        else if ((o -= 4096 / 8) < 4096 / 8)
+            src = (const uint64_t *)J.hs + o;
+        else if ((o -= 4096 / 8) < 192 / 8)
+            src = (const uint64_t *)J.nt + o;
         else
-            src = (const uint4 *)J.gc3 + (o - 192 / 16);
-        ((uint4 *)dyn)[i] = *src;
+            src = (const uint64_t *)J.gc3 + (o - 192 / 8);
+        ((uint64_t *)dyn)[i] = gload(src);
     }
     for (int k = tid; k < L; k += NT) {
-        const int a = J.aa[k];
+        const int a = gload(J.aa + k);
         pbase[k] = (uint8_t)c_syn_start[a];
         pnum[k] = (uint8_t)c_syn_num[a];
     }
@@ -704,7 +766,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         out_slot = N + it.y;
         parent = J.sorted[it.y];
     }
-    __syncthreads();
+    lds_barrier();
 
     if (MODE == 0) {
         const int d = it.y;
@@ -724,44 +786,44 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     } else if (MODE == 1) {
         const int8_t *sg = J.genome[0] + (size_t)it.y * mL;
         for (int c = 0; c < m; ++c)
-            for (int k = tid; k < L; k += NT) cod[c * L + k] = (uint8_t)(pbase[k] + sg[c * L + k]);
+            for (int k = tid; k < L; k += NT) cod[c * L + k] = (uint8_t)(pbase[k] + gload(sg + c * L + k));
     } else {
         const int c = it.y;
         const int8_t *pg = J.genome[src] + (size_t)parent * mL;
         int8_t *sg = J.genome[dst] + (size_t)c * mL;
         for (int q = 0; q < m; ++q)
             for (int k = tid; k < L; k += NT) {
-                const int8_t v = pg[q * L + k];
-                sg[q * L + k] = v;
+                const int8_t v = gload(pg + q * L + k);
+                gstore(sg + q * L + k, v);
                 cod[q * L + k] = (uint8_t)(pbase[k] + v);
             }
         // survivor records (copySolution, utils.cuh:1106-1147)
         if (tid < NOBJ) {
-            const float v = J.obj[src][(size_t)parent * NOBJ + tid];
-            J.obj[dst][(size_t)c * NOBJ + tid] = v;
+            const float v = gload(J.obj[src] + (size_t)parent * NOBJ + tid);
+            gstore(J.obj[dst] + (size_t)c * NOBJ + tid, v);
             sh.pobj[tid] = v;
         }
         if (tid < 2 * NOBJ) {
-            const int8_t v = J.oidx[src][(size_t)parent * 2 * NOBJ + tid];
-            J.oidx[dst][(size_t)c * 2 * NOBJ + tid] = v;
+            const int8_t v = gload(J.oidx[src] + (size_t)parent * 2 * NOBJ + tid);
+            gstore(J.oidx[dst] + (size_t)c * 2 * NOBJ + tid, v);
             sh.pidx[tid] = v;
         }
         if (tid < 4) {
-            const int32_t v = J.pql[src][(size_t)parent * 4 + tid];
-            J.pql[dst][(size_t)c * 4 + tid] = v;
+            const int32_t v = gload(J.pql[src] + (size_t)parent * 4 + tid);
+            gstore(J.pql[dst] + (size_t)c * 4 + tid, v);
             sh.ppql[tid] = v;
         }
         if (tid < m) {
-            CdsStat s = J.cst[src][(size_t)parent * m + tid];
+            CdsStat s = gload_stat(J.cst[src] + (size_t)parent * m + tid);
             s.pad = 0;
-            J.cst[dst][(size_t)c * m + tid] = s;
+            gstore_stat(J.cst[dst] + (size_t)c * m + tid, s);
             cst[tid] = s;
         }
         if (tid == 0) {
-            J.np[c] = 0;
-            J.np[N + c] = 0;
+            gstore(J.np + c, 0);
+            gstore(J.np + N + c, 0);
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t r[4];
         philox(gen, (uint32_t)c, 0u, TAG_TYPE, J.seed, r);
         const int type = (int)uniform_int(r[0], (uint32_t)(NOBJ + 1));  // main.cu:172-178
@@ -778,7 +840,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         default: mut_sl(A, ntb, sh, sh.dirty); break;
         }
     }
-    __syncthreads();
+    lds_barrier();
 
     float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
     __shared__ float s_obj[NOBJ];
@@ -786,10 +848,15 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     evaluate(J, D, cod, cst, MODE == 2, sh, s_obj, s_idx);
     int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
     for (int c = 0; c < m; ++c)
-        for (int k = tid; k < L; k += NT) og[c * L + k] = (int8_t)(cod[c * L + k] - pbase[k]);
-    if (tid < NOBJ - 1) oobj[tid] = s_obj[tid];  // objective 5: k_stem_finish
-    if (tid < 2 * NOBJ && tid != 10) J.oidx[dst][(size_t)out_slot * 2 * NOBJ + tid] = (int8_t)s_idx[tid];
-    if (tid < m) J.cst[dst][(size_t)out_slot * m + tid] = cst[tid];
+        for (int k = tid; k < L; k += NT) gstore(og + c * L + k, (int8_t)(cod[c * L + k] - pbase[k]));
+    if (tid < NOBJ - 1) gstore(oobj + tid, s_obj[tid]);  // objective 5: k_stem_finish
+    if (tid < 2 * NOBJ && tid != 10) gstore(J.oidx[dst] + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
+    if (tid < m) gstore_stat(J.cst[dst] + (size_t)out_slot * m + tid, cst[tid]);
+    if (tid == 0) {  // CDS evaluated (k_stem scans the same ones)
+        unsigned long long nd = 0;
+        for (int c = 0; c < m; ++c) nd += sh.dirty[c] != 0;
+        atomicAdd(&J.st->cds_scanned, nd);
+    }
 }
 
 // Buffer and slot of the design an evaluation work item refers to: children
