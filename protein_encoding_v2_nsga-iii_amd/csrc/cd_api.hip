@@ -183,7 +183,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
 }
 
 void work_counts(int P, int W, int *n_dom, int *n_asc) {
-    *n_dom = ((P + 255) / 256) * W;
+    *n_dom = ((P + cd::DOM_ROWS - 1) / cd::DOM_ROWS) * W;
     *n_asc = (P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS;
 }
 
@@ -295,8 +295,8 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     J->d_self = (cd::DevJob *)(base + o.self);
     int4 *dom = (int4 *)&host[o.domi];
     int q = 0;
-    for (int t = 0; t < (J->P + 255) / 256; ++t)
-        for (int w = 0; w < J->W; ++w) dom[q++] = make_int4(0, t * 256, w, 0);
+    for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
+        for (int w = 0; w < J->W; ++w) dom[q++] = make_int4(0, t * cd::DOM_ROWS, w, 0);
     int2 *asc = (int2 *)&host[o.asc];
     for (int t = 0; t < J->n_asc; ++t) asc[t] = make_int2(0, t * cd::ASSOC_DESIGNS);
     J->d_dom = (int4 *)(base + o.domi);
@@ -350,8 +350,8 @@ void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
         cd_job *J = jobs[j];
         wl.descs.push_back(J->h);
         for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
-        for (int t = 0; t < (J->P + 255) / 256; ++t)
-            for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * 256, w, 0));
+        for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
+            for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS, w, 0));
         for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
             wl.asc.push_back(make_int2(j, t * cd::ASSOC_DESIGNS));
         wl.lds = std::max(wl.lds, J->lds);

@@ -12,6 +12,7 @@ enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
 
 constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
 constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (one wave each)
+constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup (16 per wave)
 constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
 
 size_t design_lds_bytes(int L, int m);

@@ -968,34 +968,44 @@ __device__ __forceinline__ bool dominates(const float *a, const float *b) {
 }
 
 // Bit i of word w of row j (i = 64w + bit) is set when design i dominates
-// design j; the matrix is stored word-major, dom[w * P + j], so that a
-// workgroup's rows are one contiguous store and the front peeling reads one
-// word of consecutive rows per wave.  np[j] counts the dominators.
-// item = (job, first row, word, 0).
+// design j; the matrix is stored word-major, dom[w * P + j], so that the
+// front peeling reads one word of consecutive rows per wave.  np[j] counts
+// the dominators.  item = (job, first row, word, 0): DOM_ROWS rows of one
+// word, 16 per wave.  Lane b holds candidate 64w + b; a row's word is the
+// ballot of the 64 tests against that row (row values read from LDS as
+// broadcasts), and the wave stores its 16 words and adds their counts at
+// once.
 __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items) {
-    __shared__ float cand[64 * NOBJ];
+    constexpr int WR = DOM_ROWS / 4;  // rows per wave
+    __shared__ float rows[DOM_ROWS * NOBJ];
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
     const int P = J.P, word = it.z;
     const float *obj = J.obj[pool_buffer(J)];
-    for (int t = threadIdx.x; t < 64 * NOBJ; t += blockDim.x) {
-        const int i = word * 64 + t / NOBJ;
-        cand[t] = (i < P) ? obj[(size_t)i * NOBJ + t % NOBJ] : 0.f;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int t = threadIdx.x; t < DOM_ROWS * NOBJ; t += blockDim.x) {
+        const int r = it.y + t / NOBJ;
+        rows[t] = r < P ? gload(obj + (size_t)it.y * NOBJ + t) : 0.f;
     }
-    __syncthreads();
-    const int r = it.y + (int)threadIdx.x;
-    if (r >= P) return;
-    float mine[NOBJ];
+    const int i = word * 64 + lane;
+    float cand[NOBJ];
 #pragma unroll
-    for (int k = 0; k < NOBJ; ++k) mine[k] = obj[(size_t)r * NOBJ + k];
-    uint64_t bits = 0;
-    const int nb = min(64, P - word * 64);
-    for (int b = 0; b < nb; ++b) {
-        const int i = word * 64 + b;
-        if (i != r && dominates(cand + b * NOBJ, mine)) bits |= 1ull << b;
+    for (int k = 0; k < NOBJ; ++k) cand[k] = i < P ? gload(obj + (size_t)i * NOBJ + k) : 0.f;
+    __syncthreads();
+    const int r0 = it.y + wv * WR;
+    uint64_t mine = 0;  // lane q < WR: the word of row r0 + q
+#pragma unroll 4
+    for (int q = 0; q < WR; ++q) {
+        const int r = r0 + q;
+        const float *row = rows + (wv * WR + q) * NOBJ;
+        const uint64_t bits = __ballot(i < P && i != r && dominates(cand, row));
+        if (lane == q) mine = bits;
     }
-    J.dom[(size_t)word * P + r] = bits;
-    if (bits) atomicAdd(&J.np[r], (int)__popcll(bits));
+    if (lane < WR && r0 + lane < P) {
+        const int r = r0 + lane;
+        gstore(J.dom + (size_t)word * P + r, mine);
+        if (mine) atomicAdd(&J.np[r], (int)__popcll(mine));
+    }
 }
 
 __device__ __forceinline__ float asf(const float *x, const float *ideal, int axis) {  // sorting.cuh:428-449

@@ -39,6 +39,15 @@ __host__ __device__ inline uint32_t scan_funnel(uint32_t hi, uint32_t lo, int sh
 #endif
 }
 
+// Bit j of w broadcast to all 32 bits: 0 or ~0 (v_bfe_i32, one op).
+__host__ __device__ inline uint32_t scan_bit_mask(uint32_t w, int j) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint32_t)__builtin_amdgcn_sbfe((int)w, (uint32_t)j, 1u);
+#else
+    return 0u - ((w >> j) & 1u);
+#endif
+}
+
 __host__ __device__ inline uint64_t scan_key(int l, int p, int q) {
     return ((uint64_t)l << 40) | ((uint64_t)(0xFFFFF - p) << 20) | (uint64_t)(0xFFFFF - q);
 }

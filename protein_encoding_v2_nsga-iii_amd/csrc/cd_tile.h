@@ -176,7 +176,7 @@ __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int ba
     for (int j = J0; j < 32; ++j) {
         const int a = j - J0 + 4;
         const uint32_t xl = scan_funnel(x1.x, x0.x, j), xh = scan_funnel(x1.y, x0.y, j);
-        const uint32_t yl = 0u - ((y.x >> j) & 1u), yh = 0u - ((y.y >> j) & 1u);
+        const uint32_t yl = scan_bit_mask(y.x, j), yh = scan_bit_mask(y.y, j);
         const uint32_t m = ~((xl ^ yl) | (xh ^ yh));
         M[a] = m;
         A2[a] = m & M[a - 1];
