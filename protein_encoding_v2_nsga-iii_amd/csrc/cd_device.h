@@ -188,6 +188,33 @@ __device__ __forceinline__ int wave_max_i32(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Smallest and largest float over a full wave (fminf / fmaxf), wave-uniform;
+// the identity is FLT_MAX / -FLT_MAX, the start value of every lane's fold.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f32(float v, float ident) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(ident), __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+__device__ __forceinline__ float wave_min_f32(float v) {
+    constexpr float I = 3.402823466e+38f;
+    v = fminf(v, dpp_f32<0x111, 0xf>(v, I));
+    v = fminf(v, dpp_f32<0x112, 0xf>(v, I));
+    v = fminf(v, dpp_f32<0x114, 0xf>(v, I));
+    v = fminf(v, dpp_f32<0x118, 0xf>(v, I));
+    v = fminf(v, dpp_f32<0x142, 0xa>(v, I));
+    v = fminf(v, dpp_f32<0x143, 0xc>(v, I));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ float wave_max_f32(float v) {
+    constexpr float I = -3.402823466e+38f;
+    v = fmaxf(v, dpp_f32<0x111, 0xf>(v, I));
+    v = fmaxf(v, dpp_f32<0x112, 0xf>(v, I));
+    v = fmaxf(v, dpp_f32<0x114, 0xf>(v, I));
+    v = fmaxf(v, dpp_f32<0x118, 0xf>(v, I));
+    v = fmaxf(v, dpp_f32<0x142, 0xa>(v, I));
+    v = fmaxf(v, dpp_f32<0x143, 0xc>(v, I));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Largest v >= 0 over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8
 // leave each row's maximum in its lane 15, the row broadcasts of lanes 15
 // and 31 carry it across rows, and lane 63 ends with the wave's maximum.

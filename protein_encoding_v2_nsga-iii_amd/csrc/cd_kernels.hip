@@ -1021,35 +1021,50 @@ __device__ __forceinline__ float asf(const float *x, const float *ideal, int axi
 }
 
 // GaussianElimination and helpers, sorting.cuh:451-599 (E x = 1)
+// Every loop is unrolled so that the matrix stays in registers (the first
+// row below the pivot with a usable entry is added through a flag, not an
+// index).
 __device__ void gauss(const float *E, float *sol) {
     float AB[NOBJ][NOBJ + 1];
+#pragma unroll
     for (int r = 0; r < NOBJ; ++r) {
+#pragma unroll
         for (int c = 0; c < NOBJ; ++c) AB[r][c] = E[r * NOBJ + c];
         AB[r][NOBJ] = 1.0f;
     }
+#pragma unroll
     for (int col = 0; col < NOBJ; ++col) {
         if (fabsf(AB[col][col]) <= 1e-6f) {
-            int row = col + 1;
-            for (; row < NOBJ; ++row)
-                if (fabsf(AB[row][col]) > 1e-6f) break;
-            if (row < NOBJ)
-                for (int c = col; c < NOBJ + 1; ++c) AB[col][c] += AB[row][c];
+            bool added = false;
+#pragma unroll
+            for (int row = col + 1; row < NOBJ; ++row)
+                if (!added && fabsf(AB[row][col]) > 1e-6f) {
+#pragma unroll
+                    for (int c = col; c < NOBJ + 1; ++c) AB[col][c] += AB[row][c];
+                    added = true;
+                }
         }
+#pragma unroll
         for (int row = col + 1; row < NOBJ; ++row) {
             const float f = AB[row][col] / AB[col][col];
+#pragma unroll
             for (int c = col; c < NOBJ + 1; ++c) AB[row][c] -= f * AB[col][c];
         }
     }
+#pragma unroll
     for (int row = NOBJ - 1; row >= 0; --row) {
         float s = 0.f;
+#pragma unroll
         for (int c = row + 1; c < NOBJ; ++c) s += AB[row][c];
         float x = AB[row][NOBJ];
         if (row != NOBJ - 1) x -= s;
         x /= AB[row][row];
         AB[row][NOBJ] = x;
         AB[row][row] = 1.0f;
+#pragma unroll
         for (int r = 0; r < row; ++r) AB[r][row] *= x;
     }
+#pragma unroll
     for (int r = 0; r < NOBJ; ++r) sol[r] = AB[r][NOBJ];
 }
 
@@ -1057,6 +1072,49 @@ __device__ __forceinline__ void argmin_pair(float &v, int &id, float ov, int oid
     if (ov < v || (ov == v && oid < id)) {
         v = ov;
         id = oid;
+    }
+}
+
+// argmin_pair over a full wave (DPP row shifts and broadcasts, lane 63 read
+// back), wave-uniform; lanes outside a shifted or masked row contribute
+// (FLT_MAX, 0x7FFFFFFF), the start value of every lane's fold.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void argmin_dpp(float &v, int &id) {
+    const float ov = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(FLT_MAX), __float_as_int(v), CTRL, ROWS, 0xf, false));
+    const int oid = __builtin_amdgcn_update_dpp(0x7FFFFFFF, id, CTRL, ROWS, 0xf, false);
+    argmin_pair(v, id, ov, oid);
+}
+__device__ __forceinline__ void wave_argmin(float &v, int &id) {
+    argmin_dpp<0x111, 0xf>(v, id);
+    argmin_dpp<0x112, 0xf>(v, id);
+    argmin_dpp<0x114, 0xf>(v, id);
+    argmin_dpp<0x118, 0xf>(v, id);
+    argmin_dpp<0x142, 0xa>(v, id);
+    argmin_dpp<0x143, 0xc>(v, id);
+    v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    id = __builtin_amdgcn_readlane(id, 63);
+}
+
+// asf() for all six axes of one design: the clamped differences are divided
+// by F_PRECISION once each (the axis itself divides by 1, which is exact),
+// and each axis takes the same strict running maximum in k order.
+__device__ __forceinline__ void asf_all(const float *x, const float *ideal, float *out) {
+    float t[NOBJ], d[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        t[k] = x[k] - ideal[k];
+        if ((double)t[k] < 0.001) t[k] = 0.f;
+        d[k] = t[k] / F_PRECISION;
+    }
+#pragma unroll
+    for (int a = 0; a < NOBJ; ++a) {
+        float mx = -FLT_MAX;
+#pragma unroll
+        for (int k = 0; k < NOBJ; ++k) {
+            const float v = k == a ? t[k] : d[k];
+            if (v > mx) mx = v;
+        }
+        out[a] = mx;
     }
 }
 
@@ -1095,27 +1153,25 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
             fm[k] = -FLT_MAX;
         }
         int c = 0;
-        for (int r = tid; r < P; r += NT) {
-            const bool f0 = J.np[r] == 0;
-            J.front_of[r] = f0 ? 0 : -1;
-            if (f0) {
-                atomicOr((unsigned long long *)&ranked[r >> 6], 1ull << (r & 63));
-                ++c;
-            }
-            for (int k = 0; k < NOBJ; ++k) {
-                const float v = obj[(size_t)r * NOBJ + k];
-                mn[k] = fminf(mn[k], v);
-                mx[k] = fmaxf(mx[k], v);
-                if (f0) fm[k] = fmaxf(fm[k], v);
-            }
+        for (int r0 = 0; r0 < P; r0 += NT) {  // a wave's 64 designs are one word of the masks
+            const int r = r0 + tid;
+            const bool f0 = r < P && gload(J.np + r) == 0;
+            const uint64_t b = __ballot(f0);
+            if (r < P) gstore(J.front_of + r, f0 ? 0 : -1);
+            if (lane == 0 && r < P) ranked[r >> 6] = b;
+            c += __popcll(b);
+            if (r < P)
+                for (int k = 0; k < NOBJ; ++k) {
+                    const float v = gload(obj + (size_t)r * NOBJ + k);
+                    mn[k] = fminf(mn[k], v);
+                    mx[k] = fmaxf(mx[k], v);
+                    if (f0) fm[k] = fmaxf(fm[k], v);
+                }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            c += __shfl_xor(c, o, 64);
-            for (int k = 0; k < NOBJ; ++k) {
-                mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
-                mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
-                fm[k] = fmaxf(fm[k], __shfl_xor(fm[k], o, 64));
-            }
+        for (int k = 0; k < NOBJ; ++k) {
+            mn[k] = wave_min_f32(mn[k]);
+            mx[k] = wave_max_f32(mx[k]);
+            fm[k] = wave_max_f32(fm[k]);
         }
         if (lane == 0) {
             atomicAdd(&s_cnt, c);
@@ -1144,17 +1200,19 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
         ++f;
         // a design joins front f when all its dominators are ranked
         int c = 0;
-        for (int r = tid; r < P; r += NT) {
-            if ((ranked[r >> 6] >> (r & 63)) & 1ull) continue;
-            uint64_t any = 0;
-            for (int w = 0; w < W; ++w) any |= J.dom[(size_t)w * P + r] & ~ranked[w];
-            if (!any) {
-                atomicOr((unsigned long long *)&newm[r >> 6], 1ull << (r & 63));
-                J.front_of[r] = f;
-                ++c;
+        for (int r0 = 0; r0 < P; r0 += NT) {  // a wave's 64 designs are one word of the masks
+            const int r = r0 + tid;
+            bool joins = false;
+            if (r < P && !((ranked[r >> 6] >> (r & 63)) & 1ull)) {
+                uint64_t any = 0;
+                for (int w = 0; w < W; ++w) any |= gload(J.dom + (size_t)w * P + r) & ~ranked[w];
+                joins = !any;
             }
+            const uint64_t b = __ballot(joins);
+            if (joins) gstore(J.front_of + r, f);
+            if (lane == 0 && b) newm[r >> 6] = b;
+            c += __popcll(b);
         }
-        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if (lane == 0) atomicAdd(&s_new, c);
         __syncthreads();
         if (s_new == 0) {  // a dominance cycle: every remaining design forms the front
@@ -1191,16 +1249,12 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
         }
         for (int r = tid; r < P; r += NT) {
             if (J.front_of[r] != 0) continue;
-            float x[NOBJ];
+            float x[NOBJ], av[NOBJ];
             for (int k = 0; k < NOBJ; ++k) x[k] = obj[(size_t)r * NOBJ + k];
-            for (int a = 0; a < NOBJ; ++a) argmin_pair(bv[a], bi[a], asf(x, ideal, a), r);
+            asf_all(x, ideal, av);
+            for (int a = 0; a < NOBJ; ++a) argmin_pair(bv[a], bi[a], av[a], r);
         }
-        for (int o = 32; o > 0; o >>= 1)
-            for (int a = 0; a < NOBJ; ++a) {
-                const float ov = __shfl_xor(bv[a], o, 64);
-                const int oi = __shfl_xor(bi[a], o, 64);
-                argmin_pair(bv[a], bi[a], ov, oi);
-            }
+        for (int a = 0; a < NOBJ; ++a) wave_argmin(bv[a], bi[a]);
         if (lane == 0)
             for (int a = 0; a < NOBJ; ++a) {
                 red_v[wid * NOBJ + a] = bv[a];
@@ -1233,14 +1287,25 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
         __syncthreads();
         if (tid == 0) {
             int cur[NOBJ];  // candidate row now held by each extreme slot
+#pragma unroll
             for (int t = 0; t < NOBJ; ++t) cur[t] = t;
+#pragma unroll
             for (int axis = 0; axis < NOBJ; ++axis) {
                 float v = pv[axis];
                 int id = pid[axis];
+#pragma unroll
                 for (int t = 0; t < NOBJ; ++t) argmin_pair(v, id, casf[cur[t]][axis], P + t);
-                if (id != 0x7FFFFFFF && v < FLT_MAX) cur[axis] = (id >= P) ? cur[id - P] : NOBJ + axis;
+                if (id != 0x7FFFFFFF && v < FLT_MAX) {
+                    int held = NOBJ + axis;  // cur[id - P] when an extreme slot won, read by a select
+#pragma unroll
+                    for (int t = 0; t < NOBJ; ++t)
+                        if (id == P + t) held = cur[t];
+                    cur[axis] = held;
+                }
             }
+#pragma unroll
             for (int t = 0; t < NOBJ; ++t)
+#pragma unroll
                 for (int k = 0; k < NOBJ; ++k) ext[t * NOBJ + k] = crow[cur[t]][k];
         }
     }
@@ -1339,12 +1404,6 @@ __device__ __forceinline__ uint64_t niche_prio(const DevJob &J, uint32_t gen, in
 }
 
 // Block-wide sum of one int per thread.
-__device__ __forceinline__ int block_sum_int(int v, int *red) {
-    int total;
-    block_scan_excl(v, red, &total);
-    return total;
-}
-
 // One workgroup per job: niche counts, the level sweeps of
 // referenceBasedSorting (sorting.cuh:1182-1279) in closed form, and the
 // survivor list in ascending pool index.  Ends the generation.  Dynamic LDS
@@ -1357,9 +1416,11 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     const DevJob &J = jobs[blockIdx.x];
     SelState *st = J.st;
     const int P = J.P, N = J.N, R = J.R, tid = threadIdx.x, NT = blockDim.x;
+    const int lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     const uint32_t gen = st->gen;
     const int last = st->last;
     __shared__ int red[16];
+    __shared__ int red16[16 * 16], cq[16];
     __shared__ int s_carry;
     const int tier = niche_tier(P, R);
     const int Pa = niche_stride(P);
@@ -1418,40 +1479,48 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         const int need = N - st->base;
         // picks made through level i: C(i) = sum_r clamp(i + 1 - cnt[r], 0, rem[r]),
         // nondecreasing in i; the final level is the smallest i with C(i) >= need
-        // (or P), found by galloping from 0 (it is usually 0 or 1) then bisection
-        auto picks_through = [&](int i) -> int {
-            int sum = 0;
-            for (int r = tid; r < R; r += NT) sum += min(max(i + 1 - cnt[r], 0), rem[r]);
-            return block_sum_int(sum, red);
-        };
-        int istar, prevC;
-        const int c0 = picks_through(0);
-        if (c0 >= need) {
-            istar = 0;
-            prevC = 0;
-        } else {
-            int lo = 0, clo = c0, hi, step = 1;
-            for (;;) {  // C(lo) = clo < need
-                hi = min(lo + step, P);
-                const int ch = picks_through(hi);
-                if (ch >= need || hi == P) break;
-                lo = hi;
-                clo = ch;
-                step <<= 1;
+        // (or P).  A 16-way search: each pass evaluates C at 16 evenly spaced
+        // levels of the open interval (lo, hi) with one workgroup reduction,
+        // keeping C(lo) < need and C(hi) >= need or hi = P.
+        int lo = -1, clo = 0, hi = P;
+        while (hi - lo > 1) {
+            const int step = (hi - lo + 16) / 17;
+            int acc[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = 0;
+            for (int r = tid; r < R; r += NT) {
+                const int c = cnt[r], m = rem[r];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[q] += min(max(min(lo + step * (q + 1), hi) + 1 - c, 0), m);
             }
-            while (hi - lo > 1) {  // C(lo) < need, and C(hi) >= need or hi == P
-                const int mid = (lo + hi) >> 1;
-                const int cm = picks_through(mid);
-                if (cm >= need) {
-                    hi = mid;
-                } else {
-                    lo = mid;
-                    clo = cm;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = wave_sum_i32(acc[q]);
+            if (lane == 0)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) red16[wid * 16 + q] = acc[q];
+            __syncthreads();
+            if (tid < 16) {
+                int v = 0;
+                for (int w = 0; w < nw; ++w) v += red16[w * 16 + tid];
+                cq[tid] = v;
+            }
+            __syncthreads();
+            int nlo = lo, nclo = clo, nhi = hi;
+            for (int q = 0; q < 16; ++q) {
+                const int i = lo + step * (q + 1);
+                if (i >= hi) break;
+                if (cq[q] >= need) {
+                    nhi = i;
+                    break;
                 }
+                nlo = i;
+                nclo = cq[q];
             }
-            istar = hi;
-            prevC = clo;
+            lo = nlo;
+            clo = nclo;
+            hi = nhi;
         }
+        const int istar = hi, prevC = clo;
         const int need_final = need - prevC;
         if (tid == 0) s_carry = 0;
         __syncthreads();
@@ -1468,7 +1537,6 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         // level 0 of an empty niche takes its nearest candidate (ties to the
         // lowest index): one wave per reference point with picks
         {
-            const int lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
             for (int r = wid; r < R; r += nw) {
                 int l0 = -1;
                 if (kp[r] > 0 && cnt[r] == 0) {
@@ -1477,12 +1545,7 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
                     float bd = FLT_MAX;
                     int bi = 0x7FFFFFFF;
                     for (int t = lane; t < nb; t += 64) argmin_pair(bd, bi, ld ? ld[off[r] + t] : J.dist[b[t]], b[t]);
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) {
-                        const float od = __shfl_xor(bd, o, 64);
-                        const int oi = __shfl_xor(bi, o, 64);
-                        argmin_pair(bd, bi, od, oi);
-                    }
+                    wave_argmin(bd, bi);
                     // every distance FLT_MAX gives the lowest index, as a sweep in order would
                     l0 = bi != 0x7FFFFFFF ? bi : b[0];
                 }
@@ -1506,14 +1569,17 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
                 if (kk >= nb) {
                     sel = kk > 0;
                 } else if (kk > 0) {
+                    // rank among the bucket without the level-0 pick; the
+                    // design's own key is not below itself
                     const uint64_t my = ((uint64_t)lp[sl] << 32) | (uint32_t)d;
                     const int *b = lb + off[r];
                     const uint32_t *bp = lp + off[r];
+                    const int nr = rem[r];
                     int rank = 0;
-                    for (int u = 0; u < rem[r] && rank < kk; ++u) {
+#pragma unroll 4
+                    for (int u = 0; u < nr; ++u) {
                         const int e = b[u];
-                        if (e == d || e == l0) continue;
-                        rank += (((uint64_t)bp[u] << 32) | (uint32_t)e) < my;
+                        rank += (int)(((((uint64_t)bp[u] << 32) | (uint32_t)e) < my) & (e != l0));
                     }
                     sel = rank < kk;
                 }
