@@ -144,13 +144,16 @@ __host__ __device__ inline void tile_verify(const uint2 *X, const uint2 *Y, int 
 }
 
 // Match-chain history carried from one tile of a block to the next: the
-// matches, 2-windows and 4-windows of the last four rows.
+// last row's matches, the last two rows' 2-windows, the last four rows'
+// 4-windows (index 0 the most recent row).
 struct TileHist {
-    uint32_t m[4], a2[4], a4[4];
+    uint32_t m1, a2[2], a4[4];
 };
 
 __host__ __device__ inline void tile_hist_reset(TileHist &t) {
-    for (int i = 0; i < 4; ++i) t.m[i] = t.a2[i] = t.a4[i] = 0u;
+    t.m1 = 0u;
+    t.a2[0] = t.a2[1] = 0u;
+    t.a4[0] = t.a4[1] = t.a4[2] = t.a4[3] = 0u;
 }
 
 // Rows u0 + J0 .. u0 + 31 of block (h, base): X is the plane read along the
@@ -162,36 +165,44 @@ __host__ __device__ inline void tile_hist_reset(TileHist &t) {
 template <int J0>
 __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int base, int u0, TileHist &t,
                                           uint32_t f[3]) {
+    static_assert((32 - J0) % 8 == 0, "rows in groups of 8");
     const int w = (u0 + base) >> 5;
     const uint2 x0 = X[w], x1 = X[w + 1], y = Y[u0 >> 5];
-    uint32_t M[36], A2[36], A4[36];  // index r + 4 holds row u0 + J0 + r; 0..3 the history
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        M[i] = t.m[i];
-        A2[i] = t.a2[i];
-        A4[i] = t.a4[i];
-    }
+    uint32_t m1 = t.m1, a2_1 = t.a2[0], a2_2 = t.a2[1];
+    uint32_t a4_1 = t.a4[0], a4_2 = t.a4[1], a4_3 = t.a4[2], a4_4 = t.a4[3];
     uint32_t f4 = 0u, f7 = 0u, f8 = 0u;
+    // groups of 8 rows: the history rotates with period 4, so an unrolled
+    // group renames registers instead of moving them, and the loop over the
+    // groups keeps the kernel at 76 VGPRs (6 waves per SIMD)
+#pragma unroll 1
+    for (int j0 = J0; j0 < 32; j0 += 8) {
 #pragma unroll
-    for (int j = J0; j < 32; ++j) {
-        const int a = j - J0 + 4;
-        const uint32_t xl = scan_funnel(x1.x, x0.x, j), xh = scan_funnel(x1.y, x0.y, j);
-        const uint32_t yl = scan_bit_mask(y.x, j), yh = scan_bit_mask(y.y, j);
-        const uint32_t m = ~((xl ^ yl) | (xh ^ yh));
-        M[a] = m;
-        A2[a] = m & M[a - 1];
-        A4[a] = A2[a] & A2[a - 2];
-        f4 |= A4[a];
-        f7 |= A4[a] & A4[a - 3];
-        f8 |= A4[a] & A4[a - 4];
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = j0 + jj;
+            const uint32_t xl = scan_funnel(x1.x, x0.x, j), xh = scan_funnel(x1.y, x0.y, j);
+            const uint32_t yl = scan_bit_mask(y.x, j), yh = scan_bit_mask(y.y, j);
+            const uint32_t m = ~((xl ^ yl) | (xh ^ yh));
+            const uint32_t a2 = m & m1;
+            const uint32_t a4 = a2 & a2_2;
+            f4 |= a4;
+            f7 |= a4 & a4_3;
+            f8 |= a4 & a4_4;
+            m1 = m;
+            a2_2 = a2_1;
+            a2_1 = a2;
+            a4_4 = a4_3;
+            a4_3 = a4_2;
+            a4_2 = a4_1;
+            a4_1 = a4;
+        }
     }
-    const int cnt = 32 - J0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        t.m[i] = M[cnt + i];
-        t.a2[i] = A2[cnt + i];
-        t.a4[i] = A4[cnt + i];
-    }
+    t.m1 = m1;
+    t.a2[0] = a2_1;
+    t.a2[1] = a2_2;
+    t.a4[0] = a4_1;
+    t.a4[1] = a4_2;
+    t.a4[2] = a4_3;
+    t.a4[3] = a4_4;
     f[0] = f4;
     f[1] = f7;
     f[2] = f8;
