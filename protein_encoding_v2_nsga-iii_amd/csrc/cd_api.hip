@@ -138,7 +138,8 @@ namespace {
 
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
-    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], sorted, front, np, dom,
+    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, sorted, front,
+        np, dom,
         ref_of, dist, niche, flag, st, self, off, domi, asc;
     size_t total;
 };
@@ -162,6 +163,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
             o.cst[b] = s.add((size_t)P * m * sizeof(cd::CdsStat));
         }
         o.off = s.add((size_t)N * sizeof(int2));
+        o.planes = s.add((size_t)P * m * cd::cds_plane_bytes(L));
     }
     for (int b = 0; b < (sel_only ? 1 : 2); ++b) o.obj[b] = s.add((size_t)P * 6 * 4);
     if (sel_only) o.obj[1] = o.obj[0];
@@ -285,6 +287,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.sorted = (int32_t *)(base + o.sorted);
     h.front_of = (int32_t *)(base + o.front);
     h.np = (int32_t *)(base + o.np);
+    h.planes = sel_only ? nullptr : (uint2 *)(base + o.planes);
     h.dom = (uint64_t *)(base + o.dom);
     h.ref_of = (int32_t *)(base + o.ref_of);
     h.dist = (float *)(base + o.dist);

@@ -17,6 +17,7 @@ constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (co
 
 size_t design_lds_bytes(int L, int m);
 size_t stem_lds_bytes(int L);
+size_t cds_plane_bytes(int L);  // one CDS's two bit planes in the job's plane buffer
 int stem_parts_of(int L);  // k_stem workgroups per CDS of L codons (1 .. STEM_PARTS)
 hipError_t set_design_lds_limit(size_t bytes);
 hipError_t set_stem_lds_limit(size_t bytes);

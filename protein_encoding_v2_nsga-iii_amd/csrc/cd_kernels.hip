@@ -43,9 +43,12 @@ size_t design_lds_bytes(int L, int m) {
            align16(m * (int)sizeof(CdsStat)) + 16;
 }
 
-// dynamic LDS of k_stem: nucleotide codes [3L], bit planes of the sequence and
-// its reverse complement [2][scan_words(3L)]
-size_t stem_lds_bytes(int L) { return (size_t)align16(3 * L) + (size_t)2 * scan_words(3 * L) * sizeof(uint2); }
+// one CDS's bit planes: the sequence and its reverse complement, each
+// scan_words(3L) words of (low bits, high bits) of the nucleotide codes
+size_t cds_plane_bytes(int L) { return (size_t)2 * scan_words(3 * L) * sizeof(uint2); }
+
+// dynamic LDS of k_stem: the bit planes of one CDS
+size_t stem_lds_bytes(int L) { return cds_plane_bytes(L); }
 
 struct DesignShared {
     float pobj[NOBJ];
@@ -100,9 +103,9 @@ struct DesignLds {
 };
 
 // ---------------------------------------------------------------------------
-// Longest stem of one CDS held as nucleotide codes in LDS (calMaximumSL,
-// utils.cuh:712-888): the sequence and its reverse complement are packed into
-// bit planes with wave ballots; the lanes of the `parts` workgroups of the
+// Longest stem of one CDS held as bit planes in LDS (calMaximumSL,
+// utils.cuh:712-888; the planes are built by k_design): the lanes of the
+// `parts` workgroups of the
 // CDS split its tiles (cd_tile.h) into contiguous ranges, and the workgroup
 // takes the maximum key (longest, then smallest p, then smallest q).  A CDS
 // with no run of 4 and no seed of 4 is scanned exactly along its
@@ -116,9 +119,14 @@ struct DesignLds {
 // Each such run is a stem of the child (its pairs lie on one anti-diagonal
 // and enclose at least the parent's loop), so its length bounds the longest
 // stem from below; only its value up to 8 matters to the scan.
-__device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p, int q, int lane) {
+__device__ __forceinline__ int plane_base(const uint2 *sp, int p) {
+    const uint2 w = sp[p >> 5];
+    const int b = p & 31;
+    return (int)(((w.x >> b) & 1u) | (((w.y >> b) & 1u) << 1));
+}
+__device__ __forceinline__ int stem_seed(const uint2 *sp, int n, int l, int p, int q, int lane) {
     const int i = p + l - 1, j = q;
-    const bool pair = l > 0 && lane <= i && j + lane < n && ((ntb[i - lane] ^ ntb[j + lane]) == 3);
+    const bool pair = l > 0 && lane <= i && j + lane < n && ((plane_base(sp, i - lane) ^ plane_base(sp, j + lane)) == 3);
     uint64_t m = __ballot(pair);
     int run = 0;
     while (m) {  // each round removes the last pair of every run
@@ -128,26 +136,13 @@ __device__ __forceinline__ int stem_seed(const uint8_t *ntb, int n, int l, int p
     return run;
 }
 
-__device__ uint64_t stem_cds(const uint8_t *ntb, int n, uint2 *planes, const CdsStat *seed_from, int part, int parts,
-                             int32_t *bcast, uint64_t *redu) {
-    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+__device__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_from, int part, int parts, int32_t *bcast,
+                             uint64_t *redu) {
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const int W = scan_words(n);
-    uint2 *sp = planes, *rp = planes + W;
-    for (int p0 = wid * 64; p0 < W * 32; p0 += nw * 64) {  // W is even: words w, w+1 < W
-        const int pos = p0 + lane;
-        const int c = pos < n ? ntb[pos] : 0;
-        const int rc = pos < n ? (ntb[n - 1 - pos] ^ 3) : 0;
-        const uint64_t bl = __ballot(c & 1), bh = __ballot(c >> 1), rl = __ballot(rc & 1), rh = __ballot(rc >> 1);
-        if (lane == 0) {
-            const int w = p0 >> 5;
-            sp[w] = make_uint2((uint32_t)bl, (uint32_t)bh);
-            sp[w + 1] = make_uint2((uint32_t)(bl >> 32), (uint32_t)(bh >> 32));
-            rp[w] = make_uint2((uint32_t)rl, (uint32_t)rh);
-            rp[w + 1] = make_uint2((uint32_t)(rl >> 32), (uint32_t)(rh >> 32));
-        }
-    }
+    const uint2 *sp = planes, *rp = planes + W;
     if (wid == 0) {
-        const int seed = seed_from ? stem_seed(ntb, n, seed_from->l, seed_from->p, seed_from->q, lane) : 0;
+        const int seed = seed_from ? stem_seed(sp, n, seed_from->l, seed_from->p, seed_from->q, lane) : 0;
         if (lane == 0) *bcast = seed;
     }
     __syncthreads();
@@ -712,6 +707,33 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
     }
 }
 
+// Bit planes of one CDS (codons g [L] as global codon indices, nucleotide
+// codes nt [64*3], both in LDS) into the job's plane buffer gp: bit p of the
+// sequence plane is position p, of the reverse-complement plane position
+// n-1-p complemented; positions past n are zero.  One wave ballots 64
+// positions; its lane 0 stores the four words.  Every lane must be active.
+__device__ void build_planes(const uint8_t *g, const uint8_t *nt, int L, uint2 *gp) {
+    const int n = 3 * L, W = scan_words(n), lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint2 *sp = gp, *rp = gp + W;
+    for (int p0 = wid * 64; p0 < W * 32; p0 += nw * 64) {  // W is even: words w, w+1 < W
+        const int pos = p0 + lane;
+        int c = 0, rc = 0;
+        if (pos < n) {
+            const int k = pos / 3, q = n - 1 - pos, kq = q / 3;
+            c = nt[3 * g[k] + (pos - 3 * k)];
+            rc = nt[3 * g[kq] + (q - 3 * kq)] ^ 3;
+        }
+        const uint64_t bl = __ballot(c & 1), bh = __ballot(c >> 1), rl = __ballot(rc & 1), rh = __ballot(rc >> 1);
+        if (lane == 0) {
+            const int w = p0 >> 5;
+            gstore((uint64_t *)(sp + w), (uint64_t)((bl & 0xFFFFFFFFull) | (bh << 32)));
+            gstore((uint64_t *)(sp + w + 1), (uint64_t)((bl >> 32) | (bh & 0xFFFFFFFF00000000ull)));
+            gstore((uint64_t *)(rp + w), (uint64_t)((rl & 0xFFFFFFFFull) | (rh << 32)));
+            gstore((uint64_t *)(rp + w + 1), (uint64_t)((rl >> 32) | (rh & 0xFFFFFFFF00000000ull)));
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_design: one workgroup per design.
 //   INIT       random genome (design 0: best-CAI codons), genPopulation,
@@ -849,6 +871,8 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
     for (int c = 0; c < m; ++c)
         for (int k = tid; k < L; k += NT) gstore(og + c * L + k, (int8_t)(cod[c * L + k] - pbase[k]));
+    for (int c = 0; c < m; ++c)  // planes of the changed CDS for k_stem
+        if (sh.dirty[c]) build_planes(cod + c * L, D.nt, L, J.planes + ((size_t)out_slot * m + c) * 2 * scan_words(3 * L));
     if (tid < NOBJ - 1) gstore(oobj + tid, s_obj[tid]);  // objective 5: k_stem_finish
     if (tid < 2 * NOBJ && tid != 10) gstore(J.oidx[dst] + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
     if (tid < m) gstore_stat(J.cst[dst] + (size_t)out_slot * m + tid, cst[tid]);
@@ -898,18 +922,13 @@ __global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, c
     item_slot(J, mode, it.y, buf, slot);
     CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
     if (cs->pad == 0) return;  // same word for every thread; k_stem never writes it
-    uint8_t *ntb = dyn;
-    uint2 *planes = (uint2 *)(dyn + align16(n));
-    const int8_t *g = J.genome[buf] + ((size_t)slot * m + c) * L;
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const int gl = c_syn_start[J.aa[k]] + g[k];
-        ntb[3 * k] = J.nt[3 * gl];
-        ntb[3 * k + 1] = J.nt[3 * gl + 1];
-        ntb[3 * k + 2] = J.nt[3 * gl + 2];
-    }
+    uint64_t *planes = (uint64_t *)dyn;  // the CDS's planes as k_design left them
+    const int nwords = 2 * scan_words(n);
+    const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
     __syncthreads();
     const uint64_t key =
-        stem_cds(ntb, n, planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, &bcast, redu);
+        stem_cds(n, (const uint2 *)planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, &bcast, redu);
     if (threadIdx.x == 0 && key) atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
 }
 

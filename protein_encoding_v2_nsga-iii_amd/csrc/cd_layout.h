@@ -95,6 +95,8 @@ struct DevJob {
     int8_t *oidx[2];       // [P][12]
     int32_t *pql[2];       // [P][4] (p, q, l, 0)
     CdsStat *cst[2];       // [P][m]
+    uint2 *planes;         // [P][m][2][scan_words(3L)] bit planes (sequence, reverse complement) of the CDS
+                           // k_design evaluated, read by k_stem
     int32_t *sorted;       // [N]
     int32_t *front_of;     // [P]
     int32_t *np;           // [P] dominator counts
