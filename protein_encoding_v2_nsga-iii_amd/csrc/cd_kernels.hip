@@ -479,16 +479,22 @@ __device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty) {
     }
 }
 
-__device__ __forceinline__ void slot_draws(const MutArgs &A, uint32_t slot, uint32_t d[6]) {
-    uint32_t a[4], b[4];
+// The draws of one codon slot: d[0] the mutation test (TAG_MUT word 0),
+// d[1..5] the shuffle (TAG_MUT words 1-3, TAG_MUTB words 0-1).  The second
+// block is drawn only for slots that pass the test (draws are functions of
+// their counters, so drawing late changes nothing).
+__device__ __forceinline__ uint32_t slot_test_draw(const MutArgs &A, uint32_t slot, uint32_t a[4]) {
     philox(A.gen, A.child, slot, TAG_MUT, A.J->seed, a);
+    return a[0];
+}
+__device__ __forceinline__ void slot_shuffle_draws(const MutArgs &A, uint32_t slot, const uint32_t a[4], uint32_t d[5]) {
+    uint32_t b[4];
     philox(A.gen, A.child, slot, TAG_MUTB, A.J->seed, b);
-    d[0] = a[0];
-    d[1] = a[1];
-    d[2] = a[2];
-    d[3] = a[3];
-    d[4] = b[0];
-    d[5] = b[1];
+    d[0] = a[1];
+    d[1] = a[2];
+    d[2] = a[3];
+    d[3] = b[0];
+    d[4] = b[1];
 }
 
 // indexArrayShuffling, utils.cuh:203-218: Fisher-Yates over the n <= 6
@@ -515,18 +521,26 @@ __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
     const DevJob &J = *A.J;
     const int L = J.L, m = J.m;
     for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        uint32_t d[6];
-        slot_draws(A, (uint32_t)(cds1 * L + k), d);
+        const uint32_t slot = (uint32_t)(cds1 * L + k);
+        uint32_t a[4], d[5];
         const int ns = A.D->pnum[k];
-        if (!bern(d[0], J.thr_pm) || ns == 1) continue;
-        const uint32_t order = shuffle(d + 1, ns);
+        if (!bern(slot_test_draw(A, slot, a), J.thr_pm) || ns == 1) continue;
+        slot_shuffle_draws(A, slot, a, d);
+        const uint32_t order = shuffle(d, ns);
         const int base = A.D->pbase[k], cur = A.cod[cds1 * L + k] - base;
         const int g2 = A.cod[cds2 * L + k];
+        // the other CDS's codons at k (only cds1's codons change here, each by
+        // its own thread), in registers
+        int oc[MAX_M];
+#pragma unroll
+        for (int o = 0; o < MAX_M; ++o) oc[o] = (o < m && o != cds1) ? (int)A.cod[o * L + k] : -1;
         auto min_hd = [&](int g) -> int {
             int mn = 127;
-            for (int o = 0; o < m; ++o) {
-                if (o == cds1) continue;
-                const int t = A.D->nd[g * NCODON + A.cod[o * L + k]];
+#pragma unroll
+            for (int o = 0; o < MAX_M; ++o) {
+                if (o >= m) break;
+                if (oc[o] < 0) continue;
+                const int t = A.D->nd[g * NCODON + oc[o]];
                 mn = t < mn ? t : mn;
             }
             return mn;
@@ -560,11 +574,12 @@ __device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *
     const int L = J.L;
     const uint64_t thr = bern_threshold(dev);
     for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        uint32_t d[6];
-        slot_draws(A, (uint32_t)(cds * L + k), d);
+        const uint32_t slot = (uint32_t)(cds * L + k);
+        uint32_t a[4], d[5];
         const int ns = A.D->pnum[k];
-        if (!bern(d[0], thr) || ns == 1) continue;
-        const uint32_t order = shuffle(d + 1, ns);
+        if (!bern(slot_test_draw(A, slot, a), thr) || ns == 1) continue;
+        slot_shuffle_draws(A, slot, a, d);
+        const uint32_t order = shuffle(d, ns);
         const int base = A.D->pbase[k], cur = A.cod[cds * L + k] - base;
         const int curg = A.D->gc3[base + cur];
         for (int j = 0; j < ns; ++j) {
