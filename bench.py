@@ -33,7 +33,7 @@ PKG = "protein_encoding_v2_nsga-iii_amd"
 # the scan is measured against.
 VALU_INT32_OPS = 256 * 64 * 2.4e9
 PEAK_CELLS = 32 * VALU_INT32_OPS
-KERNELS = ("offspring", "stem", "stem_finish", "dominance", "fronts", "associate", "niche")
+KERNELS = ("offspring", "stem", "dominance", "fronts", "associate", "niche")
 # HBM bytes per k_stem launch at the default workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (scaled by a 256 MiB device copy in the same run) from
 # the rocprofv3 --pmc runs summarised in profiles/pmc_single_job_L501_m4_N1024.md

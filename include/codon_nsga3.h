@@ -111,9 +111,9 @@ int cd_job_read(cd_job *job, int8_t *genome, float *obj, int8_t *idx, int32_t *p
  * report the mean duration in milliseconds of each kernel, measured with HIP
  * events on the job's stream: kernel_ms[0] offspring (survivor copy,
  * mutation, objectives 0-4), [1] stem scan of the changed CDS, [2] stem
- * objective, [3] dominance matrix, [4] fronts + normalisation,
- * [5] association, [6] niching. */
-#define CD_PROFILE_KERNELS 7
+ * objective and dominance matrix, [3] fronts + normalisation,
+ * [4] association, [5] niching. */
+#define CD_PROFILE_KERNELS 6
 int cd_job_profile(cd_job *job, int32_t generations, float *kernel_ms);
 
 /* Number of CDS whose objectives were recomputed since cd_job_init (children

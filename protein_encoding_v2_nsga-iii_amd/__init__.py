@@ -20,9 +20,9 @@ LIB_PATH = os.path.join(HERE, "libcd_engine.so")
 CSRC = os.path.join(HERE, "csrc")
 
 CD_OK = 0
-# kernels timed by cd_job_profile: offspring, stem, stem_finish, dominance,
-# fronts, associate, niche
-PROFILE_KERNELS = 7
+# kernels timed by cd_job_profile: offspring, stem, dominance (with the stem
+# objective), fronts, associate, niche
+PROFILE_KERNELS = 6
 ERRORS = {-1: "CD_E_INVALID", -2: "CD_E_NOMEM", -3: "CD_E_DEVICE", -4: "CD_E_STATE", -5: "CD_E_BUDGET"}
 
 # Every entry point of include/codon_nsga3.h

@@ -138,7 +138,8 @@ namespace {
 
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
-    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, sorted, front,
+    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, sorted,
+        front,
         np, dom,
         ref_of, dist, niche, flag, st, self, off, domi, asc;
     size_t total;
@@ -164,6 +165,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
         }
         o.off = s.add((size_t)N * sizeof(int2));
         o.planes = s.add((size_t)P * m * cd::cds_plane_bytes(L));
+        o.stem_max = s.add((size_t)P * 4);
     }
     for (int b = 0; b < (sel_only ? 1 : 2); ++b) o.obj[b] = s.add((size_t)P * 6 * 4);
     if (sel_only) o.obj[1] = o.obj[0];
@@ -288,6 +290,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.front_of = (int32_t *)(base + o.front);
     h.np = (int32_t *)(base + o.np);
     h.planes = sel_only ? nullptr : (uint2 *)(base + o.planes);
+    h.stem_max = sel_only ? nullptr : (int32_t *)(base + o.stem_max);
     h.dom = (uint64_t *)(base + o.dom);
     h.ref_of = (int32_t *)(base + o.ref_of);
     h.dist = (float *)(base + o.dist);
@@ -324,7 +327,8 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
     hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
     if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
-    if (e == hipSuccess) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
+    // the offspring generation leaves the stem objective to k_dominance
+    if (e == hipSuccess && mode != cd::MODE_OFFSPRING) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
     return e;
 }
 
@@ -340,7 +344,7 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
     const int n_dom = w.n_dom;
     const int2 *asc = w.asc;
     const int n_asc = w.n_asc;
-    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, s);
+    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, true, s);
     if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
     if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
     if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, w.niche_lds, s);
@@ -514,17 +518,15 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(cd::launch_stem(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->m, J->stem_parts, J->stem_lds,
                                J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
-        HIPCHK(cd::launch_stem_finish(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->stream));
+        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));
-        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, J->stream));
-        HIPCHK(hipEventRecord(ev[4], J->stream));
         HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
-        HIPCHK(hipEventRecord(ev[5], J->stream));
+        HIPCHK(hipEventRecord(ev[4], J->stream));
         HIPCHK(cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream));
-        HIPCHK(hipEventRecord(ev[6], J->stream));
+        HIPCHK(hipEventRecord(ev[5], J->stream));
         HIPCHK(cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream));
-        HIPCHK(hipEventRecord(ev[7], J->stream));
-        HIPCHK(hipEventSynchronize(ev[7]));
+        HIPCHK(hipEventRecord(ev[6], J->stream));
+        HIPCHK(hipEventSynchronize(ev[6]));
         for (int k = 0; k < NK; ++k) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -607,7 +609,7 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     hipError_t e = hipMemcpy(J->h.obj[0], obj, (size_t)P * 24, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(J->h.st, &st, sizeof(st), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
-    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, J->stream);
+    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, J->stream);
     if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->stream);
     if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream);
     if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream);

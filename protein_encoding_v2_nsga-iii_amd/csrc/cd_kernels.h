@@ -30,8 +30,10 @@ hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, i
                        hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
-// items: (job, first row, word, 0)
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, hipStream_t s);
+// items: (job, first row, word, 0).  finish: the offspring generation, whose
+// children's stem objective (and stem records) k_dominance completes in place
+// of launch_stem_finish
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, hipStream_t s);
 // one workgroup per job
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);
 // items: (job, first design)

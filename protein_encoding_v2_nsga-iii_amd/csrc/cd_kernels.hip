@@ -1,12 +1,12 @@
 // cd_kernels.hip -- HIP kernels of the codon-design engine for gfx950.
 //
-// One generation is seven launches over every resident job:
+// One generation is six launches over every resident job:
 //   k_design<OFFSPRING>  one workgroup per child: survivor copy, mutation,
 //                        objectives 0-4 (mutationKernel, main.cu:121-240)
 //   k_stem               1..STEM_PARTS workgroups per (child, CDS): longest
 //                        stem of each CDS the mutation changed (calMaximumSL)
-//   k_stem_finish        one thread per child: objective 5
-//   k_dominance          bit-packed eps-domination matrix (sorting.cuh:994-1023)
+//   k_dominance          objective 5 of the children, bit-packed
+//                        eps-domination matrix (sorting.cuh:994-1023)
 //   k_fronts             one workgroup per job: front peeling, ideal/worst,
 //                        extreme points, hyperplane/MNDF nadir
 //                        (sorting.cuh:974-1089, 132-972)
@@ -891,9 +891,14 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     if (tid < NOBJ - 1) gstore(oobj + tid, s_obj[tid]);  // objective 5: k_stem_finish
     if (tid < 2 * NOBJ && tid != 10) gstore(J.oidx[dst] + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
     if (tid < m) gstore_stat(J.cst[dst] + (size_t)out_slot * m + tid, cst[tid]);
-    if (tid == 0) {  // CDS evaluated (k_stem scans the same ones)
+    if (tid == 0) {  // CDS evaluated (k_stem scans the same ones); the unchanged CDS's longest stem
         unsigned long long nd = 0;
-        for (int c = 0; c < m; ++c) nd += sh.dirty[c] != 0;
+        int lmax = 0;
+        for (int c = 0; c < m; ++c) {
+            nd += sh.dirty[c] != 0;
+            if (!sh.dirty[c]) lmax = max(lmax, cst[c].l);
+        }
+        gstore(J.stem_max + out_slot, lmax);
         atomicAdd(&J.st->cds_scanned, nd);
     }
 }
@@ -944,20 +949,16 @@ __global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, c
     __syncthreads();
     const uint64_t key =
         stem_cds(n, (const uint2 *)planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, &bcast, redu);
-    if (threadIdx.x == 0 && key) atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
+    if (threadIdx.x == 0 && key) {
+        atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
+        atomicMax(&J.stem_max[slot], (int)(key >> 40));
+    }
 }
 
-// Objective 5 of each work item's design: the longest stem over its CDS,
-// ties to the last CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880).
-// Stores the stems k_stem found and clears the marks k_design left.
-__global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
-                                                     int n_items, int mode) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_items) return;
-    const int2 it = items[i];
-    const DevJob &J = jobs[it.x];
-    int buf, slot;
-    item_slot(J, mode, it.y, buf, slot);
+// Objective 5 of a design: the longest stem over its CDS, ties to the last
+// CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880).  Stores the
+// stems k_stem found and clears the marks k_design left.
+__device__ void stem_finish(const DevJob &J, int buf, int slot) {
     const int m = J.m;
     CdsStat *cs = J.cst[buf] + (size_t)slot * m;
     int bl = 0, bp = 0, bq = 0, bc = m - 1;
@@ -982,6 +983,19 @@ __global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ 
     pq[1] = bq;
     pq[2] = bl;
     pq[3] = 0;
+}
+
+// stem_finish for every work item's design (initialisation and evaluation;
+// the offspring generation leaves it to k_dominance).
+__global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
+                                                     int n_items, int mode) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    const int2 it = items[i];
+    const DevJob &J = jobs[it.x];
+    int buf, slot;
+    item_slot(J, mode, it.y, buf, slot);
+    stem_finish(J, buf, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,22 +1023,34 @@ __device__ __forceinline__ bool dominates(const float *a, const float *b) {
 // ballot of the 64 tests against that row (row values read from LDS as
 // broadcasts), and the wave stores its 16 words and adds their counts at
 // once.
-__global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items) {
+__global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items,
+                                                   bool finish) {
     constexpr int WR = DOM_ROWS / 4;  // rows per wave
     __shared__ float rows[DOM_ROWS * NOBJ];
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
-    const int P = J.P, word = it.z;
-    const float *obj = J.obj[pool_buffer(J)];
+    const int P = J.P, N = J.N, word = it.z, buf = pool_buffer(J);
+    const float *obj = J.obj[buf];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // in the offspring generation a child's objective 5 is its longest stem
+    // over (3L-4)/2, the value stem_finish stores
+    auto obj_at = [&](int d, int k) -> float {
+        if (finish && k == NOBJ - 1 && d >= N) return (float)gload(J.stem_max + d) / (float)J.den_sl;
+        return gload(obj + (size_t)d * NOBJ + k);
+    };
     for (int t = threadIdx.x; t < DOM_ROWS * NOBJ; t += blockDim.x) {
         const int r = it.y + t / NOBJ;
-        rows[t] = r < P ? gload(obj + (size_t)it.y * NOBJ + t) : 0.f;
+        rows[t] = r < P ? obj_at(r, t % NOBJ) : 0.f;
     }
     const int i = word * 64 + lane;
     float cand[NOBJ];
 #pragma unroll
-    for (int k = 0; k < NOBJ; ++k) cand[k] = i < P ? gload(obj + (size_t)i * NOBJ + k) : 0.f;
+    for (int k = 0; k < NOBJ; ++k) cand[k] = i < P ? obj_at(i, k) : 0.f;
+    // the workgroups of word 0 complete the stem records of their children
+    if (finish && word == 0 && threadIdx.x < DOM_ROWS) {
+        const int r = it.y + (int)threadIdx.x;
+        if (r >= N && r < P) stem_finish(J, buf, r);
+    }
     __syncthreads();
     const int r0 = it.y + wv * WR;
     uint64_t mine = 0;  // lane q < WR: the word of row r0 + q
@@ -1688,9 +1714,9 @@ hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, i
     hipLaunchKernelGGL(k_stem_finish, dim3((n + 255) / 256), dim3(256), 0, s, jobs, items, n, mode);
     return hipGetLastError();
 }
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, hipStream_t s) {
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items);
+    hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items, finish);
     return hipGetLastError();
 }
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {

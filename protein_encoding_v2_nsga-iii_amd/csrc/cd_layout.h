@@ -100,6 +100,7 @@ struct DevJob {
     int32_t *sorted;       // [N]
     int32_t *front_of;     // [P]
     int32_t *np;           // [P] dominator counts
+    int32_t *stem_max;     // [P] longest stem of the design (k_design: unchanged CDS, k_stem: atomic maximum)
     uint64_t *dom;         // [P][W]: bit i of row j set when i dominates j
     int32_t *ref_of;       // [P]
     float *dist;           // [P]
