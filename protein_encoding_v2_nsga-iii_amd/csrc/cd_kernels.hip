@@ -927,7 +927,9 @@ __device__ __forceinline__ void item_slot(const DevJob &J, int mode, int y, int 
 // (children only), and each part folds its best key into the record with an
 // atomic maximum; k_stem_finish decodes it.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_stem(const DevJob *__restrict__ jobs, const int2 *__restrict__ items, int mode,
+// Six waves per SIMD (at most 80 VGPRs) keep every workgroup of a generation
+// of ~1300 changed CDS resident at once on 256 CUs.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stem(const DevJob *__restrict__ jobs, const int2 *__restrict__ items, int mode,
                                               int mg, int nic) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];

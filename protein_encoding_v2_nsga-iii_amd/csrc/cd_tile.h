@@ -159,10 +159,10 @@ __host__ __device__ inline void tile_hist_reset(TileHist &t) {
 // Rows u0 + J0 .. u0 + 31 of block (h, base): X is the plane read along the
 // diagonals (rc for half A, s for half B), Y the plane of the row's own base.
 // Sets f[0], f[1], f[2] to the diagonals with a window of 4, 7, 8 matches
-// ending in these rows and updates the history.  A 7-window is a 4-window
+// ending in these rows (with ALL false only f[2]) and updates the history.  A 7-window is a 4-window
 // and the 4-window ending three rows earlier, an 8-window the one ending
 // four rows earlier (one three-input logic operation each per row).
-template <int J0>
+template <int J0, bool ALL = true>
 __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int base, int u0, TileHist &t,
                                           uint32_t f[3]) {
     static_assert((32 - J0) % 8 == 0, "rows in groups of 8");
@@ -184,8 +184,10 @@ __host__ __device__ inline void tile_rows(const uint2 *X, const uint2 *Y, int ba
             const uint32_t m = ~((xl ^ yl) | (xh ^ yh));
             const uint32_t a2 = m & m1;
             const uint32_t a4 = a2 & a2_2;
-            f4 |= a4;
-            f7 |= a4 & a4_3;
+            if (ALL) {
+                f4 |= a4;
+                f7 |= a4 & a4_3;
+            }
             f8 |= a4 & a4_4;
             m1 = m;
             a2_2 = a2_1;
@@ -237,7 +239,7 @@ __host__ __device__ inline void tile_seek(const uint2 *sp, const uint2 *rp, int 
     if (c.g > 0) {  // the last 8 rows of the previous tile of the block
         const uint2 *X = c.h ? sp : rp, *Y = c.h ? rp : sp;
         uint32_t f[3];
-        tile_rows<24>(X, Y, 32 * c.b, 32 * (c.g - 1), c.hist, f);
+        tile_rows<24, false>(X, Y, 32 * c.b, 32 * (c.g - 1), c.hist, f);
     }
 }
 
@@ -248,7 +250,10 @@ __host__ __device__ inline void tile_step(const uint2 *sp, const uint2 *rp, int 
     const int base = 32 * c.b, u0 = 32 * c.g;
     const int W = K >= 8 ? 8 : (K == 7 ? 7 : TILE_MIN_RUN);  // K >= TILE_MIN_RUN
     uint32_t fw[3];
-    tile_rows<0>(X, Y, base, u0, c.hist, fw);
+    if (W == 8)  // K is the same in the whole wave here, so this branch is too
+        tile_rows<0, false>(X, Y, base, u0, c.hist, fw);
+    else
+        tile_rows<0, true>(X, Y, base, u0, c.hist, fw);
     uint32_t f = W == 8 ? fw[2] : (W == 7 ? fw[1] : fw[0]);
     while (f) {
         const int k = __builtin_ctz(f);
