@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     const uint32_t gen = st->gen;
     const int last = st->last;
     __shared__ int red[16];
-    __shared__ int red16[16 * 16], cq[16];
+    __shared__ int red64[16 * 64], s_lo, s_clo, s_hi;
     __shared__ int s_carry;
     const int tier = niche_tier(P, R);
     const int Pa = niche_stride(P);
@@ -1541,46 +1541,47 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         const int need = N - st->base;
         // picks made through level i: C(i) = sum_r clamp(i + 1 - cnt[r], 0, rem[r]),
         // nondecreasing in i; the final level is the smallest i with C(i) >= need
-        // (or P).  A 16-way search: each pass evaluates C at 16 evenly spaced
-        // levels of the open interval (lo, hi) with one workgroup reduction,
-        // keeping C(lo) < need and C(hi) >= need or hi = P.
+        // (or P).  A 64-way search: each pass evaluates C at 64 evenly spaced
+        // levels of the open interval (lo, hi), lane q of every wave one level
+        // over its wave's share of the reference points, keeping C(lo) < need
+        // and C(hi) >= need or hi = P (two passes for P up to 4224).
         int lo = -1, clo = 0, hi = P;
         while (hi - lo > 1) {
-            const int step = (hi - lo + 16) / 17;
-            int acc[16];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[q] = 0;
-            for (int r = tid; r < R; r += NT) {
-                const int c = cnt[r], m = rem[r];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) acc[q] += min(max(min(lo + step * (q + 1), hi) + 1 - c, 0), m);
-            }
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[q] = wave_sum_i32(acc[q]);
-            if (lane == 0)
-#pragma unroll
-                for (int q = 0; q < 16; ++q) red16[wid * 16 + q] = acc[q];
+            const int step = (hi - lo + 64) / 65;
+            const int iq = min(lo + step * (lane + 1), hi);
+            int acc = 0;
+            for (int r = wid; r < R; r += nw) acc += min(max(iq + 1 - cnt[r], 0), rem[r]);
+            red64[wid * 64 + lane] = acc;
             __syncthreads();
-            if (tid < 16) {
-                int v = 0;
-                for (int w = 0; w < nw; ++w) v += red16[w * 16 + tid];
-                cq[tid] = v;
-            }
-            __syncthreads();
-            int nlo = lo, nclo = clo, nhi = hi;
-            for (int q = 0; q < 16; ++q) {
-                const int i = lo + step * (q + 1);
-                if (i >= hi) break;
-                if (cq[q] >= need) {
-                    nhi = i;
-                    break;
+            if (wid == 0) {
+                int C = 0;
+                for (int w = 0; w < nw; ++w) C += red64[w * 64 + lane];
+                const bool valid = lo + step * (lane + 1) < hi;
+                const uint64_t hit = __ballot(valid && C >= need), val = __ballot(valid);
+                int nlo = lo, nclo = clo, nhi = hi;
+                if (hit) {  // the first probe at or past need; the one before it is below
+                    const int qf = __builtin_ctzll(hit);
+                    const int cprev = __shfl(C, qf > 0 ? qf - 1 : 0, 64);
+                    nhi = lo + step * (qf + 1);
+                    if (qf > 0) {
+                        nlo = lo + step * qf;
+                        nclo = cprev;
+                    }
+                } else {  // every probe is below need: the last one becomes lo
+                    const int ql = 63 - __builtin_clzll(val);
+                    nclo = __shfl(C, ql, 64);
+                    nlo = lo + step * (ql + 1);
                 }
-                nlo = i;
-                nclo = cq[q];
+                if (lane == 0) {
+                    s_lo = nlo;
+                    s_clo = nclo;
+                    s_hi = nhi;
+                }
             }
-            lo = nlo;
-            clo = nclo;
-            hi = nhi;
+            __syncthreads();
+            lo = s_lo;
+            clo = s_clo;
+            hi = s_hi;
         }
         const int istar = hi, prevC = clo;
         const int need_final = need - prevC;
@@ -1596,23 +1597,33 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
             if (tid == 0) s_carry += total;
             __syncthreads();
         }
-        // level 0 of an empty niche takes its nearest candidate (ties to the
-        // lowest index): one wave per reference point with picks
-        {
-            for (int r = wid; r < R; r += nw) {
-                int l0 = -1;
-                if (kp[r] > 0 && cnt[r] == 0) {
-                    const int *b = lb + off[r];
-                    const int nb = rem[r];
-                    float bd = FLT_MAX;
-                    int bi = 0x7FFFFFFF;
-                    for (int t = lane; t < nb; t += 64) argmin_pair(bd, bi, ld ? ld[off[r] + t] : J.dist[b[t]], b[t]);
-                    wave_argmin(bd, bi);
-                    // every distance FLT_MAX gives the lowest index, as a sweep in order would
-                    l0 = bi != 0x7FFFFFFF ? bi : b[0];
+        // level 0 of an empty niche takes its nearest candidate, ties to the
+        // lowest index; a sweep in ascending index starts from the lowest index
+        // and keeps it when its distance is NaN or nothing compares below
+        // FLT_MAX.  One wave per reference point with picks.
+        for (int r = wid; r < R; r += nw) {
+            int l0 = -1;
+            if (kp[r] > 0 && cnt[r] == 0) {
+                const int *b = lb + off[r];
+                const int nb = rem[r];
+                float bd = FLT_MAX, md = 0.f;
+                int bi = 0x7FFFFFFF, mi = 0x7FFFFFFF;
+                for (int t = lane; t < nb; t += 64) {
+                    const int e = b[t];
+                    const float de = ld ? ld[off[r] + t] : J.dist[e];
+                    argmin_pair(bd, bi, de, e);
+                    if (e < mi) {
+                        mi = e;
+                        md = de;
+                    }
                 }
-                if (lane == 0) lvl0[r] = l0;
+                wave_argmin(bd, bi);
+                const int lo_i = wave_min_i32(mi);
+                const uint64_t who = __ballot(mi == lo_i);
+                md = __shfl(md, (int)__builtin_ctzll(who), 64);
+                l0 = (bi == 0x7FFFFFFF || isnan(md)) ? lo_i : bi;
             }
+            if (lane == 0) lvl0[r] = l0;
         }
         for (int d = tid; d < P; d += NT) {
             const int f = fo[d];
