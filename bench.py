@@ -37,7 +37,7 @@ KERNELS = ("offspring", "stem", "dominance", "fronts", "associate", "niche")
 # HBM bytes per k_stem launch at the default workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (scaled by a 256 MiB device copy in the same run) from
 # the rocprofv3 --pmc runs summarised in profiles/pmc_single_job_L501_m4_N1024.md
-K_STEM_TRAFFIC_BYTES = int((847.3 * 2.000 + 42.3) * 1024)
+K_STEM_TRAFFIC_BYTES = int((1009.5 * 2.000 + 84.6) * 1024)
 
 
 def log(msg):
