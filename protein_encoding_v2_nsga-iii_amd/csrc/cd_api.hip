@@ -326,7 +326,14 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
 // the stem of every changed CDS, objective 5.
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
     hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
-    if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
+    if (mode == cd::MODE_OFFSPRING && w.n_stem_cls > 0) {  // items is the children list, ordered by run
+        for (int k = 0; k < w.n_stem_cls && e == hipSuccess; ++k) {
+            const cdhost::StemClass &c = w.stem_cls[k];
+            e = cd::launch_stem(mode, w.jobs, items + c.first, c.count, c.m, c.parts, c.lds, s);
+        }
+    } else if (e == hipSuccess) {
+        e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
+    }
     // the offspring generation leaves the stem objective to k_dominance
     if (e == hipSuccess && mode != cd::MODE_OFFSPRING) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
     return e;
@@ -353,10 +360,24 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
     wl = WorkLists();
+    // children ordered by (m, k_stem workgroups per CDS), one run per shape
+    std::vector<int> order(count);
+    for (int j = 0; j < count; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return std::make_pair(jobs[a]->m, jobs[a]->stem_parts) < std::make_pair(jobs[b]->m, jobs[b]->stem_parts);
+    });
+    for (int j : order) {
+        cd_job *J = jobs[j];
+        if (wl.stem_cls.empty() || wl.stem_cls.back().m != J->m || wl.stem_cls.back().parts != J->stem_parts)
+            wl.stem_cls.push_back(StemClass{(int)wl.off.size(), 0, J->m, J->stem_parts, 0});
+        StemClass &c = wl.stem_cls.back();
+        c.count += J->N;
+        c.lds = std::max(c.lds, J->stem_lds);
+        for (int k = 0; k < J->N; ++k) wl.off.push_back(make_int2(j, k));
+    }
     for (int j = 0; j < count; ++j) {
         cd_job *J = jobs[j];
         wl.descs.push_back(J->h);
-        for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
         for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
             for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS, w, 0));
         for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
@@ -683,6 +704,7 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     B->stem_lds = wl.stem_lds;
     B->niche_lds = wl.niche_lds;
     B->mg = wl.mg;
+    B->stem_cls = wl.stem_cls;
     B->stem_parts = wl.stem_parts;
     B->n_off = (int)wl.off.size();
     B->n_dom = (int)wl.dom.size();
@@ -709,7 +731,8 @@ int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
         const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom, B->n_dom, B->d_asc, B->n_asc,
-                       B->lds, B->stem_lds, B->niche_lds, B->mg, B->stem_parts};
+                       B->lds, B->stem_lds, B->niche_lds, B->mg, B->stem_parts, (int)B->stem_cls.size(),
+                       B->stem_cls.data()};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }

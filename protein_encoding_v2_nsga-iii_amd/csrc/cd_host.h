@@ -12,6 +12,16 @@
 #include "codon_nsga3.h"
 
 // One job: tables, pools and selection state in one device allocation.
+namespace cdhost {
+// A run of children of jobs with the same CDS count and k_stem workgroups per
+// CDS: k_stem is launched once per run, with that exact shape.
+struct StemClass {
+    int first, count;  // range of the children list
+    int m, parts;
+    size_t lds;
+};
+}  // namespace cdhost
+
 struct cd_job {
     cd::DevJob h{};
     cd::DevJob *d_self = nullptr;
@@ -41,6 +51,7 @@ struct cd_batch {
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0;          // largest m of the batch's jobs
     int stem_parts = 0;  // largest k_stem workgroups per CDS of the batch's jobs
+    std::vector<cdhost::StemClass> stem_cls;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -74,6 +85,10 @@ struct Launch {
     size_t lds, stem_lds, niche_lds;
     int mg;          // largest m
     int stem_parts;  // largest k_stem workgroups per CDS
+    // k_stem of the children in runs of one shape (the children list is
+    // ordered by run); none: one launch of the largest shape
+    int n_stem_cls;
+    const StemClass *stem_cls;
 };
 
 // Host images of the work lists of `jobs` (job k = index k of the list)
@@ -84,6 +99,7 @@ struct WorkLists {
     std::vector<int4> dom;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0, stem_parts = 0;
+    std::vector<StemClass> stem_cls;
 };
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl);
