@@ -20,6 +20,7 @@
 
 #include "cd_device.h"
 #include "cd_kernels.h"
+#include "cd_qgram.h"
 #include "cd_scan.h"
 #include "cd_tile.h"
 
@@ -47,8 +48,12 @@ size_t design_lds_bytes(int L, int m) {
 // scan_words(3L) words of (low bits, high bits) of the nucleotide codes
 size_t cds_plane_bytes(int L) { return (size_t)2 * scan_words(3 * L) * sizeof(uint2); }
 
-// dynamic LDS of k_stem: the bit planes of one CDS
-size_t stem_lds_bytes(int L) { return cds_plane_bytes(L); }
+// q-gram index of k_stem after the planes: QG_CODES chain heads, then the
+// chain links of at most qg_index_max(n) entries
+__host__ __device__ inline int qg_lds_bytes(int n) { return QG_CODES * 4 + align16(2 * qg_index_max(n)); }
+
+// dynamic LDS of k_stem: the bit planes of one CDS and the q-gram index
+size_t stem_lds_bytes(int L) { return cds_plane_bytes(L) + qg_lds_bytes(3 * L); }
 
 struct DesignShared {
     float pobj[NOBJ];
@@ -136,23 +141,72 @@ __device__ __forceinline__ int stem_seed(const uint2 *sp, int n, int l, int p, i
     return run;
 }
 
+// q-gram search (cd_qgram.h) of the rc positions [b0, b1) for the runs of at
+// least K: the workgroup builds the index of s in LDS (head, next), and each
+// lane looks up every NT-th position within its hit budget.  Sets *best to
+// the workgroup's maximum key and returns whether every lookup finished.
+__device__ bool stem_qgram(int n, const uint2 *sp, const uint2 *rp, int K, int b0, int b1, uint32_t *head,
+                           int32_t *flag, uint64_t *redu, uint64_t &best) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int st = qg_stride(K), NI = qg_index_count(n, st);
+    uint16_t *next = (uint16_t *)(head + QG_CODES);
+    uint4 *h4 = (uint4 *)head;
+    for (int i = tid; i < QG_CODES / 4; i += NT) h4[i] = make_uint4(QG_NIL, QG_NIL, QG_NIL, QG_NIL);
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int k = tid; k < NI; k += NT) qg_insert(sp, st, k, head, next);
+    __syncthreads();
+    int budget = qg_budget((b1 - b0 + NT - 1) / NT);
+    uint64_t b = 0;
+    bool ok = true;
+    for (int x = b0 + tid; x < b1 && ok; x += NT) ok = qg_lookup(sp, rp, n, st, K, x, head, next, budget, b);
+    if (!ok) *flag = 1;
+    best = block_max_u64(b, redu);  // its barriers order the flag writes before the read
+    return *flag == 0;
+}
+
 __device__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_from, int part, int parts, int32_t *bcast,
-                             uint64_t *redu) {
+                             uint64_t *redu, uint32_t *qhead) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const int W = scan_words(n);
     const uint2 *sp = planes, *rp = planes + W;
     if (wid == 0) {
         const int seed = seed_from ? stem_seed(sp, n, seed_from->l, seed_from->p, seed_from->q, lane) : 0;
-        if (lane == 0) *bcast = seed;
+        if (lane == 0) bcast[0] = seed;
     }
     __syncthreads();
+    const int seed = bcast[0];
+    // A changed CDS whose seed is 7 or more is searched by q-grams for the
+    // runs of at least the seed, the parts sharing the lookups: exact.  With
+    // a smaller seed, the runs of at least the parent's stem length less 2
+    // are searched, in full by every part: exact when one is found (a stem
+    // that long exists), else the tiles follow.  A part whose lookups run out
+    // of budget scans the tiles: all of them if the lookups were shared.
+    int tpart = part, tparts = parts;
+    if (seed_from) {
+        const int Kq = qg_search_k(n, seed, seed_from->l);
+        if (Kq) {
+            const bool split = seed >= QG_KMIN;
+            const int nb = n - QG_Q + 1;
+            const int b0 = split ? (int)((int64_t)part * nb / parts) : 0;
+            const int b1 = split ? (int)((int64_t)(part + 1) * nb / parts) : nb;
+            uint64_t qbest;
+            if (stem_qgram(n, sp, rp, Kq, b0, b1, qhead, bcast + 1, redu, qbest)) {
+                if (split) return qbest;
+                if ((int)(qbest >> 40) >= Kq) return part == 0 ? qbest : 0;
+            } else if (split) {
+                tpart = 0;
+                tparts = 1;
+            }
+        }
+    }
     // Every lane's K is a length some stem of this CDS has, so the wave may
     // share the largest after each step.  Trip counts are the same for every
     // lane, so every lane takes part in the exchanges.
-    int K = max(*bcast, TILE_MIN_RUN);
+    int K = max(seed, TILE_MIN_RUN);
     uint64_t best = 0;
     {
-        const int TT = 2 * tile_prefix(n, tile_blocks(n)), T = parts * NT, L = part * NT + tid;
+        const int TT = 2 * tile_prefix(n, tile_blocks(n)), T = tparts * NT, L = tpart * NT + tid;
         const int t0 = (int)((int64_t)L * TT / T), t1 = (int)((int64_t)(L + 1) * TT / T);
         const int trips = (TT + T - 1) / T;
         TileCursor cur;
@@ -168,11 +222,11 @@ __device__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_fro
     // With a seed of 4 or more the longest stem is at least 4 long, so the
     // share holding it measures it; a share without a flag holds nothing
     // longer.  Without one, a share with no run of 4 is scanned exactly.
-    if (*bcast < TILE_MIN_RUN && (int)(best >> 40) < TILE_MIN_RUN) {
+    if (seed < TILE_MIN_RUN && (int)(best >> 40) < TILE_MIN_RUN) {
         best = 0;
         K = 1;
-        const int stride = parts * NT, S0 = MIN_HAIRPIN + 1 + part * NT + tid;
-        const int Sw = MIN_HAIRPIN + 1 + part * NT + (tid & ~63), last = 2 * n - 3;
+        const int stride = tparts * NT, S0 = MIN_HAIRPIN + 1 + tpart * NT + tid;
+        const int Sw = MIN_HAIRPIN + 1 + tpart * NT + (tid & ~63), last = 2 * n - 3;
         const int trips = Sw <= last ? (last - Sw) / stride + 1 : 0;
         for (int j = 0; j < trips; ++j) {
             const int S = S0 + j * stride;
@@ -933,7 +987,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                               int mg, int nic) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];
-    __shared__ int32_t bcast;
+    __shared__ int32_t bcast[2];
     const int part = (int)(blockIdx.x / (unsigned)nic), ic = (int)(blockIdx.x % (unsigned)nic);
     const int2 it = items[ic / mg];
     const int c = ic % mg;
@@ -949,8 +1003,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
     for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
     __syncthreads();
-    const uint64_t key =
-        stem_cds(n, (const uint2 *)planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, &bcast, redu);
+    const uint64_t key = stem_cds(n, (const uint2 *)planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, bcast,
+                                  redu, (uint32_t *)(planes + nwords));
     if (threadIdx.x == 0 && key) {
         atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
         atomicMax(&J.stem_max[slot], (int)(key >> 40));

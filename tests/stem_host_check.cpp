@@ -1,35 +1,45 @@
-// Host check of the engine's stem scans (csrc/cd_scan.h, csrc/cd_tile.h):
-// reads sequences of nucleotide codes from stdin (one per line, digits 0-3,
-// optionally followed by a space and a seed length) and runs the lane loops
-// of the stem kernel (stem_cds in cd_kernels.hip) for PARTS workgroups of NT
-// lanes.  Mode "diag": lane t of part p takes anti-diagonals
-// S = 5 + p*NT + t + j*PARTS*NT.  Mode "tile": lane L = p*NT + t takes the
-// flat tiles [L*TT/T, (L+1)*TT/T), and a part with no seed of 4 and no run of
-// 4 in its tiles falls back to the "diag" lanes of that part with K = 1.  In
-// both, after every trip the 64 lanes of a wave share the largest K, as the
-// kernel's shuffle does.  Prints "l p q" of the best stem per sequence.  Planes are allocated at exactly their
-// documented size so that the address sanitizer catches any read outside
+// Host check of the engine's stem scans (csrc/cd_scan.h, csrc/cd_tile.h,
+// csrc/cd_qgram.h): reads sequences of nucleotide codes from stdin (one per
+// line, digits 0-3, optionally followed by a seed length and the parent's
+// stem length) and runs the lane loops of the stem kernel (stem_cds in
+// cd_kernels.hip) for PARTS workgroups of NT lanes.  Mode "diag": lane t of
+// part p takes anti-diagonals S = 5 + p*NT + t + j*PARTS*NT.  Mode "tile":
+// lane L = p*NT + t takes the flat tiles [L*TT/T, (L+1)*TT/T), and a part
+// with no seed of 4 and no run of 4 in its tiles falls back to the "diag"
+// lanes of that part with K = 1.  In both, after every trip the 64 lanes of a
+// wave share the largest K, as the kernel's shuffle does.  Mode "qgram": the
+// q-gram search of a changed CDS as stem_cds runs it (K from the seed or the
+// parent's length, lookups shared by the parts or searched in full, tiles
+// after an unfinished or fruitless search); an optional fifth argument
+// replaces every lane's hit budget.  Prints "l p q" of the best stem per
+// sequence.  Planes and index arrays are allocated at exactly their
+// documented size so that the address sanitizer catches any access outside
 // them.
 #include <cstdio>
+#include <cstdlib>
 #include <iostream>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "cd_qgram.h"
 #include "cd_scan.h"
 #include "cd_tile.h"
 
 int main(int argc, char **argv) {
     const int NT = argc > 1 ? std::stoi(argv[1]) : 256;  // a multiple of 64, or fewer than 64
     const int PARTS = argc > 2 ? std::stoi(argv[2]) : 1;
-    const bool TILED = argc > 3 && std::string(argv[3]) == "tile";
+    const std::string MODE = argc > 3 ? argv[3] : "diag";
+    const bool TILED = MODE == "tile" || MODE == "qgram", QGRAM = MODE == "qgram";
+    const int BUDGET = argc > 4 ? std::stoi(argv[4]) : -1;
     const int MIN_HAIRPIN = 4;
+    long n_qgram = 0, n_miss = 0, n_overflow = 0;  // q-gram parts: finished, guessed K not found, out of budget
     std::string line;
     while (std::getline(std::cin, line)) {
         std::istringstream in(line);
         std::string seq;
-        int seed = 0;
-        in >> seq >> seed;
+        int seed = 0, parent_l = 0;
+        in >> seq >> seed >> parent_l;
         const int n = (int)seq.size();
         std::vector<uint8_t> s(n);
         for (int i = 0; i < n; ++i) s[i] = (uint8_t)(seq[i] - '0');
@@ -58,11 +68,12 @@ int main(int argc, char **argv) {
                 }
                 if (b < nblk) pre += cd::tile_groups(n, 32 * b);
             }
-            const int TT = 2 * cd::tile_prefix(n, nblk), T = PARTS * NT;
-            const int trips = TT > 0 ? (TT + T - 1) / T : 0;
-            const int stride = PARTS * NT, last = 2 * n - 3;
-            for (int part = 0; part < PARTS; ++part) {
-                uint64_t bp = 0;  // the part's workgroup maximum
+            // the tile share of part `part` of `parts` (its workgroup maximum)
+            auto tile_part = [&](int part, int parts) -> uint64_t {
+                const int TT = 2 * cd::tile_prefix(n, nblk), T = parts * NT;
+                const int trips = TT > 0 ? (TT + T - 1) / T : 0;
+                const int stride = parts * NT, last = 2 * n - 3;
+                uint64_t bp = 0;
                 for (int w0 = 0; w0 < NT; w0 += 64) {
                     const int lanes = NT - w0 < 64 ? NT - w0 : 64;
                     std::vector<int> K(lanes, seed > cd::TILE_MIN_RUN ? seed : cd::TILE_MIN_RUN);
@@ -104,6 +115,60 @@ int main(int argc, char **argv) {
                         for (int l = 0; l < lanes; ++l) bp = b[l] > bp ? b[l] : bp;
                     }
                 }
+                return bp;
+            };
+            // the q-gram search of rc positions [b0, b1) by the part's NT lanes;
+            // returns whether every lookup finished, the maximum key in *qbest
+            auto qgram_part = [&](int K, int b0, int b1, uint64_t *qbest) -> bool {
+                const int st = cd::qg_stride(K), NI = cd::qg_index_count(n, st);
+                if (NI > cd::qg_index_max(n)) {
+                    std::printf("index larger than its LDS share n=%d K=%d\n", n, K);
+                    std::exit(1);
+                }
+                std::vector<uint32_t> head(cd::QG_CODES, cd::QG_NIL);
+                std::vector<uint16_t> next(NI);
+                for (int k = 0; k < NI; ++k) cd::qg_insert(sp.data(), st, k, head.data(), next.data());
+                bool ok = true;
+                uint64_t bq = 0;
+                for (int t = 0; t < NT; ++t) {
+                    int budget = BUDGET >= 0 ? BUDGET : cd::qg_budget((b1 - b0 + NT - 1) / NT);
+                    bool lane_ok = true;
+                    for (int x = b0 + t; x < b1 && lane_ok; x += NT)
+                        lane_ok = cd::qg_lookup(sp.data(), rp.data(), n, st, K, x, head.data(), next.data(), budget, bq);
+                    ok = ok && lane_ok;
+                }
+                *qbest = bq;
+                return ok;
+            };
+            for (int part = 0; part < PARTS; ++part) {
+                int tpart = part, tparts = PARTS;
+                uint64_t bp = 0;
+                bool done = false;
+                const int Kq = QGRAM ? cd::qg_search_k(n, seed, parent_l) : 0;
+                if (Kq) {
+                    const bool split = seed >= cd::QG_KMIN;
+                    const int nb = n - cd::QG_Q + 1;
+                    const int b0 = split ? (int)((int64_t)part * nb / PARTS) : 0;
+                    const int b1 = split ? (int)((int64_t)(part + 1) * nb / PARTS) : nb;
+                    uint64_t qb = 0;
+                    if (qgram_part(Kq, b0, b1, &qb)) {
+                        if (split) {
+                            bp = qb;
+                            done = true;
+                        } else if ((int)(qb >> 40) >= Kq) {
+                            bp = part == 0 ? qb : 0;
+                            done = true;
+                        }
+                        ++(done ? n_qgram : n_miss);
+                    } else {
+                        ++n_overflow;
+                        if (split) {
+                            tpart = 0;
+                            tparts = 1;
+                        }
+                    }
+                }
+                if (!done) bp = tile_part(tpart, tparts);
                 best = bp > best ? bp : best;
             }
         } else {
@@ -134,5 +199,6 @@ int main(int argc, char **argv) {
         }
         std::printf("%d %d %d\n", l, p, q);
     }
+    std::fprintf(stderr, "qgram %ld miss %ld overflow %ld\n", n_qgram, n_miss, n_overflow);
     return 0;
 }
