@@ -138,7 +138,8 @@ namespace {
 
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
-    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, sorted,
+    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
+        sctl, sorted,
         front,
         np, dom,
         ref_of, dist, niche, flag, st, self, off, domi, asc;
@@ -166,6 +167,8 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
         o.off = s.add((size_t)N * sizeof(int2));
         o.planes = s.add((size_t)P * m * cd::cds_plane_bytes(L));
         o.stem_max = s.add((size_t)P * 4);
+        o.slist = s.add((size_t)N * m * 4);
+        o.sctl = s.add(16);
     }
     for (int b = 0; b < (sel_only ? 1 : 2); ++b) o.obj[b] = s.add((size_t)P * 6 * 4);
     if (sel_only) o.obj[1] = o.obj[0];
@@ -277,6 +280,8 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
         int2 *off = (int2 *)&host[o.off];
         for (int c = 0; c < J->N; ++c) off[c] = make_int2(0, c);
         J->d_off = (int2 *)(base + o.off);
+        J->d_slist = (uint32_t *)(base + o.slist);  // the count in d_sctl starts at zero with the image
+        J->d_sctl = (int32_t *)(base + o.sctl);
         J->lds = cd::design_lds_bytes(J->L, J->m);
         J->stem_lds = cd::stem_lds_bytes(J->L);
         J->stem_parts = cd::stem_parts_of(J->L);
@@ -325,17 +330,14 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
 // Evaluate the designs of `items`: mutation (children) and objectives 0-4,
 // the stem of every changed CDS, objective 5.
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
-    hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
-    if (mode == cd::MODE_OFFSPRING && w.n_stem_cls > 0) {  // items is the children list, ordered by run
-        for (int k = 0; k < w.n_stem_cls && e == hipSuccess; ++k) {
-            const cdhost::StemClass &c = w.stem_cls[k];
-            e = cd::launch_stem(mode, w.jobs, items + c.first, c.count, c.m, c.parts, c.lds, s);
-        }
-    } else if (e == hipSuccess) {
-        e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
+    if (mode == cd::MODE_OFFSPRING) {  // items is the children list; k_dominance finishes the stem objective
+        hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s, w.stem_list, w.stem_ctl, w.n_cds);
+        if (e == hipSuccess) e = cd::launch_stem_list(w.jobs, items, w.stem_list, w.stem_ctl, w.n_cds, w.stem_lds, s);
+        return e;
     }
-    // the offspring generation leaves the stem objective to k_dominance
-    if (e == hipSuccess && mode != cd::MODE_OFFSPRING) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
+    hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
+    if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
+    if (e == hipSuccess) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
     return e;
 }
 
@@ -351,7 +353,7 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
     const int n_dom = w.n_dom;
     const int2 *asc = w.asc;
     const int n_asc = w.n_asc;
-    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, true, s);
+    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, true, w.stem_ctl, s);
     if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
     if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
     if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, w.niche_lds, s);
@@ -360,24 +362,11 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
     wl = WorkLists();
-    // children ordered by (m, k_stem workgroups per CDS), one run per shape
-    std::vector<int> order(count);
-    for (int j = 0; j < count; ++j) order[j] = j;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return std::make_pair(jobs[a]->m, jobs[a]->stem_parts) < std::make_pair(jobs[b]->m, jobs[b]->stem_parts);
-    });
-    for (int j : order) {
-        cd_job *J = jobs[j];
-        if (wl.stem_cls.empty() || wl.stem_cls.back().m != J->m || wl.stem_cls.back().parts != J->stem_parts)
-            wl.stem_cls.push_back(StemClass{(int)wl.off.size(), 0, J->m, J->stem_parts, 0});
-        StemClass &c = wl.stem_cls.back();
-        c.count += J->N;
-        c.lds = std::max(c.lds, J->stem_lds);
-        for (int k = 0; k < J->N; ++k) wl.off.push_back(make_int2(j, k));
-    }
     for (int j = 0; j < count; ++j) {
         cd_job *J = jobs[j];
         wl.descs.push_back(J->h);
+        for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
+        wl.n_cds += J->N * J->m;
         for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
             for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS, w, 0));
         for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
@@ -417,8 +406,9 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 }
 
 Launch job_launch(cd_job *J) {
-    return Launch{J->d_self, 1, J->d_off, J->N, J->d_dom, J->n_dom, J->d_asc, J->n_asc, J->lds, J->stem_lds, J->niche_lds, J->m,
-                  J->stem_parts};
+    return Launch{J->d_self, 1,      J->d_off,     J->N,         J->d_dom,   J->n_dom,   J->d_asc,
+                  J->n_asc,  J->lds, J->stem_lds, J->niche_lds, J->m,       J->stem_parts, J->d_slist,
+                  J->d_sctl, J->N * J->m};
 }
 
 }  // namespace
@@ -534,12 +524,12 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
     const Launch w = job_launch(J);
     for (int g = 0; g < generations; ++g) {
         HIPCHK(hipEventRecord(ev[0], J->stream));
-        HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream));
+        HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream, J->d_slist,
+                                 J->d_sctl, J->N * J->m));
         HIPCHK(hipEventRecord(ev[1], J->stream));
-        HIPCHK(cd::launch_stem(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->m, J->stem_parts, J->stem_lds,
-                               J->stream));
+        HIPCHK(cd::launch_stem_list(J->d_self, J->d_off, J->d_slist, J->d_sctl, J->N * J->m, J->stem_lds, J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
-        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->stream));
+        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->d_sctl, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));
         HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
         HIPCHK(hipEventRecord(ev[4], J->stream));
@@ -630,7 +620,7 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     hipError_t e = hipMemcpy(J->h.obj[0], obj, (size_t)P * 24, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(J->h.st, &st, sizeof(st), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
-    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, J->stream);
+    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, J->stream);
     if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->stream);
     if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream);
     if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream);
@@ -704,8 +694,8 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     B->stem_lds = wl.stem_lds;
     B->niche_lds = wl.niche_lds;
     B->mg = wl.mg;
-    B->stem_cls = wl.stem_cls;
     B->stem_parts = wl.stem_parts;
+    B->n_cds = wl.n_cds;
     B->n_off = (int)wl.off.size();
     B->n_dom = (int)wl.dom.size();
     B->n_asc = (int)wl.asc.size();
@@ -713,6 +703,9 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     if (e == hipSuccess) e = hipMalloc(&B->d_off, sizeof(int2) * wl.off.size());
     if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * wl.dom.size());
     if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * wl.asc.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_slist, sizeof(uint32_t) * wl.n_cds);
+    if (e == hipSuccess) e = hipMalloc(&B->d_sctl, 16);
+    if (e == hipSuccess) e = hipMemset(B->d_sctl, 0, 16);
     if (e == hipSuccess) e = hipMemcpy(B->d_jobs, wl.descs.data(), sizeof(cd::DevJob) * count, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_off, wl.off.data(), sizeof(int2) * wl.off.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_dom, wl.dom.data(), sizeof(int4) * wl.dom.size(), hipMemcpyHostToDevice);
@@ -730,9 +723,10 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
 int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
-        const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off, B->n_off, B->d_dom, B->n_dom, B->d_asc, B->n_asc,
-                       B->lds, B->stem_lds, B->niche_lds, B->mg, B->stem_parts, (int)B->stem_cls.size(),
-                       B->stem_cls.data()};
+        const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off,      B->n_off,   B->d_dom,
+                       B->n_dom,  B->d_asc,            B->n_asc,      B->lds,     B->stem_lds,
+                       B->niche_lds, B->mg,            B->stem_parts, B->d_slist, B->d_sctl,
+                       B->n_cds};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }
@@ -774,6 +768,8 @@ int cd_batch_destroy(cd_batch *B) {
     if (B->d_off) (void)hipFree(B->d_off);
     if (B->d_dom) (void)hipFree(B->d_dom);
     if (B->d_asc) (void)hipFree(B->d_asc);
+    if (B->d_slist) (void)hipFree(B->d_slist);
+    if (B->d_sctl) (void)hipFree(B->d_sctl);
     delete B;
     return CD_OK;
 }

@@ -12,16 +12,6 @@
 #include "codon_nsga3.h"
 
 // One job: tables, pools and selection state in one device allocation.
-namespace cdhost {
-// A run of children of jobs with the same CDS count and k_stem workgroups per
-// CDS: k_stem is launched once per run, with that exact shape.
-struct StemClass {
-    int first, count;  // range of the children list
-    int m, parts;
-    size_t lds;
-};
-}  // namespace cdhost
-
 struct cd_job {
     cd::DevJob h{};
     cd::DevJob *d_self = nullptr;
@@ -32,9 +22,11 @@ struct cd_job {
     int2 *d_off = nullptr;
     int4 *d_dom = nullptr;
     int2 *d_asc = nullptr;
+    uint32_t *d_slist = nullptr;  // changed CDS of a generation (N * m entries) for k_stem_list
+    int32_t *d_sctl = nullptr;    // its count, zero between generations (k_dominance clears it)
     int n_dom = 0, n_asc = 0;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    int stem_parts = 1;  // k_stem workgroups per CDS
+    int stem_parts = 1;  // k_stem workgroups per CDS (initialisation and evaluation)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool inited = false;
@@ -47,11 +39,12 @@ struct cd_batch {
     int2 *d_off = nullptr;
     int4 *d_dom = nullptr;
     int2 *d_asc = nullptr;
-    int n_off = 0, n_dom = 0, n_asc = 0;
+    uint32_t *d_slist = nullptr;  // changed CDS of a generation (n_cds entries)
+    int32_t *d_sctl = nullptr;
+    int n_off = 0, n_dom = 0, n_asc = 0, n_cds = 0;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0;          // largest m of the batch's jobs
     int stem_parts = 0;  // largest k_stem workgroups per CDS of the batch's jobs
-    std::vector<cdhost::StemClass> stem_cls;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -84,11 +77,10 @@ struct Launch {
     int n_asc;
     size_t lds, stem_lds, niche_lds;
     int mg;          // largest m
-    int stem_parts;  // largest k_stem workgroups per CDS
-    // k_stem of the children in runs of one shape (the children list is
-    // ordered by run); none: one launch of the largest shape
-    int n_stem_cls;
-    const StemClass *stem_cls;
+    int stem_parts;  // largest k_stem workgroups per CDS (initialisation and evaluation)
+    uint32_t *stem_list;  // changed CDS of a generation, n_cds entries at most
+    int32_t *stem_ctl;    // its count, zero between generations (k_dominance clears it)
+    int n_cds;            // CDS of all children: sum of N * m
 };
 
 // Host images of the work lists of `jobs` (job k = index k of the list)
@@ -98,8 +90,7 @@ struct WorkLists {
     std::vector<int2> off, asc;
     std::vector<int4> dom;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    int mg = 0, stem_parts = 0;
-    std::vector<StemClass> stem_cls;
+    int mg = 0, stem_parts = 0, n_cds = 0;
 };
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl);

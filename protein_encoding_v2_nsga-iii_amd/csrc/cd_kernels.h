@@ -14,6 +14,7 @@ constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
 constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (one wave each)
 constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup (16 per wave)
 constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
+constexpr int STEM_GRID = 1536;            // most k_stem_list workgroups (6 resident per CU)
 
 size_t design_lds_bytes(int L, int m);
 size_t stem_lds_bytes(int L);
@@ -22,23 +23,32 @@ int stem_parts_of(int L);  // k_stem workgroups per CDS of L codons (1 .. STEM_P
 hipError_t set_design_lds_limit(size_t bytes);
 hipError_t set_stem_lds_limit(size_t bytes);
 
-// items: (job, design or child)
-hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s);
-// parts workgroups per (item, CDS c < mg), parts the largest stem_parts_of of
-// the jobs: stem of each CDS k_design marked
+// items: (job, design or child).  The offspring mode lists every changed CDS
+// in stem_list (entry (item << 4) | c, stem_cap entries at most, the sum of
+// the items' m) and counts them in stem_ctl[0]; the three are required there.
+hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
+                         uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, int stem_cap = 0);
+// initialisation and evaluation: parts workgroups per (item, CDS c < mg),
+// parts the largest stem_parts_of of the jobs: stem of each CDS k_design marked
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,
                        hipStream_t s);
+// offspring generations: stem of each CDS the offspring k_design listed
+// (items as given to it), on min(capacity, STEM_GRID) workgroups, capacity
+// the list's size; launch_dominance with the same stem_ctl clears the count
+hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
+                            const int32_t *stem_ctl, int capacity, size_t lds, hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
 // items: (job, first row, word, 0).  finish: the offspring generation, whose
 // children's stem objective (and stem records) k_dominance completes in place
-// of launch_stem_finish
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, hipStream_t s);
+// of launch_stem_finish; stem_ctl (or null): the changed-CDS count of that
+// generation's k_design, cleared for the next one
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl,
+                            hipStream_t s);
 // one workgroup per job
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);
 // items: (job, first design)
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s);
-// one workgroup per job
 // one workgroup per job; lds = the largest niche_lds_bytes of the jobs
 size_t niche_lds_bytes(int P, int R);
 hipError_t set_niche_lds_limit(size_t bytes);

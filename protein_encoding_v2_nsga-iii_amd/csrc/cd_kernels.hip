@@ -48,11 +48,8 @@ size_t design_lds_bytes(int L, int m) {
 // scan_words(3L) words of (low bits, high bits) of the nucleotide codes
 size_t cds_plane_bytes(int L) { return (size_t)2 * scan_words(3 * L) * sizeof(uint2); }
 
-// q-gram index of k_stem after the planes: QG_CODES chain heads, then the
-// chain links of at most qg_index_max(n) entries
-__host__ __device__ inline int qg_lds_bytes(int n) { return QG_CODES * 4 + align16(2 * qg_index_max(n)); }
-
-// dynamic LDS of k_stem: the bit planes of one CDS and the q-gram index
+// dynamic LDS of k_stem: the bit planes of one CDS, then the q-gram search's
+// index and hit queue
 size_t stem_lds_bytes(int L) { return cds_plane_bytes(L) + qg_lds_bytes(3 * L); }
 
 struct DesignShared {
@@ -141,63 +138,100 @@ __device__ __forceinline__ int stem_seed(const uint2 *sp, int n, int l, int p, i
     return run;
 }
 
-// q-gram search (cd_qgram.h) of the rc positions [b0, b1) for the runs of at
-// least K: the workgroup builds the index of s in LDS (head, next), and each
-// lane looks up every NT-th position within its hit budget.  Sets *best to
-// the workgroup's maximum key and returns whether every lookup finished.
-__device__ bool stem_qgram(int n, const uint2 *sp, const uint2 *rp, int K, int b0, int b1, uint32_t *head,
-                           int32_t *flag, uint64_t *redu, uint64_t &best) {
-    const int tid = threadIdx.x, NT = blockDim.x;
+// q-gram search (cd_qgram.h) of a CDS for the runs of at least K.  The
+// workgroup builds the index of s in LDS (head, then the links); lane t walks
+// the lookups of the chunks t, t + NT, ... of 8 rc positions.  In rounds,
+// each wave walks until its segment of the hit queue could overflow or its
+// lanes are done, appending the hits of each step by ballot, and then the
+// whole workgroup measures the queued hits.  ctl[4..15]: per wave, the hits
+// queued, whether lanes have lookups left, whether one ran out of budget.
+// Sets *best to the workgroup's maximum key and returns whether every lookup
+// finished.
+__device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *rp, int K, uint32_t *head, int32_t *ctl,
+                           uint64_t *redu, uint64_t &best) {
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     const int st = qg_stride(K), NI = qg_index_count(n, st);
     uint16_t *next = (uint16_t *)(head + QG_CODES);
+    uint32_t *queue = (uint32_t *)((uint8_t *)next + qg_link_bytes(n));
     uint4 *h4 = (uint4 *)head;
+    __syncthreads();  // an earlier search's index and queue are no longer read
     for (int i = tid; i < QG_CODES / 4; i += NT) h4[i] = make_uint4(QG_NIL, QG_NIL, QG_NIL, QG_NIL);
-    if (tid == 0) *flag = 0;
     __syncthreads();
     for (int k = tid; k < NI; k += NT) qg_insert(sp, st, k, head, next);
     __syncthreads();
-    int budget = qg_budget((b1 - b0 + NT - 1) / NT);
+    const int xe = n - QG_Q + 1, nch = (xe + 7) >> 3;
+    QgLane ln;
+    qg_lane_start(ln, rp, head, tid, nch, xe);
+    int budget = qg_budget(8 * ((nch + NT - 1) / NT));
+    uint32_t *seg = queue + wid * QG_SEG;
     uint64_t b = 0;
-    bool ok = true;
-    for (int x = b0 + tid; x < b1 && ok; x += NT) ok = qg_lookup(sp, rp, n, st, K, x, head, next, budget, b);
-    if (!ok) *flag = 1;
-    best = block_max_u64(b, redu);  // its barriers order the flag writes before the read
-    return *flag == 0;
+    bool over = false;
+    for (;;) {
+        int cnt = 0;  // the same in every lane of the wave
+        while (cnt <= QG_SEG - 64) {
+            const bool live = !qg_lane_done(ln, nch) && budget > 0;
+            if (!__any(live)) break;
+            int a = 0, x = 0;
+            const bool hit = live && qg_lane_step(ln, rp, head, next, n, st, nch, NT, budget, a, x);
+            const uint64_t bal = __ballot(hit);
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (hit) seg[cnt + rank] = qg_pack(a, x);
+            cnt += __popcll(bal);
+        }
+        const bool left = !qg_lane_done(ln, nch);
+        const bool rem = __any(left), out = __any(left && budget <= 0);
+        if (lane == 0) {
+            ctl[4 + wid] = cnt;
+            ctl[8 + wid] = rem;
+            ctl[12 + wid] = out;
+        }
+        __syncthreads();
+        int total = 0, more = 0;
+        for (int w = 0; w < nw; ++w) {
+            total += ctl[4 + w];
+            more |= ctl[8 + w];
+            over |= ctl[12 + w] != 0;
+        }
+        for (int h = tid; h < total; h += NT) {  // hit h: wave w's entry h minus the counts before it
+            int w = 0, r = h;
+            while (r >= ctl[4 + w]) r -= ctl[4 + w++];
+            const uint32_t v = queue[w * QG_SEG + r];
+            qg_measure(sp, rp, n, (int)(v >> 16), (int)(v & 0xFFFFu), K, b);
+        }
+        __syncthreads();  // the counts and the queue are read before the next round writes them
+        if (!more || over) break;
+    }
+    best = block_max_u64(b, redu);
+    return !over;
 }
 
-__device__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_from, int part, int parts, int32_t *bcast,
+__device__ __forceinline__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_from, int part, int parts, int32_t *bcast,
                              uint64_t *redu, uint32_t *qhead) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const int W = scan_words(n);
     const uint2 *sp = planes, *rp = planes + W;
     if (wid == 0) {
-        const int seed = seed_from ? stem_seed(sp, n, seed_from->l, seed_from->p, seed_from->q, lane) : 0;
-        if (lane == 0) bcast[0] = seed;
+        const int pl = seed_from ? seed_from->l : 0;
+        const int seed = seed_from ? stem_seed(sp, n, pl, seed_from->p, seed_from->q, lane) : 0;
+        if (lane == 0) {
+            bcast[0] = seed;
+            bcast[2] = pl;
+        }
     }
     __syncthreads();
     const int seed = bcast[0];
-    // A changed CDS whose seed is 7 or more is searched by q-grams for the
-    // runs of at least the seed, the parts sharing the lookups: exact.  With
-    // a smaller seed, the runs of at least the parent's stem length less 2
-    // are searched, in full by every part: exact when one is found (a stem
-    // that long exists), else the tiles follow.  A part whose lookups run out
-    // of budget scans the tiles: all of them if the lookups were shared.
-    int tpart = part, tparts = parts;
+    // A changed CDS (one workgroup) is searched by q-grams: for the runs of
+    // at least the seed when that is 7 or more (exact), else for the runs of
+    // at least a guess (exact when one is found), then of at least 7.  The
+    // tiles follow a fruitless guess of 7, a search out of budget, or a CDS
+    // the search does not take.
     if (seed_from) {
-        const int Kq = qg_search_k(n, seed, seed_from->l);
-        if (Kq) {
-            const bool split = seed >= QG_KMIN;
-            const int nb = n - QG_Q + 1;
-            const int b0 = split ? (int)((int64_t)part * nb / parts) : 0;
-            const int b1 = split ? (int)((int64_t)(part + 1) * nb / parts) : nb;
+        for (int K = qg_first_k(n, seed, bcast[2]); K;) {
             uint64_t qbest;
-            if (stem_qgram(n, sp, rp, Kq, b0, b1, qhead, bcast + 1, redu, qbest)) {
-                if (split) return qbest;
-                if ((int)(qbest >> 40) >= Kq) return part == 0 ? qbest : 0;
-            } else if (split) {
-                tpart = 0;
-                tparts = 1;
-            }
+            if (!stem_qgram(n, sp, rp, K, qhead, bcast, redu, qbest)) break;
+            if (seed >= QG_KMIN || (int)(qbest >> 40) >= K) return qbest;
+            K = qg_next_k(K);
         }
     }
     // Every lane's K is a length some stem of this CDS has, so the wave may
@@ -206,7 +240,7 @@ __device__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_fro
     int K = max(seed, TILE_MIN_RUN);
     uint64_t best = 0;
     {
-        const int TT = 2 * tile_prefix(n, tile_blocks(n)), T = tparts * NT, L = tpart * NT + tid;
+        const int TT = 2 * tile_prefix(n, tile_blocks(n)), T = parts * NT, L = part * NT + tid;
         const int t0 = (int)((int64_t)L * TT / T), t1 = (int)((int64_t)(L + 1) * TT / T);
         const int trips = (TT + T - 1) / T;
         TileCursor cur;
@@ -225,8 +259,8 @@ __device__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_fro
     if (seed < TILE_MIN_RUN && (int)(best >> 40) < TILE_MIN_RUN) {
         best = 0;
         K = 1;
-        const int stride = tparts * NT, S0 = MIN_HAIRPIN + 1 + tpart * NT + tid;
-        const int Sw = MIN_HAIRPIN + 1 + tpart * NT + (tid & ~63), last = 2 * n - 3;
+        const int stride = parts * NT, S0 = MIN_HAIRPIN + 1 + part * NT + tid;
+        const int Sw = MIN_HAIRPIN + 1 + part * NT + (tid & ~63), last = 2 * n - 3;
         const int trips = Sw <= last ? (last - Sw) / stride + 1 : 0;
         for (int j = 0; j < trips; ++j) {
             const int S = S0 + j * stride;
@@ -811,7 +845,8 @@ __device__ void build_planes(const uint8_t *g, const uint8_t *nt, int L, uint2 *
 //   OFFSPRING  survivor copy, mutation, evaluation (mutationKernel)
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items) {
+__global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
+                                                uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_cap) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ DesignShared sh;
     const int2 it = items[blockIdx.x];
@@ -954,6 +989,11 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         }
         gstore(J.stem_max + out_slot, lmax);
         atomicAdd(&J.st->cds_scanned, nd);
+        if (MODE == MODE_OFFSPRING && nd) {  // the changed CDS for k_stem_list
+            int e = atomicAdd(stem_ctl, (int)nd);
+            for (int c = 0; c < m; ++c)
+                if (sh.dirty[c] && e < stem_cap) gstore(stem_list + e++, ((uint32_t)blockIdx.x << 4) | (uint32_t)c);
+        }
     }
 }
 
@@ -970,24 +1010,23 @@ __device__ __forceinline__ void item_slot(const DevJob &J, int mode, int y, int 
 }
 
 // ---------------------------------------------------------------------------
-// k_stem: up to STEM_PARTS workgroups per (work item, CDS); blockIdx.x =
-// part * nic + item * mg + c with mg the largest m of the jobs launched and
-// nic = items * mg.  Part-major order keeps the workgroups that do work
-// consecutive, so the round-robin placement of workgroups on the XCDs
-// spreads them over all eight (a CDS-major order with unused parts would
-// leave the working ones on the XCDs of one residue class).  CDS not marked
-// by k_design (pad 0) are skipped.  The nucleotide
-// codes are rebuilt from the genome row, the parent's stem seeds the scan
-// (children only), and each part folds its best key into the record with an
-// atomic maximum; k_stem_finish decodes it.
+// k_stem (initialisation and evaluation): up to STEM_PARTS workgroups per
+// (work item, CDS); blockIdx.x = part * nic + item * mg + c with mg the
+// largest m of the jobs launched and nic = items * mg.  Part-major order keeps
+// the workgroups that do work consecutive, so the round-robin placement of
+// workgroups on the XCDs spreads them over all eight (a CDS-major order with
+// unused parts would leave the working ones on the XCDs of one residue
+// class).  CDS not marked by k_design (pad 0) are skipped.  Each part folds
+// its best key into the record with an atomic maximum; k_stem_finish decodes
+// it.
 // ---------------------------------------------------------------------------
-// Six waves per SIMD (at most 80 VGPRs) keep every workgroup of a generation
-// of ~1300 changed CDS resident at once on 256 CUs.
+// Six waves per SIMD (at most 80 VGPRs) keep ~1500 workgroups resident at
+// once on 256 CUs.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stem(const DevJob *__restrict__ jobs, const int2 *__restrict__ items, int mode,
                                               int mg, int nic) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];
-    __shared__ int32_t bcast[2];
+    __shared__ int32_t bcast[16];
     const int part = (int)(blockIdx.x / (unsigned)nic), ic = (int)(blockIdx.x % (unsigned)nic);
     const int2 it = items[ic / mg];
     const int c = ic % mg;
@@ -1003,11 +1042,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
     for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
     __syncthreads();
-    const uint64_t key = stem_cds(n, (const uint2 *)planes, mode == MODE_OFFSPRING ? cs : nullptr, part, parts, bcast,
-                                  redu, (uint32_t *)(planes + nwords));
-    if (threadIdx.x == 0 && key) {
-        atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
-        atomicMax(&J.stem_max[slot], (int)(key >> 40));
+    const uint64_t key = stem_cds(n, (const uint2 *)planes, nullptr, part, parts, bcast, redu, nullptr);
+    if (threadIdx.x == 0 && key) atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
+}
+
+// ---------------------------------------------------------------------------
+// k_stem_list (offspring generations): the changed CDS k_design listed,
+// entry (item << 4) | c, ctl[0] of them; gridDim.x workgroups take the
+// entries round-robin, one workgroup per CDS.  The parent's stem seeds the
+// q-gram search (stem_cds).  Each entry's key goes into the CDS record and
+// its length into the design's longest stem (stem_max) with atomic maxima.
+// k_dominance clears ctl[0] for the next generation.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stem_list(
+    const DevJob *__restrict__ jobs, const int2 *__restrict__ items, const uint32_t *__restrict__ list,
+    const int32_t *__restrict__ ctl, int cap) {
+    extern __shared__ __align__(16) uint8_t dyn[];
+    __shared__ uint64_t redu[16];
+    __shared__ int32_t bcast[16];
+    const int count = min(gload(ctl), cap);
+    for (int e = (int)blockIdx.x; e < count; e += (int)gridDim.x) {
+        const uint32_t ent = gload(list + e);
+        const int2 it = items[ent >> 4];
+        const int c = (int)(ent & 15u);
+        const DevJob &J = jobs[it.x];
+        const int L = J.L, m = J.m, n = 3 * L;
+        int buf, slot;
+        item_slot(J, MODE_OFFSPRING, it.y, buf, slot);
+        CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
+        uint64_t *planes = (uint64_t *)dyn;  // the CDS's planes as k_design left them
+        const int nwords = 2 * scan_words(n);
+        const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
+        __syncthreads();  // the previous entry's LDS is no longer read
+        for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
+        __syncthreads();
+        const uint64_t key = stem_cds(n, (const uint2 *)planes, cs, 0, 1, bcast, redu, (uint32_t *)(planes + nwords));
+        if (threadIdx.x == 0 && key) {
+            atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
+            atomicMax(&J.stem_max[slot], (int)(key >> 40));
+        }
     }
 }
 
@@ -1080,9 +1153,10 @@ __device__ __forceinline__ bool dominates(const float *a, const float *b) {
 // broadcasts), and the wave stores its 16 words and adds their counts at
 // once.
 __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items,
-                                                   bool finish) {
+                                                   bool finish, int32_t *stem_ctl) {
     constexpr int WR = DOM_ROWS / 4;  // rows per wave
     __shared__ float rows[DOM_ROWS * NOBJ];
+    if (stem_ctl && blockIdx.x == 0 && threadIdx.x == 0) gstore(stem_ctl, 0);  // k_stem_list has read the count
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
     const int P = J.P, N = J.N, word = it.z, buf = pool_buffer(J);
@@ -1758,12 +1832,16 @@ __global__ void k_zero_np(const DevJob *__restrict__ jobs) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s) {
+hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
+                         uint32_t *stem_list, int32_t *stem_ctl, int stem_cap) {
     if (n <= 0) return hipSuccess;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items); break;
-    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items); break;
-    default: hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items); break;
+    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0); break;
+    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0); break;
+    default:
+        if (!stem_list || !stem_ctl || stem_cap <= 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, stem_cap);
+        break;
     }
     return hipGetLastError();
 }
@@ -1776,14 +1854,23 @@ hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, i
     hipLaunchKernelGGL(k_stem, dim3(n * mg * parts), dim3(256), lds, s, jobs, items, mode, mg, n * mg);
     return hipGetLastError();
 }
+hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
+                            const int32_t *stem_ctl, int capacity, size_t lds, hipStream_t s) {
+    if (capacity <= 0) return hipSuccess;
+    if (!stem_list || !stem_ctl) return hipErrorInvalidValue;
+    const int grid = capacity < STEM_GRID ? capacity : STEM_GRID;
+    hipLaunchKernelGGL(k_stem_list, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity);
+    return hipGetLastError();
+}
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_stem_finish, dim3((n + 255) / 256), dim3(256), 0, s, jobs, items, n, mode);
     return hipGetLastError();
 }
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, hipStream_t s) {
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl,
+                            hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items, finish);
+    hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl);
     return hipGetLastError();
 }
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {
@@ -1821,7 +1908,9 @@ hipError_t set_design_lds_limit(size_t bytes) {
     return hipFuncSetAttribute((const void *)k_design<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 hipError_t set_stem_lds_limit(size_t bytes) {
-    return hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    hipError_t e = hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_stem_list, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace cd

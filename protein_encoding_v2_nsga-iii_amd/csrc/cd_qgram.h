@@ -33,7 +33,6 @@ constexpr int QG_Q = 6;                       // bases per q-gram
 constexpr int QG_CODES = 1 << (2 * QG_Q);     // chains in the index
 constexpr int QG_KMIN = QG_Q + 1;             // smallest K searched (stride >= 2)
 constexpr uint32_t QG_NIL = 0xFFFFu;          // end of a chain
-constexpr int QG_MAX_INDEX = 0xFFFF;          // indexed positions must stay below QG_NIL
 
 // Stride of the sampled s positions for a search of runs >= K.
 __host__ __device__ inline int qg_stride(int K) { return K - QG_Q + 1; }
@@ -68,15 +67,24 @@ __host__ __device__ inline void qg_insert(const uint2 *sp, int st, int k, uint32
 }
 
 // The maximal run through the match (a, b) (a valid pair), clipped to the
-// valid pairs of its diagonal, offered to best if at least K long.
+// valid pairs of its diagonal, offered to best if at least K long.  One
+// 32-row match word from 8 rows below the hit holds the run unless the run
+// reaches an end of the word; the run counters then follow it further.
 __host__ __device__ inline void qg_measure(const uint2 *sp, const uint2 *rp, int n, int a, int b, int K,
                                            uint64_t &best) {
     const int h = b >= a ? 0 : 1;  // half of cd_tile.h: A reads rc along the diagonal, B reads s
     const uint2 *X = h ? sp : rp, *Y = h ? rp : sp;
     const int d = h ? a - b : b - a, u = h ? b : a;
     const int umax = (n - 6 - d) >> 1;
-    const int down = tile_run_down(X, Y, d, u - 1);
-    const int l = down + tile_run_up(X, Y, d, u, umax);
+    const int r0 = u >= 8 ? u - 8 : 0, k = u - r0;  // the hit is bit k of rows r0 .. r0+31
+    uint32_t w = tile_match(X, Y, d, r0);
+    if (umax - r0 < 31) w &= (2u << (umax - r0)) - 1u;
+    const uint32_t ends = ~(w >> k);  // lowest set bit: the first row above the run
+    int up = ends ? __builtin_ctz(ends) : 32;
+    int down = k ? __builtin_clz(~(w << (32 - k))) : 0;  // matches from row u-1 down, at most k
+    if (k + up == 32) up += tile_run_up(X, Y, d, r0 + 32, umax);
+    if (down == k && r0 > 0) down += tile_run_down(X, Y, d, r0 - 1);
+    const int l = down + up;
     if (l < K) return;
     const int re = u - down + l - 1;  // row of the innermost pair
     const int i = h ? re + d : re, j = h ? n - 1 - re : n - 1 - re - d;
@@ -84,35 +92,104 @@ __host__ __device__ inline void qg_measure(const uint2 *sp, const uint2 *rp, int
     if (key > best) best = key;
 }
 
-// Look up rc position b (b + QG_Q <= n): measure every hit whose q-gram is
-// valid.  Each chain entry visited costs one unit of budget; returns false,
-// leaving the lookup unfinished, when the budget runs out.
-__host__ __device__ inline bool qg_lookup(const uint2 *sp, const uint2 *rp, int n, int st, int K, int b,
-                                          const uint32_t *head, const uint16_t *next, int &budget,
-                                          uint64_t &best) {
-    uint32_t t = head[qg_code(rp, b)];
-    while (t != QG_NIL) {
-        if (--budget < 0) return false;
-        const int a = st * (int)t;
-        if (a + b + 2 * (QG_Q - 1) <= n - 6) qg_measure(sp, rp, n, a, b, K, best);
-        t = next[t];
-    }
-    return true;
+// One lane's walk over the lookups of its chunks of 8 rc positions (chunk ch
+// covers positions 8ch .. 8ch+7 below xe = n - QG_Q + 1; the lane takes the
+// chunks ch0, ch0 + stride, ...).  Each step reads one chain entry (or moves
+// to the lane's next chunk) and yields the hit it finds when its q-gram is
+// valid, so that a wave's steps stay cheap and uniform and the hits can be
+// measured apart from the walk.
+struct QgLane {
+    uint32_t lo, hi;  // plane bits of rc from the chunk's first position
+    uint32_t mask;    // positions of the chunk whose chains are still to start
+    uint32_t e;       // current chain entry, QG_NIL between chains
+    int j;            // position of the current chain within the chunk
+    int ch;           // current chunk; the walk is over once it reaches nch
+};
+
+// Code of position j of the lane's chunk.
+__host__ __device__ inline uint32_t qg_lane_code(const QgLane &s, int j) {
+    constexpr uint32_t M = (1u << QG_Q) - 1u;
+    return ((s.lo >> j) & M) | (((s.hi >> j) & M) << QG_Q);
 }
 
-// Hit budget of one lane that looks up `lookups` positions.
+// Plane bits of chunk ch (one pair of plane words) and its positions with a
+// chain (the eight heads are read together).
+__host__ __device__ inline void qg_lane_load(QgLane &s, const uint2 *rp, const uint32_t *head, int xe) {
+    const int x0 = s.ch << 3, w = x0 >> 5;
+    const uint2 p0 = rp[w], p1 = rp[w + 1];
+    s.lo = scan_funnel(p1.x, p0.x, x0 & 31);
+    s.hi = scan_funnel(p1.y, p0.y, x0 & 31);
+    uint32_t mask = 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mask |= (uint32_t)(x0 + j < xe && head[qg_lane_code(s, j)] != QG_NIL) << j;
+    s.mask = mask;
+}
+
+__host__ __device__ inline void qg_lane_start(QgLane &s, const uint2 *rp, const uint32_t *head, int ch0, int nch,
+                                              int xe) {
+    s.ch = ch0;
+    s.e = QG_NIL;
+    s.j = 0;
+    s.mask = 0u;
+    s.lo = s.hi = 0u;
+    if (ch0 < nch) qg_lane_load(s, rp, head, xe);
+}
+
+__host__ __device__ inline bool qg_lane_done(const QgLane &s, int nch) { return s.ch >= nch; }
+
+// One step of a lane that is not done: returns true with the hit (a, b)
+// when the step read a chain entry whose q-gram is valid.  Each chain entry
+// read costs one unit of budget.
+__host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const uint32_t *head, const uint16_t *next,
+                                             int n, int st, int nch, int stride, int &budget, int &a, int &b) {
+    if (s.e == QG_NIL) {
+        if (!s.mask) {  // the chunk is done: the lane's next chunk
+            s.ch += stride;
+            if (s.ch < nch) qg_lane_load(s, rp, head, n - QG_Q + 1);
+            return false;
+        }
+        s.j = __builtin_ctz(s.mask);
+        s.mask &= s.mask - 1u;
+        s.e = head[qg_lane_code(s, s.j)];  // not QG_NIL: the mask has only positions with a chain
+    }
+    --budget;
+    const uint32_t e = s.e;
+    s.e = next[e];
+    a = st * (int)e;
+    b = (s.ch << 3) + s.j;
+    return a + b + 2 * (QG_Q - 1) <= n - 6;
+}
+
+// A hit packed for the queue of hits to measure (both below 2^16).
+__host__ __device__ inline uint32_t qg_pack(int a, int b) { return ((uint32_t)a << 16) | (uint32_t)b; }
+
+// Chain entries one lane may read when it looks up `lookups` positions
+// (about 0.1 to 0.3 per lookup are expected without long repeats).
 __host__ __device__ inline int qg_budget(int lookups) { return 4 * lookups + 64; }
 
-// K of the search for a changed CDS of n bases, 0 for none: the seed (the
-// length of a stem the CDS is known to have) when it is at least QG_KMIN, so
-// that the search is exact; else the guess parent_l - 2 (two below the
-// parent's longest stem of this CDS) when that is at least QG_KMIN, exact
-// only when the search finds a stem that long.
-__host__ __device__ inline int qg_search_k(int n, int seed, int parent_l) {
-    if (qg_index_max(n) >= QG_MAX_INDEX) return 0;
+// Hits per wave measured together: the queue of a workgroup of QG_WAVES
+// waves holds QG_WAVES * QG_SEG packed hits.
+constexpr int QG_SEG = 512;
+constexpr int QG_WAVES = 4;
+
+// LDS of a search on n bases: the chain heads, the links of at most
+// qg_index_max(n) entries, the hit queue.
+__host__ __device__ inline int qg_link_bytes(int n) { return (2 * qg_index_max(n) + 15) & ~15; }
+__host__ __device__ inline int qg_lds_bytes(int n) { return QG_CODES * 4 + qg_link_bytes(n) + QG_WAVES * QG_SEG * 4; }
+
+// First K of the search of a changed CDS of n bases, 0 for none: the seed
+// (the length of a stem the CDS is known to have) when it is at least
+// QG_KMIN, so that the search is exact; else the guess max(QG_KMIN,
+// parent_l - 2), two below the parent's longest stem of this CDS, exact when
+// the search finds a stem that long.
+__host__ __device__ inline int qg_first_k(int n, int seed, int parent_l) {
+    if (n > 0xFFFF || (n - 4) / 2 < QG_KMIN) return 0;  // hits are packed in 16-bit halves
     if (seed >= QG_KMIN) return seed;
-    const int g = parent_l - 2;
-    return g >= QG_KMIN ? g : 0;
+    return parent_l - 2 > QG_KMIN ? parent_l - 2 : QG_KMIN;
 }
+
+// K of the next search after one at K found no stem of K (a guess): QG_KMIN
+// after a larger guess, else 0 (the tiles follow).
+__host__ __device__ inline int qg_next_k(int K) { return K > QG_KMIN ? QG_KMIN : 0; }
 
 }  // namespace cd

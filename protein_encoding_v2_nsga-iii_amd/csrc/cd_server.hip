@@ -83,7 +83,9 @@ struct cd_server {
     cd::DevJob *d_jobs = nullptr;
     int2 *d_off = nullptr, *d_asc = nullptr;
     int4 *d_dom = nullptr;
-    size_t cap_jobs = 0, cap_off = 0, cap_dom = 0, cap_asc = 0;
+    uint32_t *d_slist = nullptr;  // changed CDS of a tick
+    int32_t *d_sctl = nullptr;    // its count, zero between ticks
+    size_t cap_jobs = 0, cap_off = 0, cap_dom = 0, cap_asc = 0, cap_slist = 0;
     std::string fatal;  // a device error that stopped the worker
 };
 
@@ -125,6 +127,7 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res, cdhost::WorkLi
     if (e == hipSuccess) e = ensure(S->d_off, S->cap_off, wl.off.size());
     if (e == hipSuccess) e = ensure(S->d_dom, S->cap_dom, wl.dom.size());
     if (e == hipSuccess) e = ensure(S->d_asc, S->cap_asc, wl.asc.size());
+    if (e == hipSuccess) e = ensure(S->d_slist, S->cap_slist, (size_t)wl.n_cds);
     if (e == hipSuccess)
         e = hipMemcpyAsync(S->d_jobs, wl.descs.data(), sizeof(cd::DevJob) * wl.descs.size(), hipMemcpyHostToDevice,
                            S->stream);
@@ -137,7 +140,7 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res, cdhost::WorkLi
     if (e == hipSuccess) e = cdhost::set_lds_limits(wl);
     L = cdhost::Launch{S->d_jobs,   (int)wl.descs.size(), S->d_off,    (int)wl.off.size(), S->d_dom,
                        (int)wl.dom.size(), S->d_asc,     (int)wl.asc.size(), wl.lds,       wl.stem_lds,
-                       wl.niche_lds, wl.mg, wl.stem_parts, (int)wl.stem_cls.size(), wl.stem_cls.data()};
+                       wl.niche_lds, wl.mg, wl.stem_parts, S->d_slist, S->d_sctl, wl.n_cds};
     return e;
 }
 
@@ -249,7 +252,11 @@ int cd_server_create(int64_t hbm_budget_bytes, int32_t max_jobs, cd_server **out
     S->budget = hbm_budget_bytes;
     S->max_jobs = max_jobs;
     hipError_t e = hipStreamCreate(&S->stream);
+    if (e == hipSuccess) e = hipMalloc(&S->d_sctl, 16);
+    if (e == hipSuccess) e = hipMemset(S->d_sctl, 0, 16);
     if (e != hipSuccess) {
+        if (S->stream) (void)hipStreamDestroy(S->stream);
+        if (S->d_sctl) (void)hipFree(S->d_sctl);
         delete S;
         return cdhost::fail(CD_E_DEVICE, std::string("server stream: ") + hipGetErrorString(e));
     }
@@ -356,6 +363,8 @@ int cd_server_destroy(cd_server *S) {
     if (S->d_off) (void)hipFree(S->d_off);
     if (S->d_dom) (void)hipFree(S->d_dom);
     if (S->d_asc) (void)hipFree(S->d_asc);
+    if (S->d_slist) (void)hipFree(S->d_slist);
+    if (S->d_sctl) (void)hipFree(S->d_sctl);
     delete S;
     return CD_OK;
 }

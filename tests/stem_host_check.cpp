@@ -7,11 +7,12 @@
 // lane L = p*NT + t takes the flat tiles [L*TT/T, (L+1)*TT/T), and a part
 // with no seed of 4 and no run of 4 in its tiles falls back to the "diag"
 // lanes of that part with K = 1.  In both, after every trip the 64 lanes of a
-// wave share the largest K, as the kernel's shuffle does.  Mode "qgram": the
-// q-gram search of a changed CDS as stem_cds runs it (K from the seed or the
-// parent's length, lookups shared by the parts or searched in full, tiles
-// after an unfinished or fruitless search); an optional fifth argument
-// replaces every lane's hit budget.  Prints "l p q" of the best stem per
+// wave share the largest K, as the kernel's shuffle does.  Mode "qgram": a
+// changed CDS as the offspring stem kernel takes it (stem_cds on one
+// workgroup): q-gram searches with K from the seed, or guessed from the
+// parent's length and then 7, and the tiles after a fruitless guess of 7 or
+// a search out of budget; an optional fifth argument replaces every lane's
+// hit budget, and stderr counts the searches by outcome.  Prints "l p q" of the best stem per
 // sequence.  Planes and index arrays are allocated at exactly their
 // documented size so that the address sanitizer catches any access outside
 // them.
@@ -33,7 +34,7 @@ int main(int argc, char **argv) {
     const bool TILED = MODE == "tile" || MODE == "qgram", QGRAM = MODE == "qgram";
     const int BUDGET = argc > 4 ? std::stoi(argv[4]) : -1;
     const int MIN_HAIRPIN = 4;
-    long n_qgram = 0, n_miss = 0, n_overflow = 0;  // q-gram parts: finished, guessed K not found, out of budget
+    long n_qgram = 0, n_miss = 0, n_overflow = 0;  // q-gram searches: exact, guess not found, out of budget
     std::string line;
     while (std::getline(std::cin, line)) {
         std::istringstream in(line);
@@ -117,9 +118,10 @@ int main(int argc, char **argv) {
                 }
                 return bp;
             };
-            // the q-gram search of rc positions [b0, b1) by the part's NT lanes;
-            // returns whether every lookup finished, the maximum key in *qbest
-            auto qgram_part = [&](int K, int b0, int b1, uint64_t *qbest) -> bool {
+            // the q-gram search at K by NT lanes (chunks of 8 rc positions, every
+            // NT-th to a lane); returns whether every lookup finished, the
+            // maximum key in *qbest
+            auto qgram_search = [&](int K, uint64_t *qbest) -> bool {
                 const int st = cd::qg_stride(K), NI = cd::qg_index_count(n, st);
                 if (NI > cd::qg_index_max(n)) {
                     std::printf("index larger than its LDS share n=%d K=%d\n", n, K);
@@ -128,48 +130,50 @@ int main(int argc, char **argv) {
                 std::vector<uint32_t> head(cd::QG_CODES, cd::QG_NIL);
                 std::vector<uint16_t> next(NI);
                 for (int k = 0; k < NI; ++k) cd::qg_insert(sp.data(), st, k, head.data(), next.data());
+                const int xe = n - cd::QG_Q + 1, nch = (xe + 7) >> 3;
                 bool ok = true;
                 uint64_t bq = 0;
-                for (int t = 0; t < NT; ++t) {
-                    int budget = BUDGET >= 0 ? BUDGET : cd::qg_budget((b1 - b0 + NT - 1) / NT);
-                    bool lane_ok = true;
-                    for (int x = b0 + t; x < b1 && lane_ok; x += NT)
-                        lane_ok = cd::qg_lookup(sp.data(), rp.data(), n, st, K, x, head.data(), next.data(), budget, bq);
-                    ok = ok && lane_ok;
+                for (int t = 0; t < NT; ++t) {  // lane t's walk; its hits measured as they come
+                    int budget = BUDGET >= 0 ? BUDGET : cd::qg_budget(8 * ((nch + NT - 1) / NT));
+                    cd::QgLane ln;
+                    cd::qg_lane_start(ln, rp.data(), head.data(), t, nch, xe);
+                    while (!cd::qg_lane_done(ln, nch) && budget > 0) {
+                        int a = 0, x = 0;
+                        if (cd::qg_lane_step(ln, rp.data(), head.data(), next.data(), n, st, nch, NT, budget, a, x)) {
+                            const uint32_t v = cd::qg_pack(a, x);
+                            cd::qg_measure(sp.data(), rp.data(), n, (int)(v >> 16), (int)(v & 0xFFFFu), K, bq);
+                        }
+                    }
+                    ok = ok && cd::qg_lane_done(ln, nch);
                 }
                 *qbest = bq;
                 return ok;
             };
-            for (int part = 0; part < PARTS; ++part) {
-                int tpart = part, tparts = PARTS;
-                uint64_t bp = 0;
+            if (QGRAM) {
+                // a changed CDS as the offspring stem kernel takes it: one workgroup,
+                // q-gram searches from the seed or the guesses, else its tiles
                 bool done = false;
-                const int Kq = QGRAM ? cd::qg_search_k(n, seed, parent_l) : 0;
-                if (Kq) {
-                    const bool split = seed >= cd::QG_KMIN;
-                    const int nb = n - cd::QG_Q + 1;
-                    const int b0 = split ? (int)((int64_t)part * nb / PARTS) : 0;
-                    const int b1 = split ? (int)((int64_t)(part + 1) * nb / PARTS) : nb;
+                for (int K = cd::qg_first_k(n, seed, parent_l); K && !done;) {
                     uint64_t qb = 0;
-                    if (qgram_part(Kq, b0, b1, &qb)) {
-                        if (split) {
-                            bp = qb;
-                            done = true;
-                        } else if ((int)(qb >> 40) >= Kq) {
-                            bp = part == 0 ? qb : 0;
-                            done = true;
-                        }
-                        ++(done ? n_qgram : n_miss);
-                    } else {
+                    if (!qgram_search(K, &qb)) {
                         ++n_overflow;
-                        if (split) {
-                            tpart = 0;
-                            tparts = 1;
-                        }
+                        break;
+                    }
+                    if (seed >= cd::QG_KMIN || (int)(qb >> 40) >= K) {
+                        best = qb;
+                        done = true;
+                        ++n_qgram;
+                    } else {
+                        ++n_miss;
+                        K = cd::qg_next_k(K);
                     }
                 }
-                if (!done) bp = tile_part(tpart, tparts);
-                best = bp > best ? bp : best;
+                if (!done) best = tile_part(0, 1);
+            } else {
+                for (int part = 0; part < PARTS; ++part) {
+                    const uint64_t bp = tile_part(part, PARTS);
+                    best = bp > best ? bp : best;
+                }
             }
         } else {
         const int stride = PARTS * NT, last = 2 * n - 3;

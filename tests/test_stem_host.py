@@ -96,38 +96,38 @@ def test_stem_scan_edge_lengths(orc, harness, mode):
         assert g == w, len(s)
 
 
-@pytest.mark.parametrize("lanes,parts", [(256, 1), (256, 4), (64, 2), (7, 3), (1, 1)])
-def test_qgram_search_matches_oracle(orc, harness, lanes, parts):
-    # the seeded search of a changed CDS: seeds that are lower bounds (proven
-    # K, lookups shared by the parts) and parent lengths that are right, too
-    # small or too large (guessed K: exact when found, tiles when not)
-    seqs = [s for s in sequences(lanes * 3 + parts, 160, 2400) if len(s) > 0]
+@pytest.mark.parametrize("lanes", [256, 64, 7, 1])
+def test_qgram_search_matches_oracle(orc, harness, lanes):
+    # a changed CDS as the offspring stem kernel takes it: seeds that are lower
+    # bounds (exact search when 7 or more) and parent lengths that are right,
+    # too small or too large (guessed K, then 7: exact when found, tiles when
+    # not)
+    seqs = [s for s in sequences(lanes * 3 + 1, 160, 2400) if len(s) > 0]
     rng = np.random.default_rng(lanes + 11)
     seqs += [np.asarray(rng.integers(0, 4, n), np.uint8) for n in (1503, 3000, 4503)]
     want = [orc.sl(s, 0) for s in seqs]
     cases = (([w[0] for w in want], [w[0] for w in want]),
              ([int(rng.integers(0, w[0] + 1)) for w in want], [w[0] + int(rng.integers(-3, 6)) for w in want]),
              ([min(w[0], int(rng.integers(0, 7))) for w in want], [w[0] + 2 for w in want]),
-             ([min(w[0], int(rng.integers(0, 7))) for w in want], [w[0] + 3 for w in want]))
+             ([min(w[0], int(rng.integers(0, 7))) for w in want], [w[0] + 3 for w in want]),
+             ([min(w[0], int(rng.integers(0, 7))) for w in want], [0] * len(want)))
     paths = []
     for seeds, parent in cases:
-        got, path = run(harness, seqs, seeds, lanes, parts, "qgram", parent)
+        got, path = run(harness, seqs, seeds, lanes, 1, "qgram", parent)
         for s, g, w, k, pl in zip(seqs, got, want, seeds, parent):
             assert g == w, (len(s), k, pl)
         paths.append(path)
-    assert all(p["qgram"] > 0 for p in paths[:3]), paths  # these cases found stems by q-grams
-    assert paths[3]["qgram"] == 0 and paths[3]["miss"] > 0, paths  # a guess above the longest stem misses
+    assert all(p["qgram"] > 0 for p in paths), paths  # every case found stems by q-grams
+    assert paths[3]["miss"] > 0, paths  # a guess above the longest stem misses, then 7 is searched
 
 
 @pytest.mark.parametrize("budget", [0, 1, 3])
 def test_qgram_search_budget(orc, harness, budget):
-    # lookups that run out of hit budget: shared lookups leave the part to scan
-    # every tile, a full search leaves every part to its tile share
+    # searches that run out of hit budget leave the CDS to the tiles
     seqs = [s for s in sequences(budget + 40, 120, 1600) if len(s) > 0]
     want = [orc.sl(s, 0) for s in seqs]
-    for parts in (1, 3):
-        for seeds, parent in (([w[0] for w in want], [0] * len(want)), ([0] * len(want), [w[0] for w in want])):
-            got, path = run(harness, seqs, seeds, 64, parts, "qgram", parent, budget)
-            for s, g, w in zip(seqs, got, want):
-                assert g == w, (len(s), parts)
-            assert path["overflow"] > 0, path
+    for seeds, parent in (([w[0] for w in want], [0] * len(want)), ([0] * len(want), [w[0] for w in want])):
+        got, path = run(harness, seqs, seeds, 64, 1, "qgram", parent, budget)
+        for s, g, w in zip(seqs, got, want):
+            assert g == w, len(s)
+        assert path["overflow"] > 0, path
