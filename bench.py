@@ -9,10 +9,12 @@ resident in HBM before the timed region; timing uses HIP events on the job's
 stream around K graph-launched generations, after W warmup generations.
 
 Prints one JSON line with designs/s (N*K / time), per-generation latency
-percentiles, the roofline of the dominant kernel (k_stem, the stem scan of the
-changed CDS: VALU-bound, stem cells per second against the int32 VALU peak at
-32 cells per lane-op) and a CPU baseline (the oracle on the host cores,
-bounded sample).
+percentiles, the roofline of the dominant kernel of the live per-kernel
+profile (k_design's offspring launch: HBM-bound, the algorithmic bytes of a
+child, parent read plus survivor and child written, against 8 TB/s; k_stem,
+when it dominates: VALU-bound, stem cells against 32 cells per int32
+lane-op), the stem scan's cell rate beside it, and a CPU baseline (the oracle
+on the host cores, bounded sample).
 """
 import argparse
 import importlib
@@ -33,11 +35,13 @@ PKG = "protein_encoding_v2_nsga-iii_amd"
 # the scan is measured against.
 VALU_INT32_OPS = 256 * 64 * 2.4e9
 PEAK_CELLS = 32 * VALU_INT32_OPS
+HBM_BYTES_S = 8.0e12
 KERNELS = ("offspring", "stem", "dominance", "fronts", "associate", "niche")
-# HBM bytes per k_stem launch at the default workload: FETCH_SIZE x 2.000 +
+# HBM bytes per launch at the default workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (scaled by a 256 MiB device copy in the same run) from
 # the rocprofv3 --pmc runs summarised in profiles/pmc_single_job_L501_m4_N1024.md
-K_STEM_TRAFFIC_BYTES = int((1009.5 * 2.000 + 84.6) * 1024)
+TRAFFIC_BYTES = {"offspring": int((1961.6 * 2.000 + 5816.7) * 1024),
+                 "stem": int((1009.5 * 2.000 + 84.6) * 1024)}
 
 
 def log(msg):
@@ -47,6 +51,13 @@ def log(msg):
 def stem_cells(L):
     n = 3 * L
     return (n - 5) * (n - 4) // 2
+
+
+def child_bytes(L, m):
+    """Algorithmic HBM bytes of one child in the offspring launch: the parent's
+    genome read, the survivor copy and the child written (one byte per codon
+    each), and ~96 bytes of objective and stem records."""
+    return 3 * m * L + 96
 
 
 def workload(args):
@@ -122,8 +133,23 @@ def main():
     prof = [float(x) for x in job.profile(20)]
     cds_scanned = (job.cds_scanned() - c0) / 20.0  # CDS scanned per stem launch
     cells = cds_scanned * stem_cells(L)
-    achieved = cells / (prof[1] / 1e3)
+    cell_rate = cells / (prof[1] / 1e3)
     log(f"per-kernel profile: {prof}")
+    dominant = KERNELS[int(np.argmax(prof))]
+    stem_line = {"bound": "valu", "achieved": round(cell_rate / 1e12, 4), "peak": round(PEAK_CELLS / 1e12, 2),
+                 "unit": "T stem cells/s (peak: 32 cells per int32 lane-op)", "frac": round(cell_rate / PEAK_CELLS, 5),
+                 "traffic": TRAFFIC_BYTES["stem"] if default_workload else None, "kernel": "k_stem",
+                 "cells_per_launch": int(cells)}
+    if dominant == "stem":
+        roofline = stem_line
+    else:  # the offspring launch (k_design), the dominant kernel at the default workload
+        nbytes = args.N * child_bytes(L, args.m)
+        rate = nbytes / (prof[0] / 1e3)
+        roofline = {"bound": "hbm", "achieved": round(rate / 1e9, 2), "peak": HBM_BYTES_S / 1e9, "unit": "GB/s",
+                    "frac": round(rate / HBM_BYTES_S, 5),
+                    "traffic": TRAFFIC_BYTES["offspring"] if default_workload else None,
+                    "kernel": "k_design<offspring>", "bytes_per_launch": int(nbytes),
+                    "dominant_in_profile": dominant}
     out = {
         "metric": "designs/s + p50/p99 latency, batched vs per-instance on MI355X at equal HV (±1%)",
         "value": round(designs_s, 1),
@@ -143,11 +169,8 @@ def main():
         "latency_ms_per_generation": {"p50": round(float(np.percentile(lat, 50)), 4),
                                       "p99": round(float(np.percentile(lat, 99)), 4)},
         "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in prof])),
-        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4), "peak": round(PEAK_CELLS / 1e12, 2),
-                     "unit": "T stem cells/s (peak: 32 cells per int32 lane-op)",
-                     "frac": round(achieved / PEAK_CELLS, 5),
-                     "traffic": K_STEM_TRAFFIC_BYTES if default_workload else None, "kernel": "k_stem",
-                     "cells_per_launch": int(cells)},
+        "roofline": roofline,
+        "stem_scan": stem_line,
     }
     if args.batched_gens > 0:
         out["batched"] = bench_batched(cdp, args)

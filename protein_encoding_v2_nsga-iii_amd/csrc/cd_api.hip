@@ -331,7 +331,12 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
 // the stem of every changed CDS, objective 5.
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
     if (mode == cd::MODE_OFFSPRING) {  // items is the children list; k_dominance finishes the stem objective
-        hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s, w.stem_list, w.stem_ctl, w.n_cds);
+        hipError_t e = hipSuccess;
+        const int ns = w.n_small > 0 && w.n_small < n_items ? w.n_small : 0;
+        if (ns) e = cd::launch_design(mode, w.jobs, items, ns, w.lds_small, s, w.stem_list, w.stem_ctl, w.n_cds, 0);
+        if (e == hipSuccess)
+            e = cd::launch_design(mode, w.jobs, items + ns, n_items - ns, w.lds, s, w.stem_list, w.stem_ctl, w.n_cds,
+                                  ns);
         if (e == hipSuccess) e = cd::launch_stem_list(w.jobs, items, w.stem_list, w.stem_ctl, w.n_cds, w.stem_lds, s);
         return e;
     }
@@ -362,10 +367,20 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
     wl = WorkLists();
+    // children of the jobs whose k_design fits six workgroups per CU first
+    for (int pass = 0; pass < 2; ++pass)
+        for (int j = 0; j < count; ++j) {
+            cd_job *J = jobs[j];
+            if ((J->lds <= cd::DESIGN_LDS_SIX) != (pass == 0)) continue;
+            for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
+            if (pass == 0) {
+                wl.n_small += J->N;
+                wl.lds_small = std::max(wl.lds_small, J->lds);
+            }
+        }
     for (int j = 0; j < count; ++j) {
         cd_job *J = jobs[j];
         wl.descs.push_back(J->h);
-        for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
         wl.n_cds += J->N * J->m;
         for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
             for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS, w, 0));
@@ -408,7 +423,7 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 Launch job_launch(cd_job *J) {
     return Launch{J->d_self, 1,      J->d_off,     J->N,         J->d_dom,   J->n_dom,   J->d_asc,
                   J->n_asc,  J->lds, J->stem_lds, J->niche_lds, J->m,       J->stem_parts, J->d_slist,
-                  J->d_sctl, J->N * J->m};
+                  J->d_sctl, J->N * J->m, 0, 0};
 }
 
 }  // namespace
@@ -696,6 +711,8 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     B->mg = wl.mg;
     B->stem_parts = wl.stem_parts;
     B->n_cds = wl.n_cds;
+    B->n_small = wl.n_small;
+    B->lds_small = wl.lds_small;
     B->n_off = (int)wl.off.size();
     B->n_dom = (int)wl.dom.size();
     B->n_asc = (int)wl.asc.size();
@@ -726,7 +743,7 @@ int cd_batch_step(cd_batch *B, int32_t generations) {
         const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off,      B->n_off,   B->d_dom,
                        B->n_dom,  B->d_asc,            B->n_asc,      B->lds,     B->stem_lds,
                        B->niche_lds, B->mg,            B->stem_parts, B->d_slist, B->d_sctl,
-                       B->n_cds};
+                       B->n_cds,     B->n_small,       B->lds_small};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);
         if (rc != CD_OK) return rc;
     }

@@ -42,6 +42,8 @@ struct cd_batch {
     uint32_t *d_slist = nullptr;  // changed CDS of a generation (n_cds entries)
     int32_t *d_sctl = nullptr;
     int n_off = 0, n_dom = 0, n_asc = 0, n_cds = 0;
+    int n_small = 0;        // children first in d_off whose k_design fits six per CU
+    size_t lds_small = 0;   // their k_design LDS
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0;          // largest m of the batch's jobs
     int stem_parts = 0;  // largest k_stem workgroups per CDS of the batch's jobs
@@ -81,6 +83,11 @@ struct Launch {
     uint32_t *stem_list;  // changed CDS of a generation, n_cds entries at most
     int32_t *stem_ctl;    // its count, zero between generations (k_dominance clears it)
     int n_cds;            // CDS of all children: sum of N * m
+    // the first n_small children (0: none apart) are of jobs whose k_design
+    // LDS, at most lds_small, fits six workgroups per CU; they are launched
+    // apart from the others, which take lds
+    int n_small;
+    size_t lds_small;
 };
 
 // Host images of the work lists of `jobs` (job k = index k of the list)
@@ -91,6 +98,8 @@ struct WorkLists {
     std::vector<int4> dom;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0, stem_parts = 0, n_cds = 0;
+    int n_small = 0;  // children (first in off) of jobs with lds at most cd::DESIGN_LDS_SIX
+    size_t lds_small = 0;
 };
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl);

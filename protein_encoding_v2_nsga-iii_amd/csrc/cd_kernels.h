@@ -24,10 +24,17 @@ hipError_t set_design_lds_limit(size_t bytes);
 hipError_t set_stem_lds_limit(size_t bytes);
 
 // items: (job, design or child).  The offspring mode lists every changed CDS
-// in stem_list (entry (item << 4) | c, stem_cap entries at most, the sum of
-// the items' m) and counts them in stem_ctl[0]; the three are required there.
+// in stem_list (entry ((item_base + item) << 4) | c, stem_cap entries at
+// most, the sum of m over the children list) and counts them in
+// stem_ctl[0]; the three are required there.  item_base: the position of
+// items[0] in the children list k_stem_list reads.
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
-                         uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, int stem_cap = 0);
+                         uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, int stem_cap = 0,
+                         int item_base = 0);
+// Dynamic LDS of k_design at which six workgroups fit in one CU's LDS (with
+// its static LDS); the batched and served forms launch the children of such
+// jobs apart from the others
+constexpr size_t DESIGN_LDS_SIX = 24 * 1024;
 // initialisation and evaluation: parts workgroups per (item, CDS c < mg),
 // parts the largest stem_parts_of of the jobs: stem of each CDS k_design marked
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,

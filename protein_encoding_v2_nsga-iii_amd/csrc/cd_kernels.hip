@@ -147,15 +147,16 @@ __device__ __forceinline__ int stem_seed(const uint2 *sp, int n, int l, int p, i
 // queued, whether lanes have lookups left, whether one ran out of budget.
 // Sets *best to the workgroup's maximum key and returns whether every lookup
 // finished.
-__device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *rp, int K, uint32_t *head, int32_t *ctl,
+__device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *rp, int K, uint16_t *head, int32_t *ctl,
                            uint64_t *redu, uint64_t &best) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     const int st = qg_stride(K), NI = qg_index_count(n, st);
-    uint16_t *next = (uint16_t *)(head + QG_CODES);
+    uint16_t *next = head + QG_CODES;
     uint32_t *queue = (uint32_t *)((uint8_t *)next + qg_link_bytes(n));
     uint4 *h4 = (uint4 *)head;
+    constexpr uint32_t NIL2 = QG_NIL | (QG_NIL << 16);
     __syncthreads();  // an earlier search's index and queue are no longer read
-    for (int i = tid; i < QG_CODES / 4; i += NT) h4[i] = make_uint4(QG_NIL, QG_NIL, QG_NIL, QG_NIL);
+    for (int i = tid; i < QG_CODES / 8; i += NT) h4[i] = make_uint4(NIL2, NIL2, NIL2, NIL2);
     __syncthreads();
     for (int k = tid; k < NI; k += NT) qg_insert(sp, st, k, head, next);
     __syncthreads();
@@ -207,7 +208,7 @@ __device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *
 }
 
 __device__ __forceinline__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_from, int part, int parts, int32_t *bcast,
-                             uint64_t *redu, uint32_t *qhead) {
+                             uint64_t *redu, uint16_t *qhead) {
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const int W = scan_words(n);
     const uint2 *sp = planes, *rp = planes + W;
@@ -846,7 +847,8 @@ __device__ void build_planes(const uint8_t *g, const uint8_t *nt, int L, uint2 *
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
-                                                uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_cap) {
+                                                uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_cap,
+                                                int item_base) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ DesignShared sh;
     const int2 it = items[blockIdx.x];
@@ -992,7 +994,8 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         if (MODE == MODE_OFFSPRING && nd) {  // the changed CDS for k_stem_list
             int e = atomicAdd(stem_ctl, (int)nd);
             for (int c = 0; c < m; ++c)
-                if (sh.dirty[c] && e < stem_cap) gstore(stem_list + e++, ((uint32_t)blockIdx.x << 4) | (uint32_t)c);
+                if (sh.dirty[c] && e < stem_cap)
+                    gstore(stem_list + e++, ((uint32_t)(item_base + (int)blockIdx.x) << 4) | (uint32_t)c);
         }
     }
 }
@@ -1076,7 +1079,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         __syncthreads();  // the previous entry's LDS is no longer read
         for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
         __syncthreads();
-        const uint64_t key = stem_cds(n, (const uint2 *)planes, cs, 0, 1, bcast, redu, (uint32_t *)(planes + nwords));
+        const uint64_t key = stem_cds(n, (const uint2 *)planes, cs, 0, 1, bcast, redu, (uint16_t *)(planes + nwords));
         if (threadIdx.x == 0 && key) {
             atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
             atomicMax(&J.stem_max[slot], (int)(key >> 40));
@@ -1833,14 +1836,15 @@ __global__ void k_zero_np(const DevJob *__restrict__ jobs) {
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
-                         uint32_t *stem_list, int32_t *stem_ctl, int stem_cap) {
+                         uint32_t *stem_list, int32_t *stem_ctl, int stem_cap, int item_base) {
     if (n <= 0) return hipSuccess;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0); break;
-    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0); break;
+    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0, 0); break;
+    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0, 0); break;
     default:
-        if (!stem_list || !stem_ctl || stem_cap <= 0) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, stem_cap);
+        if (!stem_list || !stem_ctl || stem_cap <= 0 || item_base < 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, stem_cap,
+                           item_base);
         break;
     }
     return hipGetLastError();

@@ -54,16 +54,25 @@ __host__ __device__ inline uint32_t qg_code(const uint2 *P, int a) {
 }
 
 // Insert entry k (s position st*k) at the head of its chain.  head holds
-// QG_CODES entries, QG_NIL when empty; next[k] receives the old head.
-__host__ __device__ inline void qg_insert(const uint2 *sp, int st, int k, uint32_t *head, uint16_t *next) {
+// QG_CODES 16-bit entries, QG_NIL when empty, two to a 32-bit word (the
+// exchange is a compare-and-swap of that word); next[k] receives the old
+// head.
+__host__ __device__ inline void qg_insert(const uint2 *sp, int st, int k, uint16_t *head, uint16_t *next) {
     const uint32_t g = qg_code(sp, st * k);
+    uint32_t *w = (uint32_t *)head + (g >> 1);
+    const int sh = (int)(g & 1u) * 16;
+    const uint32_t keep = ~(0xFFFFu << sh), put = (uint32_t)k << sh;
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint32_t old = atomicExch(head + g, (uint32_t)k);
+    uint32_t cur = *w, prev;
+    do {
+        prev = cur;
+        cur = atomicCAS(w, prev, (prev & keep) | put);
+    } while (cur != prev);
 #else
-    const uint32_t old = head[g];
-    head[g] = (uint32_t)k;
+    const uint32_t prev = *w;
+    *w = (prev & keep) | put;
 #endif
-    next[k] = (uint16_t)old;
+    next[k] = (uint16_t)(prev >> sh);
 }
 
 // The maximal run through the match (a, b) (a valid pair), clipped to the
@@ -114,7 +123,7 @@ __host__ __device__ inline uint32_t qg_lane_code(const QgLane &s, int j) {
 
 // Plane bits of chunk ch (one pair of plane words) and its positions with a
 // chain (the eight heads are read together).
-__host__ __device__ inline void qg_lane_load(QgLane &s, const uint2 *rp, const uint32_t *head, int xe) {
+__host__ __device__ inline void qg_lane_load(QgLane &s, const uint2 *rp, const uint16_t *head, int xe) {
     const int x0 = s.ch << 3, w = x0 >> 5;
     const uint2 p0 = rp[w], p1 = rp[w + 1];
     s.lo = scan_funnel(p1.x, p0.x, x0 & 31);
@@ -125,7 +134,7 @@ __host__ __device__ inline void qg_lane_load(QgLane &s, const uint2 *rp, const u
     s.mask = mask;
 }
 
-__host__ __device__ inline void qg_lane_start(QgLane &s, const uint2 *rp, const uint32_t *head, int ch0, int nch,
+__host__ __device__ inline void qg_lane_start(QgLane &s, const uint2 *rp, const uint16_t *head, int ch0, int nch,
                                               int xe) {
     s.ch = ch0;
     s.e = QG_NIL;
@@ -140,7 +149,7 @@ __host__ __device__ inline bool qg_lane_done(const QgLane &s, int nch) { return 
 // One step of a lane that is not done: returns true with the hit (a, b)
 // when the step read a chain entry whose q-gram is valid.  Each chain entry
 // read costs one unit of budget.
-__host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const uint32_t *head, const uint16_t *next,
+__host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const uint16_t *head, const uint16_t *next,
                                              int n, int st, int nch, int stride, int &budget, int &a, int &b) {
     if (s.e == QG_NIL) {
         if (!s.mask) {  // the chunk is done: the lane's next chunk
@@ -169,13 +178,13 @@ __host__ __device__ inline int qg_budget(int lookups) { return 4 * lookups + 64;
 
 // Hits per wave measured together: the queue of a workgroup of QG_WAVES
 // waves holds QG_WAVES * QG_SEG packed hits.
-constexpr int QG_SEG = 512;
+constexpr int QG_SEG = 256;
 constexpr int QG_WAVES = 4;
 
 // LDS of a search on n bases: the chain heads, the links of at most
 // qg_index_max(n) entries, the hit queue.
 __host__ __device__ inline int qg_link_bytes(int n) { return (2 * qg_index_max(n) + 15) & ~15; }
-__host__ __device__ inline int qg_lds_bytes(int n) { return QG_CODES * 4 + qg_link_bytes(n) + QG_WAVES * QG_SEG * 4; }
+__host__ __device__ inline int qg_lds_bytes(int n) { return QG_CODES * 2 + qg_link_bytes(n) + QG_WAVES * QG_SEG * 4; }
 
 // First K of the search of a changed CDS of n bases, 0 for none: the seed
 // (the length of a stem the CDS is known to have) when it is at least

@@ -140,7 +140,8 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res, cdhost::WorkLi
     if (e == hipSuccess) e = cdhost::set_lds_limits(wl);
     L = cdhost::Launch{S->d_jobs,   (int)wl.descs.size(), S->d_off,    (int)wl.off.size(), S->d_dom,
                        (int)wl.dom.size(), S->d_asc,     (int)wl.asc.size(), wl.lds,       wl.stem_lds,
-                       wl.niche_lds, wl.mg, wl.stem_parts, S->d_slist, S->d_sctl, wl.n_cds};
+                       wl.niche_lds, wl.mg, wl.stem_parts, S->d_slist, S->d_sctl, wl.n_cds,
+                       wl.n_small,   wl.lds_small};
     return e;
 }
 

@@ -127,7 +127,7 @@ int main(int argc, char **argv) {
                     std::printf("index larger than its LDS share n=%d K=%d\n", n, K);
                     std::exit(1);
                 }
-                std::vector<uint32_t> head(cd::QG_CODES, cd::QG_NIL);
+                std::vector<uint16_t> head(cd::QG_CODES, (uint16_t)cd::QG_NIL);
                 std::vector<uint16_t> next(NI);
                 for (int k = 0; k < NI; ++k) cd::qg_insert(sp.data(), st, k, head.data(), next.data());
                 const int xe = n - cd::QG_Q + 1, nch = (xe + 7) >> 3;
