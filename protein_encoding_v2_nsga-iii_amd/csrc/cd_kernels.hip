@@ -106,21 +106,19 @@ struct DesignLds {
 
 // ---------------------------------------------------------------------------
 // Longest stem of one CDS held as bit planes in LDS (calMaximumSL,
-// utils.cuh:712-888; the planes are built by k_design): the lanes of the
-// `parts` workgroups of the
-// CDS split its tiles (cd_tile.h) into contiguous ranges, and the workgroup
-// takes the maximum key (longest, then smallest p, then smallest q).  A CDS
-// with no run of 4 and no seed of 4 is scanned exactly along its
-// anti-diagonals (cd_scan.h).
-// seed_from, when given, is the parent's record of this CDS: the length of
-// the parent's stem as it stands in the child is a stem the child has, so the
-// scan starts with that window.
+// utils.cuh:712-888; the planes are built by k_design).  A changed CDS of an
+// offspring generation is searched by q-grams (cd_qgram.h) from its parent's
+// stem; the initial population, and a CDS the search cannot decide, by tiles
+// (cd_tile.h): the lanes of the `parts` workgroups of the CDS split its tiles
+// into contiguous ranges.  Each workgroup takes the maximum key (longest, then
+// smallest p, then smallest q).  A CDS with no run of 4 and no seed of 4 is
+// scanned exactly along its anti-diagonals (cd_scan.h).
 // ---------------------------------------------------------------------------
-// Wave 0: the longest run of pairs (i - k, j + k), k < 64, that still pair
-// in the child, with (i, j) = (p + l - 1, q) the parent's innermost pair.
-// Each such run is a stem of the child (its pairs lie on one anti-diagonal
-// and enclose at least the parent's loop), so its length bounds the longest
-// stem from below; only its value up to 8 matters to the scan.
+// The seed (wave 0): the longest run of pairs (i - k, j + k), k < 64, that
+// still pair in the child, with (i, j) = (p + l - 1, q) the parent's
+// innermost pair.  Each such run is a stem of the child (its pairs lie on one
+// anti-diagonal and enclose at least the parent's loop), so its length bounds
+// the longest stem from below.
 __device__ __forceinline__ int plane_base(const uint2 *sp, int p) {
     const uint2 w = sp[p >> 5];
     const int b = p & 31;
@@ -154,9 +152,12 @@ __device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *
     uint16_t *next = head + QG_CODES;
     uint32_t *queue = (uint32_t *)((uint8_t *)next + qg_link_bytes(n));
     uint4 *h4 = (uint4 *)head;
-    constexpr uint32_t NIL2 = QG_NIL | (QG_NIL << 16);
+    // two empty heads (QG_NIL) per word; made here rather than kept in
+    // registers across the search loop and the tile fallback's call
+    uint32_t nil2;
+    asm volatile("v_mov_b32 %0, -1" : "=v"(nil2));
     __syncthreads();  // an earlier search's index and queue are no longer read
-    for (int i = tid; i < QG_CODES / 8; i += NT) h4[i] = make_uint4(NIL2, NIL2, NIL2, NIL2);
+    for (int i = tid; i < QG_CODES / 8; i += NT) h4[i] = make_uint4(nil2, nil2, nil2, nil2);
     __syncthreads();
     for (int k = tid; k < NI; k += NT) qg_insert(sp, st, k, head, next);
     __syncthreads();
@@ -207,34 +208,12 @@ __device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *
     return !over;
 }
 
-__device__ __forceinline__ uint64_t stem_cds(int n, const uint2 *planes, const CdsStat *seed_from, int part, int parts, int32_t *bcast,
-                             uint64_t *redu, uint16_t *qhead) {
-    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
-    const int W = scan_words(n);
-    const uint2 *sp = planes, *rp = planes + W;
-    if (wid == 0) {
-        const int pl = seed_from ? seed_from->l : 0;
-        const int seed = seed_from ? stem_seed(sp, n, pl, seed_from->p, seed_from->q, lane) : 0;
-        if (lane == 0) {
-            bcast[0] = seed;
-            bcast[2] = pl;
-        }
-    }
-    __syncthreads();
-    const int seed = bcast[0];
-    // A changed CDS (one workgroup) is searched by q-grams: for the runs of
-    // at least the seed when that is 7 or more (exact), else for the runs of
-    // at least a guess (exact when one is found), then of at least 7.  The
-    // tiles follow a fruitless guess of 7, a search out of budget, or a CDS
-    // the search does not take.
-    if (seed_from) {
-        for (int K = qg_first_k(n, seed, bcast[2]); K;) {
-            uint64_t qbest;
-            if (!stem_qgram(n, sp, rp, K, qhead, bcast, redu, qbest)) break;
-            if (seed >= QG_KMIN || (int)(qbest >> 40) >= K) return qbest;
-            K = qg_next_k(K);
-        }
-    }
+// The tiled scan of share `part` of `parts` of a CDS with a stem of `seed`
+// known (0: none), and its exact scan when there is neither a seed of 4 nor
+// a run of 4; returns the workgroup's maximum key.
+__device__ __forceinline__ uint64_t stem_tiles(int n, const uint2 *sp, const uint2 *rp, int seed, int part, int parts,
+                                               uint64_t *redu) {
+    const int tid = threadIdx.x, NT = blockDim.x;
     // Every lane's K is a length some stem of this CDS has, so the wave may
     // share the largest after each step.  Trip counts are the same for every
     // lane, so every lane takes part in the exchanges.
@@ -272,6 +251,46 @@ __device__ __forceinline__ uint64_t stem_cds(int n, const uint2 *planes, const C
         best = block_max_u64(best, redu);
     }
     return best;
+}
+
+// The whole CDS's tiled scan out of line, for the offspring search's rare
+// fallback: inlined, its registers would raise the pressure of the search
+// path.  The LDS pointers keep their address space, so the scan's loads stay
+// LDS loads.
+typedef __attribute__((address_space(3))) uint2 lds_uint2;
+typedef __attribute__((address_space(3))) uint64_t lds_uint64;
+__device__ __noinline__ uint64_t stem_tiles_whole(int n, const lds_uint2 *planes, int seed, lds_uint64 *redu) {
+    const uint2 *sp = (const uint2 *)planes;
+    return stem_tiles(n, sp, sp + scan_words(n), seed, 0, 1, (uint64_t *)redu);
+}
+
+// Longest stem of a changed CDS of an offspring generation (one workgroup),
+// seeded by its parent's record: q-gram searches for the runs of at least
+// the seed when that is 7 or more (exact), else for the runs of at least a
+// guess (exact when one is found), then of at least 7; the tiles follow a
+// fruitless guess of 7, a search out of budget, or a CDS the search does not
+// take.
+__device__ __forceinline__ uint64_t stem_child(int n, const uint2 *planes, const CdsStat *parent, int32_t *bcast,
+                                               uint64_t *redu, uint16_t *qhead) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint2 *sp = planes, *rp = planes + scan_words(n);
+    if (wid == 0) {
+        const int pl = parent->l;
+        const int seed = stem_seed(sp, n, pl, parent->p, parent->q, lane);
+        if (lane == 0) {
+            bcast[0] = seed;
+            bcast[2] = pl;
+        }
+    }
+    __syncthreads();
+    const int seed = bcast[0];
+    for (int K = qg_first_k(n, seed, bcast[2]); K;) {
+        uint64_t qbest;
+        if (!stem_qgram(n, sp, rp, K, qhead, bcast, redu, qbest)) break;
+        if (seed >= QG_KMIN || (int)(qbest >> 40) >= K) return qbest;
+        K = qg_next_k(K);
+    }
+    return stem_tiles_whole(n, (const lds_uint2 *)planes, seed, (lds_uint64 *)redu);
 }
 
 // Workgroups of k_stem per CDS of n bases: about three tiles per lane or
@@ -1029,7 +1048,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                               int mg, int nic) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];
-    __shared__ int32_t bcast[16];
     const int part = (int)(blockIdx.x / (unsigned)nic), ic = (int)(blockIdx.x % (unsigned)nic);
     const int2 it = items[ic / mg];
     const int c = ic % mg;
@@ -1045,7 +1063,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
     for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
     __syncthreads();
-    const uint64_t key = stem_cds(n, (const uint2 *)planes, nullptr, part, parts, bcast, redu, nullptr);
+    const uint2 *sp = (const uint2 *)planes;
+    const uint64_t key = stem_tiles(n, sp, sp + scan_words(n), 0, part, parts, redu);
     if (threadIdx.x == 0 && key) atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
 }
 
@@ -1053,7 +1072,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // k_stem_list (offspring generations): the changed CDS k_design listed,
 // entry (item << 4) | c, ctl[0] of them; gridDim.x workgroups take the
 // entries round-robin, one workgroup per CDS.  The parent's stem seeds the
-// q-gram search (stem_cds).  Each entry's key goes into the CDS record and
+// q-gram search (stem_child).  Each entry's key goes into the CDS record and
 // its length into the design's longest stem (stem_max) with atomic maxima.
 // k_dominance clears ctl[0] for the next generation.
 // ---------------------------------------------------------------------------
@@ -1079,7 +1098,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         __syncthreads();  // the previous entry's LDS is no longer read
         for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
         __syncthreads();
-        const uint64_t key = stem_cds(n, (const uint2 *)planes, cs, 0, 1, bcast, redu, (uint16_t *)(planes + nwords));
+        const uint64_t key = stem_child(n, (const uint2 *)planes, cs, bcast, redu, (uint16_t *)(planes + nwords));
         if (threadIdx.x == 0 && key) {
             atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
             atomicMax(&J.stem_max[slot], (int)(key >> 40));
