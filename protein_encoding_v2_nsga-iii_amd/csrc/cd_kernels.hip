@@ -539,16 +539,27 @@ __device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty) {
         const int base = A.D->pbase[k], cur = A.cod[t] - base;
         const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
         const int right = A.cod[t + 1];
-        auto score = [&](int g) -> float {
+        // every synonym's score, the pair-score loads issued together
+        // (addresses clamped into the table, values masked: the same sums)
+        const bool hl = left >= 0, hr = right < NSENSE;
+        const float *rowl = J.cps + (hl ? left : 0) * NSENSE, *colr = J.cps + (hr ? right : 0);
+        float sc[MAX_SYN];
+#pragma unroll
+        for (int j = 0; j < MAX_SYN; ++j) {
+            const int g = base + (j < ns ? j : 0);
+            const float a = gload(rowl + g), b = gload(colr + g * NSENSE);
             float v = 0.f;
-            if (left >= 0) v += J.cps[left * NSENSE + g];
-            if (right < NSENSE) v += J.cps[g * NSENSE + right];
-            return v;
-        };
-        const float curv = score(base + cur);
+            v += hl ? a : 0.f;
+            v += hr ? b : 0.f;
+            sc[j] = v;
+        }
+        float curv = sc[0];
+#pragma unroll
+        for (int j = 1; j < MAX_SYN; ++j) curv = j == cur ? sc[j] : curv;
         uint32_t cand = 0;  // synonyms scoring above the current codon, in order
-        for (int j = 0; j < ns; ++j)
-            if (j != cur && score(base + j) > curv) cand |= 1u << j;
+#pragma unroll
+        for (int j = 0; j < MAX_SYN; ++j)
+            if (j < ns && j != cur && sc[j] > curv) cand |= 1u << j;
         if (cand) {
             A.cod[t] = (uint8_t)(base + nth_bit(cand, uniform_int(r[1], (uint32_t)__popc(cand))));
             dirty[cds] = 1;
@@ -628,6 +639,7 @@ __device__ __forceinline__ int perm_at(uint32_t perm, int j) { return (int)((per
 __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
     const DevJob &J = *A.J;
     const int L = J.L, m = J.m;
+    const uint8_t *nd = A.D->nd;
     for (int k = threadIdx.x; k < L; k += blockDim.x) {
         const uint32_t slot = (uint32_t)(cds1 * L + k);
         uint32_t a[4], d[5];
@@ -637,30 +649,31 @@ __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
         const uint32_t order = shuffle(d, ns);
         const int base = A.D->pbase[k], cur = A.cod[cds1 * L + k] - base;
         const int g2 = A.cod[cds2 * L + k];
-        // the other CDS's codons at k (only cds1's codons change here, each by
-        // its own thread), in registers
-        int oc[MAX_M];
+        // The table rows of every candidate (the synonyms in shuffled order,
+        // slot MAX_SYN the current codon) are read before any is compared,
+        // so the loads of a CDS overlap: the smallest distance to the other
+        // CDS's codons at k (only cds1's codons change here, each by its own
+        // thread) and the distance to cds2's codon.
+        int row[MAX_SYN + 1], mn[MAX_SYN + 1], pr[MAX_SYN + 1];
 #pragma unroll
-        for (int o = 0; o < MAX_M; ++o) oc[o] = (o < m && o != cds1) ? (int)A.cod[o * L + k] : -1;
-        auto min_hd = [&](int g) -> int {
-            int mn = 127;
+        for (int j = 0; j <= MAX_SYN; ++j) {
+            row[j] = (base + (j == MAX_SYN ? cur : perm_at(order, j < ns ? j : ns - 1))) * NCODON;
+            mn[j] = 127;
+            pr[j] = nd[row[j] + g2];
+        }
+        for (int o = 0; o < m; ++o) {
+            if (o == cds1) continue;
+            const int oc = A.cod[o * L + k];
 #pragma unroll
-            for (int o = 0; o < MAX_M; ++o) {
-                if (o >= m) break;
-                if (oc[o] < 0) continue;
-                const int t = A.D->nd[g * NCODON + oc[o]];
-                mn = t < mn ? t : mn;
-            }
-            return mn;
-        };
-        const int cur_pair = A.D->nd[(base + cur) * NCODON + g2];
-        const int cur_mhd = min_hd(base + cur);
+            for (int j = 0; j <= MAX_SYN; ++j) mn[j] = min(mn[j], (int)nd[row[j] + oc]);
+        }
+        const int cur_pair = pr[MAX_SYN], cur_mhd = mn[MAX_SYN];
         int best_pair = -1, best_mhd = -1, best = -1;
-        for (int j = 0; j < ns; ++j) {
+#pragma unroll
+        for (int j = 0; j < MAX_SYN; ++j) {
             const int sh = perm_at(order, j);
-            if (sh == cur) continue;
-            const int np_ = A.D->nd[(base + sh) * NCODON + g2];
-            const int nm = min_hd(base + sh);
+            if (j >= ns || sh == cur) continue;
+            const int np_ = pr[j], nm = mn[j];
             if (nm > cur_mhd && nm > best_mhd) {
                 best_mhd = nm;
                 best = sh;
@@ -913,8 +926,9 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         out_slot = N + it.y;
         parent = J.sorted[it.y];
     }
-    lds_barrier();
-
+    // The genome is read in the same phase as the tables: position k's
+    // synonym range was written above by this same thread (k = tid + j*NT in
+    // both loops), so no barrier is needed before it, and the loads overlap.
     if (MODE == 0) {
         const int d = it.y;
         for (int c = 0; c < m; ++c)
@@ -970,7 +984,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
             gstore(J.np + c, 0);
             gstore(J.np + N + c, 0);
         }
-        lds_barrier();
+        lds_barrier();  // tables, codons and records
         uint32_t r[4];
         philox(gen, (uint32_t)c, 0u, TAG_TYPE, J.seed, r);
         const int type = (int)uniform_int(r[0], (uint32_t)(NOBJ + 1));  // main.cu:172-178

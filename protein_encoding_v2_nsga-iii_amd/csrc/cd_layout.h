@@ -14,6 +14,7 @@ constexpr int MIN_HAIRPIN = 4;           // common.cuh:34
 constexpr float F_PRECISION = 0.000001f; // common.cuh:53
 constexpr int GC_UP = 1, GC_DOWN = -1;   // common.cuh:42-43
 constexpr int MAX_M = 16;                // CDS copies per design supported
+constexpr int MAX_SYN = 6;               // most synonyms of one amino acid (Leu, Ser, Arg)
 constexpr int MAX_W = 256;               // dominance words per row: pools up to 16384
 
 // Philox stream tags: counter = (generation, design | child | ref, slot, tag)
