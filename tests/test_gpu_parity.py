@@ -86,6 +86,27 @@ def test_loop_bit_exact(cdp, orc, L, m, N, H, G, kind):
     assert gj.generation == G + 1
 
 
+@pytest.mark.parametrize("pattern", ["L", "A", "AR"])
+def test_loop_bit_exact_low_complexity(cdp, orc, pattern):
+    # repeats steer the stem search off its usual path: poly-Leu CDS have
+    # stems of 2-4 (guesses of 7 miss; the tiles and the exact scan decide),
+    # poly-Ala and Ala-Arg ones stems of 15-30 that match themselves often
+    _say("loop low complexity", pattern)
+    T = cdp.tables
+    tab = T.synth_tables(7)
+    aa = np.array([T.amino_index("M")] + [T.amino_index(x) for x in pattern] * (150 // len(pattern)) + [T.STOP],
+                  np.int8)
+    gj = cdp.Job(aa, tab, 2, 32, 3, 0.1, 5).init()
+    oj = orc.Job(aa, tab, 2, 32, 3, 0.1, 5)
+    oj.init()
+    for gen in range(7):
+        a, b = gj.read(), oj.read()
+        for k in ("genome", "obj", "idx", "pql", "sorted", "front_of"):
+            assert np.array_equal(a[k], b[k]), (pattern, gen, k, np.argwhere(a[k] != b[k])[:5])
+        gj.step(1)
+        oj.step(1)
+
+
 def test_errors_are_reported(cdp):
     T = cdp.tables
     tab = T.reference_tables()
