@@ -1764,15 +1764,21 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
         // level 0 of an empty niche takes its nearest candidate, ties to the
         // lowest index; a sweep in ascending index starts from the lowest index
         // and keeps it when its distance is NaN or nothing compares below
-        // FLT_MAX.  One wave per reference point with picks.
-        for (int r = wid; r < R; r += nw) {
-            int l0 = -1;
-            if (kp[r] > 0 && cnt[r] == 0) {
+        // FLT_MAX.  A group of gw lanes per reference point: 16 (four points
+        // per wave) when the last front's buckets hold 8 designs or fewer on
+        // average, else the whole wave; both minima are folded over the group
+        // by butterflies (the (distance, index) minimum never takes a NaN, so
+        // its order does not matter).
+        const int gw = total_last <= 8 * R ? 16 : 64, gpw = 64 / gw;
+        for (int r0 = wid * gpw; r0 < R; r0 += nw * gpw) {
+            const int r = r0 + lane / gw, gl = lane & (gw - 1);
+            const bool pick = r < R && kp[r] > 0 && cnt[r] == 0;
+            float bd = FLT_MAX, md = 0.f;
+            int bi = 0x7FFFFFFF, mi = 0x7FFFFFFF;
+            if (pick) {
                 const int *b = lb + off[r];
                 const int nb = rem[r];
-                float bd = FLT_MAX, md = 0.f;
-                int bi = 0x7FFFFFFF, mi = 0x7FFFFFFF;
-                for (int t = lane; t < nb; t += 64) {
+                for (int t = gl; t < nb; t += gw) {
                     const int e = b[t];
                     const float de = ld ? ld[off[r] + t] : J.dist[e];
                     argmin_pair(bd, bi, de, e);
@@ -1781,13 +1787,17 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
                         md = de;
                     }
                 }
-                wave_argmin(bd, bi);
-                const int lo_i = wave_min_i32(mi);
-                const uint64_t who = __ballot(mi == lo_i);
-                md = __shfl(md, (int)__builtin_ctzll(who), 64);
-                l0 = (bi == 0x7FFFFFFF || isnan(md)) ? lo_i : bi;
             }
-            if (lane == 0) lvl0[r] = l0;
+            for (int o = gw >> 1; o > 0; o >>= 1) {  // every lane takes part
+                const float ov = __shfl_xor(bd, o, 64), omd = __shfl_xor(md, o, 64);
+                const int oi = __shfl_xor(bi, o, 64), omi = __shfl_xor(mi, o, 64);
+                argmin_pair(bd, bi, ov, oi);
+                if (omi < mi) {
+                    mi = omi;
+                    md = omd;
+                }
+            }
+            if (gl == 0 && r < R) lvl0[r] = pick ? ((bi == 0x7FFFFFFF || isnan(md)) ? mi : bi) : -1;
         }
         for (int d = tid; d < P; d += NT) {
             const int f = fo[d];
