@@ -1921,7 +1921,10 @@ hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool f
     return hipGetLastError();
 }
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_fronts, dim3(njobs), dim3(1024), 0, s, jobs);
+    // a lone job's workgroup is the only work on its CU, where the waves'
+    // repeated reductions bound it: 8 waves (measured 22.7 -> 21.6 us at
+    // L = 501, N = 1024); a batch's slowest job gains from 16
+    hipLaunchKernelGGL(k_fronts, dim3(njobs), dim3(njobs == 1 ? 512 : 1024), 0, s, jobs);
     return hipGetLastError();
 }
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
