@@ -11,8 +11,8 @@ stream around K graph-launched generations, after W warmup generations.
 Prints one JSON line with designs/s (N*K / time), per-generation latency
 percentiles, the roofline of the dominant kernel of the live per-kernel
 profile (k_design's offspring launch: HBM-bound, the algorithmic bytes of a
-child, parent read plus survivor and child written, against 8 TB/s; k_stem,
-when it dominates: VALU-bound, stem cells against 32 cells per int32
+child, parent read plus survivor and child written, against 8 TB/s; the stem
+scan (k_stem_list), when it dominates: VALU-bound, stem cells against 32 cells per int32
 lane-op), the stem scan's cell rate beside it, and a CPU baseline (the oracle
 on the host cores, bounded sample).
 """
@@ -138,7 +138,7 @@ def main():
     dominant = KERNELS[int(np.argmax(prof))]
     stem_line = {"bound": "valu", "achieved": round(cell_rate / 1e12, 4), "peak": round(PEAK_CELLS / 1e12, 2),
                  "unit": "T stem cells/s (peak: 32 cells per int32 lane-op)", "frac": round(cell_rate / PEAK_CELLS, 5),
-                 "traffic": TRAFFIC_BYTES["stem"] if default_workload else None, "kernel": "k_stem",
+                 "traffic": TRAFFIC_BYTES["stem"] if default_workload else None, "kernel": "k_stem_list",
                  "cells_per_launch": int(cells)}
     if dominant == "stem":
         roofline = stem_line

@@ -1,8 +1,10 @@
 """The engine's stem scans (csrc/cd_scan.h along anti-diagonals, csrc/cd_tile.h
-across them), compiled for the host with the address sanitizer and run lane
-by lane as the stem kernel runs them, against the oracle's closed form (with
-its tie rule) on random, repetitive, skewed and planted-hairpin sequences,
-with and without a seed (a stem length known to exist)."""
+across them, csrc/cd_qgram.h the seeded q-gram search), compiled for the host
+with the address sanitizer and run lane by lane as the stem kernels run them,
+against the oracle's closed form (with its tie rule) on random, repetitive,
+skewed and planted-hairpin sequences, with and without a seed (a stem length
+known to exist), with guesses of the parent's length that are right, too
+small or too large, and with hit budgets that run out."""
 import os
 import shutil
 import subprocess
