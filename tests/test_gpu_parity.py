@@ -86,6 +86,28 @@ def test_loop_bit_exact(cdp, orc, L, m, N, H, G, kind):
     assert gj.generation == G + 1
 
 
+def test_loop_bit_exact_benchmark_job(cdp, orc):
+    # the benchmark's single job (500 residues, m = 4, N = 1024, H = 5, seed
+    # 2): last fronts of ~1480 designs and niching buckets of 100-400 designs
+    _say("loop benchmark job")
+    T = cdp.tables
+    aa = T.synth_protein(500, 2)
+    tab = T.synth_tables(2)
+    gj = cdp.Job(aa, tab, 4, 1024, 5, 0.1, 2).init()
+    oj = orc.Job(aa, tab, 4, 1024, 5, 0.1, 2)
+    oj.init()
+    for gen in range(4):
+        a, b = gj.read(), oj.read()
+        for k in ("genome", "obj", "idx", "pql", "sorted", "front_of", "extremes", "ideal", "worst", "nadir"):
+            if gen == 0 and k in ("ideal", "worst", "nadir"):
+                continue
+            assert np.array_equal(a[k], b[k]), (gen, k, np.argwhere(a[k] != b[k])[:5])
+        if gen < 3:
+            gj.step(1)
+            oj.step(1)
+        _say("benchmark job generation", gen, "equal")
+
+
 @pytest.mark.parametrize("pattern", ["L", "A", "AR"])
 def test_loop_bit_exact_low_complexity(cdp, orc, pattern):
     # repeats steer the stem search off its usual path: poly-Leu CDS have
