@@ -109,7 +109,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(jobs shard across GPUs with no communication, see DESIGN.md)"}))
+                                   "(jobs would shard across GPUs by job, with no communication)"}))
         sys.exit(2)
 
     if args.served > 0:
