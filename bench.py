@@ -108,7 +108,7 @@ def main():
                          "kernel of known size, to calibrate hardware byte counters); not part of the measurement")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
                                    "(jobs would shard across GPUs by job, with no communication)"}))
         sys.exit(2)
 
