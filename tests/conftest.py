@@ -12,7 +12,7 @@ PKG = "protein_encoding_v2_nsga-iii_amd"
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on a machine that has one)")
 
 
 @pytest.fixture(scope="session")
