@@ -358,7 +358,7 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
     const int n_dom = w.n_dom;
     const int2 *asc = w.asc;
     const int n_asc = w.n_asc;
-    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, true, w.stem_ctl, s);
+    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, true, w.stem_ctl, w.dom_rows, s);
     if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
     if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
     if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, w.niche_lds, s);
@@ -382,8 +382,8 @@ void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
         cd_job *J = jobs[j];
         wl.descs.push_back(J->h);
         wl.n_cds += J->N * J->m;
-        for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
-            for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS, w, 0));
+        for (int t = 0; t < (J->P + cd::DOM_ROWS_BATCH - 1) / cd::DOM_ROWS_BATCH; ++t)
+            for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS_BATCH, w, 0));
         for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
             wl.asc.push_back(make_int2(j, t * cd::ASSOC_DESIGNS));
         wl.lds = std::max(wl.lds, J->lds);
@@ -421,7 +421,7 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 }
 
 Launch job_launch(cd_job *J) {
-    return Launch{J->d_self, 1,      J->d_off,     J->N,         J->d_dom,   J->n_dom,   J->d_asc,
+    return Launch{J->d_self, 1,      J->d_off,     J->N,         J->d_dom,   J->n_dom,   cd::DOM_ROWS, J->d_asc,
                   J->n_asc,  J->lds, J->stem_lds, J->niche_lds, J->m,       J->stem_parts, J->d_slist,
                   J->d_sctl, J->N * J->m, 0, 0};
 }
@@ -544,7 +544,7 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(hipEventRecord(ev[1], J->stream));
         HIPCHK(cd::launch_stem_list(J->d_self, J->d_off, J->d_slist, J->d_sctl, J->N * J->m, J->stem_lds, J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
-        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->d_sctl, J->stream));
+        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->d_sctl, cd::DOM_ROWS, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));
         HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
         HIPCHK(hipEventRecord(ev[4], J->stream));
@@ -635,7 +635,7 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     hipError_t e = hipMemcpy(J->h.obj[0], obj, (size_t)P * 24, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(J->h.st, &st, sizeof(st), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
-    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, J->stream);
+    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, cd::DOM_ROWS, J->stream);
     if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->stream);
     if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream);
     if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream);
@@ -741,7 +741,7 @@ int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
         const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off,      B->n_off,   B->d_dom,
-                       B->n_dom,  B->d_asc,            B->n_asc,      B->lds,     B->stem_lds,
+                       B->n_dom,  cd::DOM_ROWS_BATCH,  B->d_asc,      B->n_asc,   B->lds,     B->stem_lds,
                        B->niche_lds, B->mg,            B->stem_parts, B->d_slist, B->d_sctl,
                        B->n_cds,     B->n_small,       B->lds_small};
         int rc = capture(B->stream, &B->graph, &B->gexec, w);

@@ -73,8 +73,8 @@ struct Launch {
     int njobs;
     const int2 *off;  // (job, child) of every child
     int n_off;
-    const int4 *dom;  // (job, first row, word, 0) domination tiles
-    int n_dom;
+    const int4 *dom;  // (job, first row, word, 0) domination tiles of dom_rows rows
+    int n_dom, dom_rows;
     const int2 *asc;  // (job, first design) association groups
     int n_asc;
     size_t lds, stem_lds, niche_lds;

@@ -12,7 +12,8 @@ enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
 
 constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
 constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (one wave each)
-constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup (16 per wave)
+constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup of a lone job (16 per wave)
+constexpr int DOM_ROWS_BATCH = 128;        // rows per k_dominance workgroup of a batch (32 per wave)
 constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
 constexpr int STEM_GRID = 1536;            // most k_stem_list workgroups (6 resident per CU)
 
@@ -46,11 +47,12 @@ hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_
                             const int32_t *stem_ctl, int capacity, size_t lds, hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
-// items: (job, first row, word, 0).  finish: the offspring generation, whose
-// children's stem objective (and stem records) k_dominance completes in place
-// of launch_stem_finish; stem_ctl (or null): the changed-CDS count of that
+// items: (job, first row, word, 0), rows (DOM_ROWS or DOM_ROWS_BATCH) rows
+// per item.  finish: the offspring generation, whose children's stem
+// objective (and stem records) k_dominance completes in place of
+// launch_stem_finish; stem_ctl (or null): the changed-CDS count of that
 // generation's k_design, cleared for the next one
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl,
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
                             hipStream_t s);
 // one workgroup per job
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);

@@ -1183,15 +1183,16 @@ __device__ __forceinline__ bool dominates(const float *a, const float *b) {
 // Bit i of word w of row j (i = 64w + bit) is set when design i dominates
 // design j; the matrix is stored word-major, dom[w * P + j], so that the
 // front peeling reads one word of consecutive rows per wave.  np[j] counts
-// the dominators.  item = (job, first row, word, 0): DOM_ROWS rows of one
-// word, 16 per wave.  Lane b holds candidate 64w + b; a row's word is the
+// the dominators.  item = (job, first row, word, 0): ROWS rows of one word,
+// ROWS/4 per wave.  Lane b holds candidate 64w + b; a row's word is the
 // ballot of the 64 tests against that row (row values read from LDS as
-// broadcasts), and the wave stores its 16 words and adds their counts at
-// once.
+// broadcasts), and the wave stores its words and adds their counts at once.
+template <int ROWS>
 __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items,
                                                    bool finish, int32_t *stem_ctl) {
-    constexpr int WR = DOM_ROWS / 4;  // rows per wave
-    __shared__ float rows[DOM_ROWS * NOBJ];
+    static_assert(ROWS % 4 == 0 && ROWS / 4 <= 64, "a wave's rows fit one word per lane");
+    constexpr int WR = ROWS / 4;  // rows per wave
+    __shared__ float rows[ROWS * NOBJ];
     if (stem_ctl && blockIdx.x == 0 && threadIdx.x == 0) gstore(stem_ctl, 0);  // k_stem_list has read the count
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
@@ -1204,7 +1205,7 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         if (finish && k == NOBJ - 1 && d >= N) return (float)gload(J.stem_max + d) / (float)J.den_sl;
         return gload(obj + (size_t)d * NOBJ + k);
     };
-    for (int t = threadIdx.x; t < DOM_ROWS * NOBJ; t += blockDim.x) {
+    for (int t = threadIdx.x; t < ROWS * NOBJ; t += blockDim.x) {
         const int r = it.y + t / NOBJ;
         rows[t] = r < P ? obj_at(r, t % NOBJ) : 0.f;
     }
@@ -1213,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
 #pragma unroll
     for (int k = 0; k < NOBJ; ++k) cand[k] = i < P ? obj_at(i, k) : 0.f;
     // the workgroups of word 0 complete the stem records of their children
-    if (finish && word == 0 && threadIdx.x < DOM_ROWS) {
+    if (finish && word == 0 && threadIdx.x < ROWS) {
         const int r = it.y + (int)threadIdx.x;
         if (r >= N && r < P) stem_finish(J, buf, r);
     }
@@ -1914,10 +1915,15 @@ hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, i
     hipLaunchKernelGGL(k_stem_finish, dim3((n + 255) / 256), dim3(256), 0, s, jobs, items, n, mode);
     return hipGetLastError();
 }
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl,
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dominance, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl);
+    if (rows == DOM_ROWS)
+        hipLaunchKernelGGL(k_dominance<DOM_ROWS>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl);
+    else if (rows == DOM_ROWS_BATCH)
+        hipLaunchKernelGGL(k_dominance<DOM_ROWS_BATCH>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {

@@ -139,7 +139,7 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res, cdhost::WorkLi
         e = hipMemcpyAsync(S->d_asc, wl.asc.data(), sizeof(int2) * wl.asc.size(), hipMemcpyHostToDevice, S->stream);
     if (e == hipSuccess) e = cdhost::set_lds_limits(wl);
     L = cdhost::Launch{S->d_jobs,   (int)wl.descs.size(), S->d_off,    (int)wl.off.size(), S->d_dom,
-                       (int)wl.dom.size(), S->d_asc,     (int)wl.asc.size(), wl.lds,       wl.stem_lds,
+                       (int)wl.dom.size(), cd::DOM_ROWS_BATCH, S->d_asc,     (int)wl.asc.size(), wl.lds,       wl.stem_lds,
                        wl.niche_lds, wl.mg, wl.stem_parts, S->d_slist, S->d_sctl, wl.n_cds,
                        wl.n_small,   wl.lds_small};
     return e;
