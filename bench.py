@@ -40,8 +40,8 @@ KERNELS = ("offspring", "stem", "dominance", "fronts", "associate", "niche")
 # HBM bytes per launch at the default workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000 (scaled by a 256 MiB device copy in the same run) from
 # the rocprofv3 --pmc runs summarised in profiles/pmc_single_job_L501_m4_N1024.md
-TRAFFIC_BYTES = {"offspring": int((1937.2 * 2.000 + 5865.2) * 1024),
-                 "stem": int((817.2 * 2.000 + 82.6) * 1024)}
+TRAFFIC_BYTES = {"offspring": int((1952.2 * 2.000 + 5865.2) * 1024),
+                 "stem": int((817.3 * 2.000 + 82.6) * 1024)}
 
 
 def log(msg):
