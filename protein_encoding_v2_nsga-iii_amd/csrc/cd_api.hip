@@ -337,7 +337,8 @@ hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_i
         if (e == hipSuccess)
             e = cd::launch_design(mode, w.jobs, items + ns, n_items - ns, w.lds, s, w.stem_list, w.stem_ctl, w.n_cds,
                                   ns);
-        if (e == hipSuccess) e = cd::launch_stem_list(w.jobs, items, w.stem_list, w.stem_ctl, w.n_cds, w.stem_lds, s);
+        if (e == hipSuccess) e = cd::launch_stem_list(w.jobs, items, w.stem_list, w.stem_ctl, w.n_cds,
+                                                      w.njobs > 1 ? cd::STEM_WAVES_BATCH : cd::STEM_WAVES, w.stem_lds, s);
         return e;
     }
     hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
@@ -542,7 +543,8 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
         HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream, J->d_slist,
                                  J->d_sctl, J->N * J->m));
         HIPCHK(hipEventRecord(ev[1], J->stream));
-        HIPCHK(cd::launch_stem_list(J->d_self, J->d_off, J->d_slist, J->d_sctl, J->N * J->m, J->stem_lds, J->stream));
+        HIPCHK(cd::launch_stem_list(J->d_self, J->d_off, J->d_slist, J->d_sctl, J->N * J->m, cd::STEM_WAVES, J->stem_lds,
+                                    J->stream));
         HIPCHK(hipEventRecord(ev[2], J->stream));
         HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->d_sctl, cd::DOM_ROWS, J->stream));
         HIPCHK(hipEventRecord(ev[3], J->stream));

@@ -15,7 +15,9 @@ constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup 
 constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup of a lone job (16 per wave)
 constexpr int DOM_ROWS_BATCH = 128;        // rows per k_dominance workgroup of a batch (32 per wave)
 constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
-constexpr int STEM_GRID = 1536;            // most k_stem_list workgroups (6 resident per CU)
+constexpr int STEM_CUS = 256;              // compute units of an MI355X: k_stem_list runs a resident grid
+constexpr int STEM_WAVES = 6;              // k_stem_list waves per SIMD (workgroups per CU) for a lone job
+constexpr int STEM_WAVES_BATCH = 7;        // the same for a batch (fewer registers, more searches at once)
 
 size_t design_lds_bytes(int L, int m);
 size_t stem_lds_bytes(int L);
@@ -41,10 +43,12 @@ constexpr size_t DESIGN_LDS_SIX = 24 * 1024;
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,
                        hipStream_t s);
 // offspring generations: stem of each CDS the offspring k_design listed
-// (items as given to it), on min(capacity, STEM_GRID) workgroups, capacity
-// the list's size; launch_dominance with the same stem_ctl clears the count
+// (items as given to it), capacity the list's size, on as many workgroups
+// as are resident at once (STEM_CUS x the workgroups per CU that `waves`,
+// STEM_WAVES or STEM_WAVES_BATCH, and the LDS allow), at most capacity;
+// launch_dominance with the same stem_ctl clears the count
 hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
-                            const int32_t *stem_ctl, int capacity, size_t lds, hipStream_t s);
+                            const int32_t *stem_ctl, int capacity, int waves, size_t lds, hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
 // items: (job, first row, word, 0), rows (DOM_ROWS or DOM_ROWS_BATCH) rows

@@ -1090,7 +1090,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // its length into the design's longest stem (stem_max) with atomic maxima.
 // k_dominance clears ctl[0] for the next generation.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stem_list(
+template <int WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_stem_list(
     const DevJob *__restrict__ jobs, const int2 *__restrict__ items, const uint32_t *__restrict__ list,
     const int32_t *__restrict__ ctl, int cap) {
     extern __shared__ __align__(16) uint8_t dyn[];
@@ -1903,11 +1904,21 @@ hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, i
     return hipGetLastError();
 }
 hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
-                            const int32_t *stem_ctl, int capacity, size_t lds, hipStream_t s) {
+                            const int32_t *stem_ctl, int capacity, int waves, size_t lds, hipStream_t s) {
     if (capacity <= 0) return hipSuccess;
-    if (!stem_list || !stem_ctl) return hipErrorInvalidValue;
-    const int grid = capacity < STEM_GRID ? capacity : STEM_GRID;
-    hipLaunchKernelGGL(k_stem_list, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity);
+    if (!stem_list || !stem_ctl || (waves != STEM_WAVES && waves != STEM_WAVES_BATCH)) return hipErrorInvalidValue;
+    // workgroups take the entries round-robin, so every one must be resident
+    // from the start: per CU, the waves per SIMD or what the LDS (160 KiB,
+    // with the kernel's static 256 bytes) holds
+    const int by_lds = (int)((160u * 1024u) / (lds + 256u));
+    const int per_cu = by_lds < waves ? (by_lds > 1 ? by_lds : 1) : waves;
+    const int grid = capacity < STEM_CUS * per_cu ? capacity : STEM_CUS * per_cu;
+    if (waves == STEM_WAVES)
+        hipLaunchKernelGGL(k_stem_list<STEM_WAVES>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl,
+                           capacity);
+    else
+        hipLaunchKernelGGL(k_stem_list<STEM_WAVES_BATCH>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list,
+                           stem_ctl, capacity);
     return hipGetLastError();
 }
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
@@ -1966,7 +1977,10 @@ hipError_t set_design_lds_limit(size_t bytes) {
 hipError_t set_stem_lds_limit(size_t bytes) {
     hipError_t e = hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_stem_list, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    e = hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES_BATCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
 }
 
 }  // namespace cd
