@@ -41,7 +41,7 @@ KERNELS = ("offspring", "stem", "dominance", "fronts", "associate", "niche")
 # WRITE_SIZE x 1.000 (scaled by a 256 MiB device copy in the same run) from
 # the rocprofv3 --pmc runs summarised in profiles/pmc_single_job_L501_m4_N1024.md
 TRAFFIC_BYTES = {"offspring": int((1952.2 * 2.000 + 5865.2) * 1024),
-                 "stem": int((817.3 * 2.000 + 82.6) * 1024)}
+                 "stem": int((816.7 * 2.000 + 82.6) * 1024)}
 
 
 def log(msg):
