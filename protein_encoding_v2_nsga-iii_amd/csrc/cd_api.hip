@@ -368,9 +368,17 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
 
 void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
     wl = WorkLists();
-    // children of the jobs whose k_design fits six workgroups per CU first
+    // children of the jobs whose k_design fits six workgroups per CU first;
+    // within each class the jobs with the most codons per child (L * m)
+    // first, so that the longest workgroups start early and short ones fill
+    // the end of the launch
+    std::vector<int> order(count);
+    for (int j = 0; j < count; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (int64_t)jobs[a]->L * jobs[a]->m > (int64_t)jobs[b]->L * jobs[b]->m;
+    });
     for (int pass = 0; pass < 2; ++pass)
-        for (int j = 0; j < count; ++j) {
+        for (int j : order) {
             cd_job *J = jobs[j];
             if ((J->lds <= cd::DESIGN_LDS_SIX) != (pass == 0)) continue;
             for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
