@@ -51,6 +51,33 @@ def test_batch_equals_alone(cdp, orc):
         j.close()
 
 
+def test_batch_two_lds_classes(cdp):
+    """A batch whose jobs fall in both k_design launch classes (the 1300-residue
+    job with m=8 needs about 26 KiB of LDS, more than six workgroups per CU
+    allow; the others fit six) and are reordered longest first gives every job
+    exactly its pools alone."""
+    print("batch two LDS classes", flush=True)
+    specs = make_jobs(cdp, [(60, 2, 32, 2, 31), (1300, 8, 16, 2, 32), (900, 4, 24, 2, 33), (200, 3, 40, 3, 34)])
+    G = 3
+    alone = []
+    for s in specs:
+        j = cdp.Job(*s).init()
+        j.step(G)
+        alone.append(j.read())
+        j.close()
+    jobs = [cdp.Job(*s).init() for s in specs]
+    b = cdp.Batch(jobs)
+    b.step(G)
+    b.sync()
+    for i, (j, a) in enumerate(zip(jobs, alone)):
+        r = j.read()
+        for k in ("genome", "obj", "idx", "pql", "sorted", "front_of"):
+            assert np.array_equal(r[k], a[k]), (i, k)
+    b.close()
+    for j in jobs:
+        j.close()
+
+
 def test_batch_rejects_uninitialised(cdp):
     specs = make_jobs(cdp, SPECS[:2])
     j0 = cdp.Job(*specs[0]).init()
