@@ -140,7 +140,10 @@ int cd_select(const float *obj, int32_t N, int32_t H, uint32_t gen, uint64_t see
 
 /* Batched form: the listed jobs (already initialised, distinct, with their
  * own tables and sizes) advance together, one generation per launch
- * sequence of persistent kernels over all of them. */
+ * sequence of persistent kernels over all of them.  While a batch holds a
+ * job, cd_job_step, cd_job_profile and cd_job_destroy of that job fail with
+ * CD_E_STATE, and cd_job_read, cd_job_sync and cd_job_generation first wait
+ * for the batch's stream; cd_batch_destroy releases the jobs. */
 int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out);
 int cd_batch_step(cd_batch *batch, int32_t generations);
 int cd_batch_step_timed(cd_batch *batch, int32_t generations, float *ms);

@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY: the checker the HIP engine is compared against.
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -25,35 +26,77 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _load(path):
+    L = C.CDLL(path)
+    _bind(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
-        L.or_det_exp.argtypes = [C.c_double]
-        L.or_det_exp.restype = C.c_double
-        L.or_dasdennis.argtypes = [C.c_int, C.c_int, f32p]
-        L.or_sl_cds.argtypes = [u8p, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                C.POINTER(C.c_int32)]
-        L.or_eval.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, C.c_float, i8p, C.c_int,
-                              f32p, i8p, i32p]
-        L.or_cai_cpb_ref.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, i8p, C.c_int, C.c_int,
-                                     f32p, f64p]
-        L.or_nds.argtypes = [f32p, C.c_int, C.c_int, i32p, C.POINTER(C.c_int32)]
-        L.or_select.argtypes = [f32p, C.c_int, C.c_int, C.c_uint32, C.c_uint64, f32p, i32p, i32p, f32p, i32p]
-        L.or_job_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, i8p, C.c_char_p,
-                                    f32p, f32p, C.c_float]
-        L.or_job_create.restype = C.c_void_p
-        L.or_job_init.argtypes = [C.c_void_p]
-        L.or_job_step.argtypes = [C.c_void_p, C.c_int]
-        L.or_job_read.argtypes = [C.c_void_p, i8p, f32p, i8p, i32p, i32p, i32p, f32p]
-        L.or_job_generation.argtypes = [C.c_void_p]
-        L.or_job_destroy.argtypes = [C.c_void_p]
-        L.or_set_threads.argtypes = [C.c_int]
-        _lib = L
+        _lib = _load(LIB)
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Route every call (and every Job created) inside the block to another
+    build of the oracle, e.g. one with a deliberately planted mistake that a
+    test must detect; the default library is restored afterwards."""
+    global _lib
+    saved = lib()
+    _lib = _load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def build_variant(out_path, patches):
+    """Compile the oracle with source substitutions into out_path (test use:
+    planting a known mistake).  patches: (function signature, old, new)
+    triples; `old` must occur exactly once inside that function's body."""
+    src = open(os.path.join(HERE, "cd_oracle.cpp")).read()
+    for sig, old, new in patches:
+        a = src.index(sig)
+        b = src.index("\n}\n", a)
+        body = src[a:b]
+        assert body.count(old) == 1, (sig, old)
+        src = src[:a] + body.replace(old, new) + src[b:]
+    cpp = out_path + ".cpp"
+    with open(cpp, "w") as f:
+        f.write(src)
+    return subprocess.Popen(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-I", HERE,
+                             "-o", out_path, cpp])
+
+
+def _bind(L):
+    L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+    L.or_det_exp.argtypes = [C.c_double]
+    L.or_det_exp.restype = C.c_double
+    L.or_dasdennis.argtypes = [C.c_int, C.c_int, f32p]
+    L.or_sl_cds.argtypes = [u8p, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                            C.POINTER(C.c_int32)]
+    L.or_eval.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, C.c_float, i8p, C.c_int,
+                          f32p, i8p, i32p]
+    L.or_cai_cpb_ref.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, i8p, C.c_int, C.c_int,
+                                 f32p, f64p]
+    L.or_mutate.argtypes = [C.c_int, C.c_int, i8p, C.c_char_p, f32p, f32p, C.c_float, C.c_float, C.c_uint64,
+                            C.c_uint32, C.c_uint32, i8p, f32p, i8p, i32p, i8p]
+    L.or_nds.argtypes = [f32p, C.c_int, C.c_int, i32p, C.POINTER(C.c_int32)]
+    L.or_select.argtypes = [f32p, C.c_int, C.c_int, C.c_uint32, C.c_uint64, f32p, i32p, i32p, f32p, i32p]
+    L.or_job_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, i8p, C.c_char_p,
+                                f32p, f32p, C.c_float]
+    L.or_job_create.restype = C.c_void_p
+    L.or_job_init.argtypes = [C.c_void_p]
+    L.or_job_step.argtypes = [C.c_void_p, C.c_int]
+    L.or_job_read.argtypes = [C.c_void_p, i8p, f32p, i8p, i32p, i32p, i32p, f32p]
+    L.or_job_generation.argtypes = [C.c_void_p]
+    L.or_job_destroy.argtypes = [C.c_void_p]
+    L.or_set_threads.argtypes = [C.c_int]
 
 
 def set_threads(n):
@@ -118,6 +161,20 @@ def cai_cpb_reference(aa, tables, syn, threads=256):
     return o32, o64
 
 
+def mutate(aa, tables, Pm, seed, gen, child, parent, pobj, pidx, ppql):
+    """The child genome [m][L] that child `child` of generation `gen` gets
+    from its parent's genome [m][L] and record."""
+    parent = np.ascontiguousarray(parent, np.int8)
+    m, L = parent.shape
+    out = np.zeros_like(parent)
+    rc = lib().or_mutate(L, m, np.ascontiguousarray(aa, np.int8), tables.codon_bytes(), tables.weights,
+                         tables.cps, tables.gc3_ref, float(Pm), int(seed), int(gen), int(child), parent,
+                         np.ascontiguousarray(pobj, np.float32), np.ascontiguousarray(pidx, np.int8),
+                         np.ascontiguousarray(ppql, np.int32), out)
+    assert rc == 0
+    return out
+
+
 def nds(obj, N):
     obj = np.ascontiguousarray(obj, np.float32)
     P = obj.shape[0]
@@ -145,20 +202,21 @@ class Job:
     def __init__(self, aa, tables, m, N, H, Pm, seed):
         self.aa = np.ascontiguousarray(aa, np.int8)
         self.L, self.m, self.N = len(aa), m, N
-        self.h = lib().or_job_create(self.L, m, N, H, Pm, seed, self.aa, tables.codon_bytes(), tables.weights,
-                                     tables.cps, tables.gc3_ref)
+        self._L = lib()  # the library that owns the handle
+        self.h = self._L.or_job_create(self.L, m, N, H, Pm, seed, self.aa, tables.codon_bytes(), tables.weights,
+                                       tables.cps, tables.gc3_ref)
         if not self.h:
             raise ValueError("invalid job")
 
     def init(self):
-        lib().or_job_init(self.h)
+        self._L.or_job_init(self.h)
 
     def step(self, gens=1):
-        lib().or_job_step(self.h, gens)
+        self._L.or_job_step(self.h, gens)
 
     @property
     def generation(self):
-        return lib().or_job_generation(self.h)
+        return self._L.or_job_generation(self.h)
 
     def read(self):
         P = 2 * self.N
@@ -169,11 +227,11 @@ class Job:
         srt = np.zeros(self.N, np.int32)
         fo = np.zeros(P, np.int32)
         st = np.zeros(54, np.float32)
-        lib().or_job_read(self.h, g, obj, idx, pql, srt, fo, st)
+        self._L.or_job_read(self.h, g, obj, idx, pql, srt, fo, st)
         return dict(genome=g, obj=obj, idx=idx, pql=pql, sorted=srt, front_of=fo, extremes=st[:36].reshape(6, 6),
                     ideal=st[36:42], worst=st[42:48], nadir=st[48:54])
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_job_destroy(self.h)
+            self._L.or_job_destroy(self.h)
             self.h = None

@@ -982,19 +982,13 @@ void eval_into(Job &J, int buf, int slot) {
     eval_design(J.T, J.aa.data(), J.L, J.m, &J.genome[buf][(size_t)slot * J.m * J.L], J.rec[buf][slot]);
 }
 
-void offspring(Job &J, int src, int dst, int c) {
-    const size_t G = (size_t)J.m * J.L;
-    const int parent = J.sorted[c];
-    // copySolution x2, main.cu:169-170
-    std::memcpy(&J.genome[dst][(size_t)c * G], &J.genome[src][(size_t)parent * G], G);
-    J.rec[dst][c] = J.rec[src][parent];
-    int8_t *s = &J.genome[dst][(size_t)(J.N + c) * G];
-    std::memcpy(s, &J.genome[src][(size_t)parent * G], G);
-    const EvalOut &pr = J.rec[src][parent];
-    MutCtx mc{&J.T, J.aa.data(), J.L, J.m, J.seed, J.gen, (uint32_t)c, bern_threshold(J.Pm),
-              bern_threshold(J.Pm * 2.0f)};
+// The operator of child `child` in generation `gen` (main.cu:172-211) applied
+// to s, a copy of its parent's genome, given the parent's record.
+void mutate_child(const Tables &T, const int8_t *aa, int L, int m, uint64_t seed, float Pm, uint32_t gen,
+                  uint32_t child, const EvalOut &pr, int8_t *s) {
+    MutCtx mc{&T, aa, L, m, seed, gen, child, bern_threshold(Pm), bern_threshold(Pm * 2.0f)};
     uint32_t r[4];
-    philox(J.gen, (uint32_t)c, 0u, TAG_TYPE, J.seed, r);
+    philox(gen, child, 0u, TAG_TYPE, seed, r);
     const int type = (int)uniform_int(r[0], (uint32_t)(NOBJ + 1));  // main.cu:172-178
     switch (type) {  // main.cu:181-211
     case 0: mut_random(mc, s); break;
@@ -1007,6 +1001,17 @@ void offspring(Job &J, int src, int dst, int c) {
         break;
     case 6: mut_sl(mc, s, pr); break;
     }
+}
+
+void offspring(Job &J, int src, int dst, int c) {
+    const size_t G = (size_t)J.m * J.L;
+    const int parent = J.sorted[c];
+    // copySolution x2, main.cu:169-170
+    std::memcpy(&J.genome[dst][(size_t)c * G], &J.genome[src][(size_t)parent * G], G);
+    J.rec[dst][c] = J.rec[src][parent];
+    int8_t *s = &J.genome[dst][(size_t)(J.N + c) * G];
+    std::memcpy(s, &J.genome[src][(size_t)parent * G], G);
+    mutate_child(J.T, J.aa.data(), J.L, J.m, J.seed, J.Pm, J.gen, (uint32_t)c, J.rec[src][parent], s);
     eval_into(J, dst, J.N + c);
 }
 
@@ -1119,6 +1124,21 @@ int or_cai_cpb_ref(int L, int m, const int8_t *aa, const char *codon_nt, const f
             out64[o] = std::exp(lsum / L);
             out64[o + 1] = psum / (double)(L - 2);
         }
+    return 0;
+}
+
+int or_mutate(int L, int m, const int8_t *aa, const char *codon_nt, const float *w, const float *cps,
+              float gc3_ref, float Pm, uint64_t seed, uint32_t gen, uint32_t child, const int8_t *parent,
+              const float *pobj, const int8_t *pidx, const int32_t *ppql, int8_t *out) {
+    Tables T;
+    if (!T.build(codon_nt, w, cps, gc3_ref)) return -1;
+    if (L < 3 || m < 2) return -1;
+    EvalOut pr;
+    std::memcpy(pr.obj, pobj, sizeof(pr.obj));
+    std::memcpy(pr.idx, pidx, sizeof(pr.idx));
+    std::memcpy(pr.pql, ppql, sizeof(pr.pql));
+    std::memcpy(out, parent, (size_t)m * L);
+    mutate_child(T, aa, L, m, seed, Pm, gen, child, pr, out);
     return 0;
 }
 

@@ -48,6 +48,14 @@ int or_cai_cpb_ref(int L, int m, const int8_t *aa, const char *codon_nt, const f
                    const float *cps, const int8_t *syn, int count, int threads,
                    float *out32, double *out64);
 
+/* The mutation of one child (mutationKernel's operator choice and operators,
+ * main.cu:172-211 with include/mutation.cuh): child `child` of generation
+ * `gen` from its parent's genome [m][L] and record (obj[6], idx[12], pql[3]);
+ * out [m][L] receives the mutated genome.                                   */
+int or_mutate(int L, int m, const int8_t *aa, const char *codon_nt, const float *w, const float *cps,
+              float gc3_ref, float Pm, uint64_t seed, uint32_t gen, uint32_t child, const int8_t *parent,
+              const float *pobj, const int8_t *pidx, const int32_t *ppql, int8_t *out);
+
 /* ---- selection ---------------------------------------------------------- */
 /* Non-dominated sorting (sorting.cuh:974-1089) of P objective vectors, peeled
  * until at least N are ranked.  front_of[P] = front index or -1.            */

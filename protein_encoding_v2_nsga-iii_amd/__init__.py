@@ -13,6 +13,7 @@ import subprocess
 import numpy as np
 
 from . import tables  # noqa: F401
+from . import workloads  # noqa: F401
 from .tables import Tables
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -184,8 +185,10 @@ class Job:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().cd_job_destroy(self.h)
-            self.h = None
+            # CD_E_STATE: a live batch still holds the job (only possible at
+            # interpreter exit, since a Batch keeps its jobs referenced)
+            if lib().cd_job_destroy(self.h) == CD_OK:
+                self.h = None
 
     def __del__(self):
         self.close()

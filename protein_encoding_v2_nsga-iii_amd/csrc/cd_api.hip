@@ -435,6 +435,15 @@ Launch job_launch(cd_job *J) {
                   J->d_sctl, J->N * J->m, 0, 0};
 }
 
+// Wait for the job's own stream and for the streams of the batches that hold it.
+hipError_t sync_job(cd_job *J) {
+    for (cd_batch *B : J->batches) {
+        const hipError_t e = hipStreamSynchronize(B->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(J->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -468,6 +477,7 @@ int cd_job_init(cd_job *J) {
 int cd_job_step(cd_job *J, int32_t generations) {
     if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
     if (!J->inited) return fail(CD_E_STATE, "cd_job_step before cd_job_init");
+    if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: advance it through the batch");
     if (generations < 0) return fail(CD_E_INVALID, "negative generation count");
     if (!J->gexec) {
         int rc = capture(J->stream, &J->graph, &J->gexec, job_launch(J));
@@ -497,13 +507,13 @@ int cd_job_step_timed(cd_job *J, int32_t generations, float *ms) {
 
 int cd_job_sync(cd_job *J) {
     if (!J) return fail(CD_E_INVALID, "null job");
-    HIPCHK(hipStreamSynchronize(J->stream));
+    HIPCHK(sync_job(J));
     return CD_OK;
 }
 
 int cd_job_generation(cd_job *J, int32_t *generation) {
     if (!J || !generation) return fail(CD_E_INVALID, "null argument");
-    HIPCHK(hipStreamSynchronize(J->stream));
+    HIPCHK(sync_job(J));
     cd::SelState st;
     HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
     *generation = (int32_t)st.gen;
@@ -514,7 +524,7 @@ int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql
                 float *state) {
     if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
     if (!J->inited) return fail(CD_E_STATE, "cd_job_read before cd_job_init");
-    HIPCHK(hipStreamSynchronize(J->stream));
+    HIPCHK(sync_job(J));
     cd::SelState st;
     HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
     const int b = (int)(st.gen & 1u), P = J->P;
@@ -541,6 +551,7 @@ int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql
 int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
     if (!J || J->selection_only || !kernel_ms) return fail(CD_E_INVALID, "null argument");
     if (!J->inited) return fail(CD_E_STATE, "cd_job_profile before cd_job_init");
+    if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: advance it through the batch");
     const int NK = CD_PROFILE_KERNELS;
     hipEvent_t ev[NK + 1];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
@@ -576,7 +587,7 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
 
 int cd_job_cds_scanned(cd_job *J, int64_t *count) {
     if (!J || !count) return fail(CD_E_INVALID, "null argument");
-    HIPCHK(hipStreamSynchronize(J->stream));
+    HIPCHK(sync_job(J));
     cd::SelState st;
     HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
     *count = (int64_t)st.cds_scanned;
@@ -585,6 +596,7 @@ int cd_job_cds_scanned(cd_job *J, int64_t *count) {
 
 int cd_job_destroy(cd_job *J) {
     if (!J) return CD_OK;
+    if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: destroy the batch first");
     if (J->stream) (void)hipStreamSynchronize(J->stream);
     if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
     if (J->graph) (void)hipGraphDestroy(J->graph);
@@ -743,6 +755,7 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
         cd_batch_destroy(B);
         return fail(CD_E_DEVICE, std::string("batch setup: ") + hipGetErrorString(e));
     }
+    for (cd_job *J : B->jobs) J->batches.push_back(B);
     *out = B;
     return CD_OK;
 }
@@ -788,6 +801,7 @@ int cd_batch_step_timed(cd_batch *B, int32_t generations, float *ms) {
 int cd_batch_destroy(cd_batch *B) {
     if (!B) return CD_OK;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
+    for (cd_job *J : B->jobs) J->batches.erase(std::remove(J->batches.begin(), J->batches.end(), B), J->batches.end());
     if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     if (B->stream) (void)hipStreamDestroy(B->stream);

@@ -11,8 +11,14 @@
 #include "cd_kernels.h"
 #include "codon_nsga3.h"
 
+struct cd_batch;
+
 // One job: tables, pools and selection state in one device allocation.
 struct cd_job {
+    // live batches that hold the job: while any does, the job is advanced
+    // only through them (cd_job_step and cd_job_destroy fail) and reads wait
+    // for their streams
+    std::vector<cd_batch *> batches;
     cd::DevJob h{};
     cd::DevJob *d_self = nullptr;
     int L = 0, m = 0, N = 0, H = 0, R = 0, P = 0, W = 0;

@@ -17,6 +17,7 @@
 // Work lists map blockIdx.x to (job, item) so the same kernels advance one
 // job or a batch of uneven jobs.
 #include <cfloat>
+#include <mutex>
 
 #include "cd_device.h"
 #include "cd_kernels.h"
@@ -1950,8 +1951,27 @@ hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStr
     return hipGetLastError();
 }
 size_t niche_lds_bytes(int P, int R) { return niche_lds_need(P, R); }
+
+// The dynamic-LDS limit of a kernel is an attribute of the process, shared by
+// every job, batch and captured graph that launches it, so it only ever
+// rises: a job created after a larger one must not lower the limit under the
+// larger one's launches.  `held` is the limit set so far for that group of
+// kernels; `apply` sets the attribute on each of them.
+template <class F>
+static hipError_t raise_lds_limit(size_t &held, size_t bytes, F apply) {
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (bytes <= held) return hipSuccess;
+    const hipError_t e = apply((int)bytes);
+    if (e == hipSuccess) held = bytes;
+    return e;
+}
+static size_t g_design_lds = 0, g_stem_lds = 0, g_niche_lds = 0;
+
 hipError_t set_niche_lds_limit(size_t bytes) {
-    return hipFuncSetAttribute((const void *)k_niche, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return raise_lds_limit(g_niche_lds, bytes, [](int b) {
+        return hipFuncSetAttribute((const void *)k_niche, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    });
 }
 hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(1024), lds, s, jobs);
@@ -1967,20 +1987,23 @@ hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s) {
 }
 
 hipError_t set_design_lds_limit(size_t bytes) {
-    hipError_t e = hipSuccess;
-    e = hipFuncSetAttribute((const void *)k_design<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_design<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_design<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return raise_lds_limit(g_design_lds, bytes, [](int b) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_design<0>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)k_design<1>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        if (e != hipSuccess) return e;
+        return hipFuncSetAttribute((const void *)k_design<2>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    });
 }
 hipError_t set_stem_lds_limit(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES_BATCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes);
+    return raise_lds_limit(g_stem_lds, bytes, [](int b) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        if (e != hipSuccess) return e;
+        return hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES_BATCH>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    });
 }
 
 }  // namespace cd

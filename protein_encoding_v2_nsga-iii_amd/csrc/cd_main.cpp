@@ -105,11 +105,13 @@ int main(int argc, char **argv) {
     }
     aa.push_back(20);
     const int L = (int)aa.size();
-    if (H <= 0) {  // main.cu:564-577: the largest p with C(5+p, p) <= N, at least 1
+    if (H <= 0) {  // main.cu:564-577: the largest p with C(5+p, p) <= N; at least 1 (the
+                   // reference's 0 for N < 6 is a single all-zero reference direction)
         int p = 1;
         while (comb(5 + p, p) <= N) ++p;
         H = p - 1 < 1 ? 1 : p - 1;
     }
+    std::printf("Reference point divisions (H) : %d, reference points : %lld\n", H, comb(H + 5, 5));
     std::string codons;
     std::vector<float> w, cps;
     float gc3 = 0.381f;

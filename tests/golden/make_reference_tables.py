@@ -1,7 +1,7 @@
 """Extract the reference's codon, CAI-weight and codon-pair tables as data.
 
 Reads include/info.cuh of the reference checkout as text (never compiles or
-imports it) and writes reference_tables.json next to this script:
+imports it) and writes the package's data/reference_tables.json:
   codons   -- 64 codons, 3 letters each, in the reference's order
               (info.cuh:11-31: amino-acid blocks A C D E F G H I K L M N P Q R
               S T V W Y Z, synonyms by ascending weight, stops last)
@@ -19,7 +19,8 @@ import sys
 import numpy as np
 
 SRC = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/include/info.cuh"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_tables.json")
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+OUT = os.path.join(ROOT, "protein_encoding_v2_nsga-iii_amd", "data", "reference_tables.json")
 
 
 def main():

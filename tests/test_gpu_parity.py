@@ -50,7 +50,11 @@ def test_nds_bit_exact(cdp, orc, P, N):
     assert nf_g == nf_o and np.array_equal(fo_g, fo_o)
 
 
-@pytest.mark.parametrize("N,H,seed", [(8, 1, 1), (50, 2, 2), (128, 4, 3), (1024, 5, 4), (300, 3, 5)])
+# k_niche keeps its per-reference-point arrays and the last front in LDS when
+# they fit (P <= 2048 here), in LDS without the copies of the pool arrays at
+# P = 8192 with 252 points, and in global memory at P = 8192 with 2002 points
+@pytest.mark.parametrize("N,H,seed", [(8, 1, 1), (50, 2, 2), (128, 4, 3), (1024, 5, 4), (300, 3, 5), (4096, 5, 6),
+                                      (4096, 9, 7)])
 def test_select_bit_exact(cdp, orc, N, H, seed):
     _say("select", N, H, seed)
     rng = np.random.default_rng(seed)

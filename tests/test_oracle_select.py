@@ -3,9 +3,12 @@
 Fronts are checked against repeated extraction of the eps-non-dominated set,
 the hyperplane solve against numpy.linalg.solve, extremes against a direct
 ASF argmin, association against FP64 perpendicular distances, and niching
-against the NSGA-III balance property.
+against the NSGA-III balance property and the level-0 rule (a reference line
+with no member in the earlier fronts takes its nearest candidate first).  The
+level-0 check is also run against an oracle built with a planted mistake.
 """
 import numpy as np
+import pytest
 
 EPS = np.float32(1e-6)
 
@@ -160,3 +163,63 @@ def test_niche_balance(orc):
         if picked and leftover:
             assert max(final[a] for a in picked) <= min(final[a] for a in leftover) + 1
     assert seen_niching > 5
+
+
+def level0_check(orc, trials=40):
+    """referenceBasedSorting's level 0 (sorting.cuh:1191-1245): a reference
+    point with no associated design in the fronts before the last takes, as
+    its first pick from the last front, the candidate nearest to its line.
+    So whenever any candidate of such a point survives, its nearest one
+    does.  Association and distances in float64; cases where float32 could
+    order two lines or two candidates differently are skipped."""
+    rng = np.random.default_rng(11)
+    checked = 0
+    for trial in range(trials):
+        N = int(rng.integers(20, 120))
+        H = int(rng.integers(2, 5))
+        obj = rng.normal(size=(2 * N, 6)).astype(np.float32)
+        obj[:, 3] = -obj[:, 0] - obj[:, 1]  # a large first front: most lines start at count 0
+        r = orc.select(obj, N, H, trial, 5, np.zeros((6, 6), np.float32))
+        if r["n_cut"]:
+            continue
+        fo, srt = r["front_of"], set(r["sorted"].tolist())
+        last = fo.max()
+        ideal, nadir = r["ideal"].astype(np.float64), r["nadir"].astype(np.float64)
+        den = nadir - ideal
+        refs = orc.dasdennis(6, H).astype(np.float64)
+        ranked = np.nonzero(fo >= 0)[0]
+        f = np.where(den == 0, 0, (obj[ranked] - ideal) / np.where(den == 0, 1, den))
+        k = (f @ refs.T) / (refs * refs).sum(axis=1)[None, :]
+        d = np.sqrt(((k[:, :, None] * refs[None] - f[:, None, :]) ** 2).sum(axis=2))
+        order = np.sort(d, axis=1)
+        if len(refs) > 1 and (order[:, 1] - order[:, 0] < 1e-5).any():
+            continue
+        line = dict(zip(ranked.tolist(), np.argmin(d, axis=1).tolist()))
+        dist = dict(zip(ranked.tolist(), order[:, 0].tolist()))
+        count = np.zeros(len(refs), int)
+        for i, a in line.items():
+            if fo[i] < last:
+                count[a] += 1
+        for a in range(len(refs)):
+            cand = [i for i, b in line.items() if b == a and fo[i] == last]
+            if count[a] or len(cand) < 2 or not any(i in srt for i in cand):
+                continue
+            ds = sorted(dist[i] for i in cand)
+            if ds[1] - ds[0] < 1e-5 * max(1.0, ds[0]):
+                continue
+            nearest = min(cand, key=lambda i: dist[i])
+            assert nearest in srt, (trial, a, nearest, sorted(cand))
+            checked += 1
+    assert checked >= 50, checked
+
+
+def test_niche_level0_takes_the_nearest(orc):
+    level0_check(orc)
+
+
+def test_niche_level0_check_catches_planted_mistake(orc, tmp_path):
+    path = str(tmp_path / "libcd_oracle_level0.so")
+    assert orc.build_variant(path, [("void select_pool(", "if (level == 0)", "if (level < 0)")]).wait() == 0
+    with orc.use_library(path):
+        with pytest.raises(AssertionError):
+            level0_check(orc)
