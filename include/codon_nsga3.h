@@ -131,6 +131,18 @@ int cd_eval_batch(const cd_job_desc *desc, const int8_t *genome, int32_t count,
  * at least N are ranked: front_of [P] (-1 for unranked), *n_fronts. */
 int cd_nds(const float *obj, int32_t P, int32_t N, int32_t *front_of, int32_t *n_fronts);
 
+/* No counterpart in the reference: cd_nds for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  The
+ * same sort over device buffers -- obj [P][6] in, front_of [P] and
+ * n_fronts [1] out -- enqueued on `stream` (a hipStream_t; NULL for the null
+ * stream) without waiting.  It calls no allocator: its scratch is a
+ * fixed-size device array (about 8.3 MiB) that every GPU the library runs on
+ * holds until the process ends, so P is at most CD_NOALLOC_MAX_POOL, and
+ * calls that overlap on one GPU (other streams or threads) race on it;
+ * calls in one stream are ordered. */
+#define CD_NOALLOC_MAX_POOL 8192
+int cd_nds_noalloc(const float *obj, int32_t P, int32_t N, int32_t *front_of, int32_t *n_fronts, void *stream);
+
 /* One NSGA-III selection (sortingKernel, main.cu:409-454) of N survivors
  * from 2N objective vectors.  extremes [36] is read and updated;
  * sorted [N]; front_of [2N]; norm [18] = ideal, worst, nadir; flags [2] =

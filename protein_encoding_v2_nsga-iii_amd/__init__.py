@@ -32,7 +32,7 @@ SYMBOLS = ("cd_job_create", "cd_job_init", "cd_job_step", "cd_job_step_timed", "
            "cd_batch_create", "cd_batch_step", "cd_batch_step_timed", "cd_batch_sync", "cd_batch_destroy",
            "cd_job_device_bytes", "cd_last_error", "cd_hypervolume", "cd_normalize", "cd_min_distance_spread",
            "cd_server_create", "cd_server_submit", "cd_server_wait", "cd_server_result", "cd_server_stats",
-           "cd_server_destroy")
+           "cd_server_destroy", "cd_nds_noalloc")
 SERVER_QUEUED, SERVER_RUNNING, SERVER_DONE, SERVER_FAILED = 0, 1, 2, 3
 
 
@@ -86,6 +86,7 @@ def lib():
         L.cd_job_destroy.argtypes = [vp]
         L.cd_eval_batch.argtypes = [C.POINTER(_Desc), i8, C.c_int32, f32, i8, i32]
         L.cd_nds.argtypes = [f32, C.c_int32, C.c_int32, i32, C.POINTER(C.c_int32)]
+        L.cd_nds_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp]
         L.cd_select.argtypes = [f32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint64, f32, i32, i32, f32, i32]
         L.cd_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
         L.cd_batch_step.argtypes = [vp, C.c_int32]
