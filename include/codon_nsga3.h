@@ -160,7 +160,38 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out);
 int cd_batch_step(cd_batch *batch, int32_t generations);
 int cd_batch_step_timed(cd_batch *batch, int32_t generations, float *ms);
 int cd_batch_sync(cd_batch *batch);
+/* As cd_job_profile, for one generation of every job of the batch: the two
+ * launch classes of the offspring kernel are timed together as kernel_ms[0]. */
+int cd_batch_profile(cd_batch *batch, int32_t generations, float *kernel_ms);
 int cd_batch_destroy(cd_batch *batch);
+
+/* Launch shapes of a generation's kernels (no counterpart in the reference,
+ * which sizes its cooperative launches from occupancy queries,
+ * main.cu:603-613).  Every shape gives the same results; only the time
+ * differs. */
+typedef struct cd_shapes {
+    int32_t design_split;   /* k_design LDS bytes up to which children launch apart (0: one launch) */
+    int32_t stem_waves;     /* k_stem_list workgroups per CU: 5 to 8 */
+    int32_t dom_rows;       /* k_dominance rows per workgroup: 64, 128 or 256 */
+    int32_t fronts_threads; /* k_fronts workgroup size: 256, 512 or 1024 */
+    int32_t assoc_designs;  /* k_associate designs per workgroup: 4, 8 or 16 */
+    int32_t niche_threads;  /* k_niche workgroup size: 256, 512 or 1024 */
+} cd_shapes;
+
+/* Autotuner: a coordinate search over the shapes, each candidate timed
+ * (HIP events around each kernel) over `gens_per_candidate` ordinary
+ * generations of the job or batch -- which therefore advances by
+ * *generations_run generations, with exactly the results it would have had
+ * in any shape.  The chosen shapes are installed and cached for the device
+ * under a coarse signature of the work (job count, children, codons per
+ * child, pool size, in power-of-two buckets); jobs, batches and the
+ * server's ticks created later with the same signature start from them. */
+int cd_job_tune(cd_job *job, int32_t gens_per_candidate, cd_shapes *chosen, int32_t *generations_run);
+int cd_batch_tune(cd_batch *batch, int32_t gens_per_candidate, cd_shapes *chosen, int32_t *generations_run);
+/* Read (get) and/or install (set) the shapes of a job or batch; either
+ * pointer may be NULL. */
+int cd_job_shapes(cd_job *job, cd_shapes *get, const cd_shapes *set);
+int cd_batch_shapes(cd_batch *batch, cd_shapes *get, const cd_shapes *set);
 
 /* Served form (no counterpart in the reference, which runs one job per
  * process): continuous batching of jobs of different protein lengths,

@@ -73,6 +73,16 @@ def build_variant(out_path, patches):
                              "-o", out_path, cpp])
 
 
+def build_native(out_path):
+    """The oracle compiled for the host it runs on (-O3 -march=native, no
+    fast-math; the same results): the CPU baseline's build.  Returns the
+    library path, or None when the compiler fails."""
+    r = subprocess.run(["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                        "-shared", "-I", HERE, "-o", out_path, os.path.join(HERE, "cd_oracle.cpp")],
+                       capture_output=True, text=True)
+    return out_path if r.returncode == 0 else None
+
+
 def _bind(L):
     L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
     L.or_det_exp.argtypes = [C.c_double]

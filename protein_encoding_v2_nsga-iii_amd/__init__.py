@@ -29,10 +29,12 @@ ERRORS = {-1: "CD_E_INVALID", -2: "CD_E_NOMEM", -3: "CD_E_DEVICE", -4: "CD_E_STA
 # Every entry point of include/codon_nsga3.h
 SYMBOLS = ("cd_job_create", "cd_job_init", "cd_job_step", "cd_job_step_timed", "cd_job_sync", "cd_job_generation",
            "cd_job_read", "cd_job_profile", "cd_job_cds_scanned", "cd_job_destroy", "cd_eval_batch", "cd_nds", "cd_select",
-           "cd_batch_create", "cd_batch_step", "cd_batch_step_timed", "cd_batch_sync", "cd_batch_destroy",
+           "cd_batch_create", "cd_batch_step", "cd_batch_step_timed", "cd_batch_sync", "cd_batch_profile",
+           "cd_batch_destroy",
            "cd_job_device_bytes", "cd_last_error", "cd_hypervolume", "cd_normalize", "cd_min_distance_spread",
            "cd_server_create", "cd_server_submit", "cd_server_wait", "cd_server_result", "cd_server_stats",
-           "cd_server_destroy", "cd_nds_noalloc")
+           "cd_server_destroy", "cd_nds_noalloc", "cd_job_tune", "cd_batch_tune", "cd_job_shapes",
+           "cd_batch_shapes")
 SERVER_QUEUED, SERVER_RUNNING, SERVER_DONE, SERVER_FAILED = 0, 1, 2, 3
 
 
@@ -45,6 +47,22 @@ class CdError(RuntimeError):
 class _Tables(C.Structure):
     _fields_ = [("codons", C.c_char_p), ("weights", C.POINTER(C.c_float)), ("cps", C.POINTER(C.c_float)),
                 ("gc3_ref", C.c_float)]
+
+
+# launch shapes (cd_shapes): every shape gives the same results
+SHAPE_FIELDS = ("design_split", "stem_waves", "dom_rows", "fronts_threads", "assoc_designs", "niche_threads")
+
+
+class _Shapes(C.Structure):
+    _fields_ = [(f, C.c_int32) for f in SHAPE_FIELDS]
+
+
+def _shapes_dict(s):
+    return {f: int(getattr(s, f)) for f in SHAPE_FIELDS}
+
+
+def _shapes_struct(d):
+    return _Shapes(*[int(d[f]) for f in SHAPE_FIELDS])
 
 
 class _Desc(C.Structure):
@@ -87,11 +105,17 @@ def lib():
         L.cd_eval_batch.argtypes = [C.POINTER(_Desc), i8, C.c_int32, f32, i8, i32]
         L.cd_nds.argtypes = [f32, C.c_int32, C.c_int32, i32, C.POINTER(C.c_int32)]
         L.cd_nds_noalloc.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp]
+        SP = C.POINTER(_Shapes)
+        L.cd_job_tune.argtypes = [vp, C.c_int32, SP, C.POINTER(C.c_int32)]
+        L.cd_batch_tune.argtypes = [vp, C.c_int32, SP, C.POINTER(C.c_int32)]
+        L.cd_job_shapes.argtypes = [vp, SP, SP]
+        L.cd_batch_shapes.argtypes = [vp, SP, SP]
         L.cd_select.argtypes = [f32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint64, f32, i32, i32, f32, i32]
         L.cd_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
         L.cd_batch_step.argtypes = [vp, C.c_int32]
         L.cd_batch_step_timed.argtypes = [vp, C.c_int32, C.POINTER(C.c_float)]
         L.cd_batch_sync.argtypes = [vp]
+        L.cd_batch_profile.argtypes = [vp, C.c_int32, f32]
         L.cd_batch_destroy.argtypes = [vp]
         L.cd_job_device_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.cd_job_device_bytes.restype = C.c_int64
@@ -166,6 +190,20 @@ class Job:
         _check(lib().cd_job_profile(self.h, generations, out))
         return out
 
+    def tune(self, gens_per_candidate=3):
+        """Autotune the launch shapes on this job's own generations (cd_job_tune);
+        returns (chosen shapes, generations run)."""
+        sh, n = _Shapes(), C.c_int32()
+        _check(lib().cd_job_tune(self.h, gens_per_candidate, C.byref(sh), C.byref(n)))
+        return _shapes_dict(sh), n.value
+
+    def shapes(self, set=None):
+        """The job's launch shapes; with `set` (a dict), install those first."""
+        sh = _Shapes()
+        st = C.byref(_shapes_struct(set)) if set is not None else None
+        _check(lib().cd_job_shapes(self.h, C.byref(sh), st))
+        return _shapes_dict(sh)
+
     def cds_scanned(self):
         n = C.c_int64()
         _check(lib().cd_job_cds_scanned(self.h, C.byref(n)))
@@ -215,6 +253,26 @@ class Batch:
 
     def sync(self):
         _check(lib().cd_batch_sync(self.h))
+
+    def tune(self, gens_per_candidate=3):
+        """Autotune the launch shapes on the batch's own generations (cd_batch_tune);
+        returns (chosen shapes, generations run)."""
+        sh, n = _Shapes(), C.c_int32()
+        _check(lib().cd_batch_tune(self.h, gens_per_candidate, C.byref(sh), C.byref(n)))
+        return _shapes_dict(sh), n.value
+
+    def shapes(self, set=None):
+        """The batch's launch shapes; with `set` (a dict), install those first."""
+        sh = _Shapes()
+        st = C.byref(_shapes_struct(set)) if set is not None else None
+        _check(lib().cd_batch_shapes(self.h, C.byref(sh), st))
+        return _shapes_dict(sh)
+
+    def profile(self, generations):
+        """Mean milliseconds of each phase of a batched generation (cd_batch_profile)."""
+        out = np.zeros(PROFILE_KERNELS, np.float32)
+        _check(lib().cd_batch_profile(self.h, generations, out))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -189,9 +191,20 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
     return o;
 }
 
+// Room for the work lists of a job of pool P in any shape (the smallest
+// tile and group give the most items).
 void work_counts(int P, int W, int *n_dom, int *n_asc) {
     *n_dom = ((P + cd::DOM_ROWS - 1) / cd::DOM_ROWS) * W;
     *n_asc = (P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS;
+}
+
+// Domination tiles and association groups of one job (job index j) in the
+// given shapes.
+void job_lists(int P, int W, const cd::Shapes &sh, std::vector<int4> &dom, std::vector<int2> &asc, int j = 0) {
+    for (int t = 0; t < (P + sh.dom_rows - 1) / sh.dom_rows; ++t)
+        for (int w = 0; w < W; ++w) dom.push_back(make_int4(j, t * sh.dom_rows, w, 0));
+    for (int t = 0; t < (P + sh.assoc_designs - 1) / sh.assoc_designs; ++t)
+        asc.push_back(make_int2(j, t * sh.assoc_designs));
 }
 
 int validate(const cd_job_desc *d) {
@@ -304,12 +317,22 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.st = (cd::SelState *)(base + o.st);
     std::memcpy(&host[o.self], &h, sizeof(h));
     J->d_self = (cd::DevJob *)(base + o.self);
-    int4 *dom = (int4 *)&host[o.domi];
-    int q = 0;
-    for (int t = 0; t < (J->P + cd::DOM_ROWS - 1) / cd::DOM_ROWS; ++t)
-        for (int w = 0; w < J->W; ++w) dom[q++] = make_int4(0, t * cd::DOM_ROWS, w, 0);
-    int2 *asc = (int2 *)&host[o.asc];
-    for (int t = 0; t < J->n_asc; ++t) asc[t] = make_int2(0, t * cd::ASSOC_DESIGNS);
+    if (!sel_only) {  // the tuned shapes of this kind of job, if any
+        cd_job *self = J;
+        cd::Shapes sh;
+        if (cdhost::cached_shapes(cdhost::work_signature(&self, 1), &sh)) J->shapes = sh;
+    } else {
+        J->shapes.dom_rows = cd::DOM_ROWS;
+    }
+    {
+        std::vector<int4> dom;
+        std::vector<int2> asc;
+        job_lists(J->P, J->W, J->shapes, dom, asc);
+        std::memcpy(&host[o.domi], dom.data(), dom.size() * sizeof(int4));
+        std::memcpy(&host[o.asc], asc.data(), asc.size() * sizeof(int2));
+        J->n_dom = (int)dom.size();
+        J->n_asc = (int)asc.size();
+    }
     J->d_dom = (int4 *)(base + o.domi);
     J->d_asc = (int2 *)(base + o.asc);
     if (e == hipSuccess) e = hipMemcpy(J->mem, host.data(), o.total, hipMemcpyHostToDevice);
@@ -329,18 +352,9 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
 
 // Evaluate the designs of `items`: mutation (children) and objectives 0-4,
 // the stem of every changed CDS, objective 5.
+// Initialisation and evaluation of the designs of `items` (offspring
+// generations go through enqueue_generation).
 hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_items, hipStream_t s) {
-    if (mode == cd::MODE_OFFSPRING) {  // items is the children list; k_dominance finishes the stem objective
-        hipError_t e = hipSuccess;
-        const int ns = w.n_small > 0 && w.n_small < n_items ? w.n_small : 0;
-        if (ns) e = cd::launch_design(mode, w.jobs, items, ns, w.lds_small, s, w.stem_list, w.stem_ctl, w.n_cds, 0);
-        if (e == hipSuccess)
-            e = cd::launch_design(mode, w.jobs, items + ns, n_items - ns, w.lds, s, w.stem_list, w.stem_ctl, w.n_cds,
-                                  ns);
-        if (e == hipSuccess) e = cd::launch_stem_list(w.jobs, items, w.stem_list, w.stem_ctl, w.n_cds,
-                                                      w.njobs > 1 ? cd::STEM_WAVES_BATCH : cd::STEM_WAVES, w.stem_lds, s);
-        return e;
-    }
     hipError_t e = cd::launch_design(mode, w.jobs, items, n_items, w.lds, s);
     if (e == hipSuccess) e = cd::launch_stem(mode, w.jobs, items, n_items, w.mg, w.stem_parts, w.stem_lds, s);
     if (e == hipSuccess) e = cd::launch_stem_finish(mode, w.jobs, items, n_items, s);
@@ -351,27 +365,43 @@ hipError_t enqueue_designs(const Launch &w, int mode, const int2 *items, int n_i
 
 namespace cdhost {
 
-hipError_t enqueue_generation(const Launch &w, hipStream_t s) {
-    hipError_t e = enqueue_designs(w, cd::MODE_OFFSPRING, w.off, w.n_off, s);
-    const cd::DevJob *jobs = w.jobs;
-    const int njobs = w.njobs;
-    const int4 *dom = w.dom;
-    const int n_dom = w.n_dom;
-    const int2 *asc = w.asc;
-    const int n_asc = w.n_asc;
-    if (e == hipSuccess) e = cd::launch_dominance(jobs, dom, n_dom, true, w.stem_ctl, w.dom_rows, s);
-    if (e == hipSuccess) e = cd::launch_fronts(jobs, njobs, s);
-    if (e == hipSuccess) e = cd::launch_associate(jobs, asc, n_asc, s);
-    if (e == hipSuccess) e = cd::launch_niche(jobs, njobs, w.niche_lds, s);
+hipError_t enqueue_generation(const Launch &w, hipStream_t s, hipEvent_t *ev) {
+    // ev (optional): CD_PROFILE_KERNELS + 1 events recorded around the six
+    // phases -- offspring (both k_design launch classes), stem list,
+    // dominance, fronts, association, niching
+    auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s) : hipSuccess; };
+    hipError_t e = mark(0);
+    if (e == hipSuccess) {
+        const int ns = w.n_small > 0 && w.n_small < w.n_off ? w.n_small : 0;
+        if (ns)
+            e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off, ns, w.lds_small, s, w.stem_list, w.stem_ctl,
+                                  w.n_cds, 0);
+        if (e == hipSuccess)
+            e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off + ns, w.n_off - ns, w.lds, s, w.stem_list,
+                                  w.stem_ctl, w.n_cds, ns);
+    }
+    if (e == hipSuccess) e = mark(1);
+    if (e == hipSuccess)
+        e = cd::launch_stem_list(w.jobs, w.off, w.stem_list, w.stem_ctl, w.n_cds, w.sh.stem_waves, w.stem_lds, s);
+    if (e == hipSuccess) e = mark(2);
+    if (e == hipSuccess) e = cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, w.stem_ctl, w.sh.dom_rows, s);
+    if (e == hipSuccess) e = mark(3);
+    if (e == hipSuccess) e = cd::launch_fronts(w.jobs, w.njobs, w.sh.fronts_threads, s);
+    if (e == hipSuccess) e = mark(4);
+    if (e == hipSuccess) e = cd::launch_associate(w.jobs, w.asc, w.n_asc, w.sh.assoc_designs, s);
+    if (e == hipSuccess) e = mark(5);
+    if (e == hipSuccess) e = cd::launch_niche(w.jobs, w.njobs, w.niche_lds, w.sh.niche_threads, s);
+    if (e == hipSuccess) e = mark(6);
     return e;
 }
 
-void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
+void build_lists(cd_job *const *jobs, int count, const cd::Shapes &sh, WorkLists &wl) {
     wl = WorkLists();
-    // children of the jobs whose k_design fits six workgroups per CU first;
-    // within each class the jobs with the most codons per child (L * m)
-    // first, so that the longest workgroups start early and short ones fill
-    // the end of the launch
+    // children of the jobs whose k_design LDS is at most sh.design_split
+    // first (a launch of its own, so that more of them share a CU); within
+    // each class the jobs with the most codons per child (L * m) first, so
+    // that the longest workgroups start early and short ones fill the end of
+    // the launch
     std::vector<int> order(count);
     for (int j = 0; j < count; ++j) order[j] = j;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -380,7 +410,7 @@ void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
     for (int pass = 0; pass < 2; ++pass)
         for (int j : order) {
             cd_job *J = jobs[j];
-            if ((J->lds <= cd::DESIGN_LDS_SIX) != (pass == 0)) continue;
+            if ((J->lds <= (size_t)sh.design_split) != (pass == 0)) continue;
             for (int c = 0; c < J->N; ++c) wl.off.push_back(make_int2(j, c));
             if (pass == 0) {
                 wl.n_small += J->N;
@@ -391,10 +421,7 @@ void build_lists(cd_job *const *jobs, int count, WorkLists &wl) {
         cd_job *J = jobs[j];
         wl.descs.push_back(J->h);
         wl.n_cds += J->N * J->m;
-        for (int t = 0; t < (J->P + cd::DOM_ROWS_BATCH - 1) / cd::DOM_ROWS_BATCH; ++t)
-            for (int w = 0; w < J->W; ++w) wl.dom.push_back(make_int4(j, t * cd::DOM_ROWS_BATCH, w, 0));
-        for (int t = 0; t < (J->P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS; ++t)
-            wl.asc.push_back(make_int2(j, t * cd::ASSOC_DESIGNS));
+        job_lists(J->P, J->W, sh, wl.dom, wl.asc, j);
         wl.lds = std::max(wl.lds, J->lds);
         wl.stem_lds = std::max(wl.stem_lds, J->stem_lds);
         wl.niche_lds = std::max(wl.niche_lds, J->niche_lds);
@@ -419,7 +446,7 @@ namespace {
 
 int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_generation(w, s);
+    hipError_t e = enqueue_generation(w, s, nullptr);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &graph);
     if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
@@ -430,9 +457,9 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 }
 
 Launch job_launch(cd_job *J) {
-    return Launch{J->d_self, 1,      J->d_off,     J->N,         J->d_dom,   J->n_dom,   cd::DOM_ROWS, J->d_asc,
-                  J->n_asc,  J->lds, J->stem_lds, J->niche_lds, J->m,       J->stem_parts, J->d_slist,
-                  J->d_sctl, J->N * J->m, 0, 0};
+    return Launch{J->d_self, 1,          J->d_off,     J->N,          J->d_dom,   J->n_dom,   J->d_asc,
+                  J->n_asc,  J->lds,     J->stem_lds,  J->niche_lds,  J->m,       J->stem_parts, J->d_slist,
+                  J->d_sctl, J->N * J->m, 0,           0,             J->shapes};
 }
 
 // Wait for the job's own stream and for the streams of the batches that hold it.
@@ -444,7 +471,129 @@ hipError_t sync_job(cd_job *J) {
     return hipStreamSynchronize(J->stream);
 }
 
+// Run `generations` generations of `w` launching every kernel apart, with
+// HIP events on stream s around each phase; kernel_ms receives the mean
+// duration of each phase (CD_PROFILE_KERNELS values).
+int profile_generations(const Launch &w, hipStream_t s, int generations, float *kernel_ms) {
+    constexpr int NK = CD_PROFILE_KERNELS;
+    hipEvent_t ev[NK + 1];
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[NK] = {0};
+    for (int g = 0; g < generations; ++g) {
+        HIPCHK(cdhost::enqueue_generation(w, s, ev));
+        HIPCHK(hipEventSynchronize(ev[NK]));
+        for (int k = 0; k < NK; ++k) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            acc[k] += t;
+        }
+    }
+    for (int k = 0; k < NK; ++k) kernel_ms[k] = generations > 0 ? (float)(acc[k] / generations) : 0.f;
+    for (auto &e : ev) HIPCHK(hipEventDestroy(e));
+    return CD_OK;
+}
+
+// Coordinate search over the launch shapes.  Every candidate runs `gens`
+// ordinary generations of the work (all shapes give the same results, so
+// the job's trajectory does not depend on the search); a candidate's cost is
+// the mean time of the phases its parameter changes, from HIP events around
+// each phase.  Parameters are settled one after another, each from the best
+// shapes so far; `apply` installs shapes and returns the launch description.
+template <class Apply>
+int tune_shapes(const cd::Shapes &start, int njobs, int gens, hipStream_t s, Apply apply, cd::Shapes *best,
+                int *run) {
+    struct Param {
+        int cd::Shapes::*field;
+        std::vector<int> values;
+        unsigned phases;  // bit k: phase k of the generation profile
+    };
+    const std::vector<Param> params = {
+        {&cd::Shapes::design_split, {0, 20 * 1024, 24 * 1024, 29 * 1024}, 0x3u},
+        {&cd::Shapes::stem_waves, {5, 6, 7, 8}, 0x2u},
+        {&cd::Shapes::dom_rows, {64, 128, 256}, 0x4u},
+        {&cd::Shapes::fronts_threads, {256, 512, 1024}, 0x8u},
+        {&cd::Shapes::assoc_designs, {4, 8, 16}, 0x10u},
+        {&cd::Shapes::niche_threads, {256, 512, 1024}, 0x20u},
+    };
+    cd::Shapes cur = start;
+    *run = 0;
+    for (const Param &p : params) {
+        if (p.field == &cd::Shapes::design_split && njobs == 1) continue;  // one job is one launch class
+        double best_cost = 1e30;
+        int best_v = cur.*(p.field);
+        for (int v : p.values) {
+            cd::Shapes cand = cur;
+            cand.*(p.field) = v;
+            Launch w{};
+            int rc = apply(cand, &w);
+            if (rc != CD_OK) return rc;
+            float ms[CD_PROFILE_KERNELS];
+            rc = profile_generations(w, s, gens, ms);
+            if (rc != CD_OK) return rc;
+            *run += gens;
+            double cost = 0.0;
+            for (int k = 0; k < CD_PROFILE_KERNELS; ++k)
+                if (p.phases >> k & 1u) cost += ms[k];
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_v = v;
+            }
+        }
+        cur.*(p.field) = best_v;
+    }
+    Launch w{};
+    const int rc = apply(cur, &w);
+    if (rc != CD_OK) return rc;
+    *best = cur;
+    return CD_OK;
+}
+
 }  // namespace
+
+namespace cdhost {
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, cd::Shapes> g_tuned;
+
+int log2_bucket(int64_t x) {
+    int b = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++b;
+    }
+    return b;
+}
+}  // namespace
+
+std::string work_signature(cd_job *const *jobs, int count) {
+    int64_t children = 0, lm = 0, pool = 0;
+    for (int j = 0; j < count; ++j) {
+        children += jobs[j]->N;
+        lm = std::max(lm, (int64_t)jobs[j]->L * jobs[j]->m);
+        pool = std::max(pool, (int64_t)jobs[j]->P);
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return "dev" + std::to_string(dev) + "/jobs" + std::to_string(log2_bucket(count)) + "/children" +
+           std::to_string(log2_bucket(children)) + "/codons" + std::to_string(log2_bucket(lm)) + "/pool" +
+           std::to_string(log2_bucket(pool));
+}
+
+bool cached_shapes(const std::string &sig, cd::Shapes *out) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(sig);
+    if (it == g_tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void store_shapes(const std::string &sig, const cd::Shapes &sh) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[sig] = sh;
+}
+
+}  // namespace cdhost
 
 extern "C" {
 
@@ -552,37 +701,7 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
     if (!J || J->selection_only || !kernel_ms) return fail(CD_E_INVALID, "null argument");
     if (!J->inited) return fail(CD_E_STATE, "cd_job_profile before cd_job_init");
     if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: advance it through the batch");
-    const int NK = CD_PROFILE_KERNELS;
-    hipEvent_t ev[NK + 1];
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-    double acc[NK] = {0};
-    const Launch w = job_launch(J);
-    for (int g = 0; g < generations; ++g) {
-        HIPCHK(hipEventRecord(ev[0], J->stream));
-        HIPCHK(cd::launch_design(cd::MODE_OFFSPRING, J->d_self, J->d_off, J->N, J->lds, J->stream, J->d_slist,
-                                 J->d_sctl, J->N * J->m));
-        HIPCHK(hipEventRecord(ev[1], J->stream));
-        HIPCHK(cd::launch_stem_list(J->d_self, J->d_off, J->d_slist, J->d_sctl, J->N * J->m, cd::STEM_WAVES, J->stem_lds,
-                                    J->stream));
-        HIPCHK(hipEventRecord(ev[2], J->stream));
-        HIPCHK(cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, J->d_sctl, cd::DOM_ROWS, J->stream));
-        HIPCHK(hipEventRecord(ev[3], J->stream));
-        HIPCHK(cd::launch_fronts(J->d_self, 1, J->stream));
-        HIPCHK(hipEventRecord(ev[4], J->stream));
-        HIPCHK(cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream));
-        HIPCHK(hipEventRecord(ev[5], J->stream));
-        HIPCHK(cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream));
-        HIPCHK(hipEventRecord(ev[6], J->stream));
-        HIPCHK(hipEventSynchronize(ev[6]));
-        for (int k = 0; k < NK; ++k) {
-            float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-            acc[k] += t;
-        }
-    }
-    for (int k = 0; k < NK; ++k) kernel_ms[k] = generations > 0 ? (float)(acc[k] / generations) : 0.f;
-    for (auto &e : ev) HIPCHK(hipEventDestroy(e));
-    return CD_OK;
+    return profile_generations(job_launch(J), J->stream, generations, kernel_ms);
 }
 
 int cd_job_cds_scanned(cd_job *J, int64_t *count) {
@@ -657,10 +776,11 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     hipError_t e = hipMemcpy(J->h.obj[0], obj, (size_t)P * 24, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(J->h.st, &st, sizeof(st), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
-    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, cd::DOM_ROWS, J->stream);
-    if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->stream);
-    if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, J->stream);
-    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, J->stream);
+    const cd::Shapes &sh = J->shapes;
+    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, sh.dom_rows, J->stream);
+    if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, sh.fronts_threads, J->stream);
+    if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, sh.assoc_designs, J->stream);
+    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, sh.niche_threads, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
     if (e != hipSuccess) {
         cd_job_destroy(J);
@@ -725,8 +845,18 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
         (void)hipStreamSynchronize(jobs[j]->stream);
         B->jobs.push_back(jobs[j]);
     }
+    cd::Shapes sh;
+    if (cdhost::cached_shapes(cdhost::work_signature(jobs, count), &sh)) B->shapes = sh;
     cdhost::WorkLists wl;
-    cdhost::build_lists(jobs, count, wl);
+    cdhost::build_lists(jobs, count, B->shapes, wl);
+    // room for the lists in any shape: 64-row tiles and 4-design groups give the most items
+    size_t cap_dom = 0, cap_asc = 0;
+    for (int j = 0; j < count; ++j) {
+        int nd = 0, na = 0;
+        work_counts(jobs[j]->P, jobs[j]->W, &nd, &na);
+        cap_dom += nd;
+        cap_asc += na;
+    }
     B->lds = wl.lds;
     B->stem_lds = wl.stem_lds;
     B->niche_lds = wl.niche_lds;
@@ -740,8 +870,8 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     B->n_asc = (int)wl.asc.size();
     hipError_t e = hipMalloc(&B->d_jobs, sizeof(cd::DevJob) * count);
     if (e == hipSuccess) e = hipMalloc(&B->d_off, sizeof(int2) * wl.off.size());
-    if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * wl.dom.size());
-    if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * wl.asc.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * cap_dom);
+    if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * cap_asc);
     if (e == hipSuccess) e = hipMalloc(&B->d_slist, sizeof(uint32_t) * wl.n_cds);
     if (e == hipSuccess) e = hipMalloc(&B->d_sctl, 16);
     if (e == hipSuccess) e = hipMemset(B->d_sctl, 0, 16);
@@ -760,14 +890,22 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     return CD_OK;
 }
 
+static Launch batch_launch(cd_batch *B) {
+    return Launch{B->d_jobs,    (int)B->jobs.size(), B->d_off,      B->n_off,   B->d_dom,
+                  B->n_dom,     B->d_asc,            B->n_asc,      B->lds,     B->stem_lds,
+                  B->niche_lds, B->mg,               B->stem_parts, B->d_slist, B->d_sctl,
+                  B->n_cds,     B->n_small,          B->lds_small,  B->shapes};
+}
+
+int cd_batch_profile(cd_batch *B, int32_t generations, float *kernel_ms) {
+    if (!B || !kernel_ms) return fail(CD_E_INVALID, "null argument");
+    return profile_generations(batch_launch(B), B->stream, generations, kernel_ms);
+}
+
 int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
     if (!B->gexec) {
-        const Launch w{B->d_jobs, (int)B->jobs.size(), B->d_off,      B->n_off,   B->d_dom,
-                       B->n_dom,  cd::DOM_ROWS_BATCH,  B->d_asc,      B->n_asc,   B->lds,     B->stem_lds,
-                       B->niche_lds, B->mg,            B->stem_parts, B->d_slist, B->d_sctl,
-                       B->n_cds,     B->n_small,       B->lds_small};
-        int rc = capture(B->stream, &B->graph, &B->gexec, w);
+        int rc = capture(B->stream, &B->graph, &B->gexec, batch_launch(B));
         if (rc != CD_OK) return rc;
     }
     for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(B->gexec, B->stream));
@@ -795,6 +933,122 @@ int cd_batch_step_timed(cd_batch *B, int32_t generations, float *ms) {
     if (ms) *ms = t;
     HIPCHK(hipEventDestroy(a));
     HIPCHK(hipEventDestroy(b));
+    return CD_OK;
+}
+
+// Install shapes on a batch: its work lists in that shape, the captured
+// graph dropped (the next step captures the new launches).
+static int batch_apply_shapes(cd_batch *B, const cd::Shapes &sh) {
+    cdhost::WorkLists wl;
+    cdhost::build_lists(B->jobs.data(), (int)B->jobs.size(), sh, wl);
+    HIPCHK(hipStreamSynchronize(B->stream));
+    HIPCHK(hipMemcpy(B->d_off, wl.off.data(), sizeof(int2) * wl.off.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->d_dom, wl.dom.data(), sizeof(int4) * wl.dom.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->d_asc, wl.asc.data(), sizeof(int2) * wl.asc.size(), hipMemcpyHostToDevice));
+    B->n_dom = (int)wl.dom.size();
+    B->n_asc = (int)wl.asc.size();
+    B->n_small = wl.n_small;
+    B->lds_small = wl.lds_small;
+    B->shapes = sh;
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->gexec = nullptr;
+    B->graph = nullptr;
+    return CD_OK;
+}
+
+static int job_apply_shapes(cd_job *J, const cd::Shapes &sh) {
+    std::vector<int4> dom;
+    std::vector<int2> asc;
+    job_lists(J->P, J->W, sh, dom, asc);
+    HIPCHK(hipStreamSynchronize(J->stream));
+    HIPCHK(hipMemcpy(J->d_dom, dom.data(), sizeof(int4) * dom.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(J->d_asc, asc.data(), sizeof(int2) * asc.size(), hipMemcpyHostToDevice));
+    J->n_dom = (int)dom.size();
+    J->n_asc = (int)asc.size();
+    J->shapes = sh;
+    if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
+    if (J->graph) (void)hipGraphDestroy(J->graph);
+    J->gexec = nullptr;
+    J->graph = nullptr;
+    return CD_OK;
+}
+
+static_assert(sizeof(cd_shapes) == sizeof(cd::Shapes), "cd_shapes mirrors cd::Shapes");
+static cd_shapes to_abi(const cd::Shapes &s) {
+    cd_shapes o;
+    std::memcpy(&o, &s, sizeof(o));
+    return o;
+}
+static cd::Shapes from_abi(const cd_shapes &s) {
+    cd::Shapes o;
+    std::memcpy(&o, &s, sizeof(o));
+    return o;
+}
+
+int cd_batch_shapes(cd_batch *B, cd_shapes *get, const cd_shapes *set) {
+    if (!B) return fail(CD_E_INVALID, "null batch");
+    if (set) {
+        const cd::Shapes sh = from_abi(*set);
+        if (!cd::shapes_valid(sh)) return fail(CD_E_INVALID, "invalid launch shapes");
+        const int rc = batch_apply_shapes(B, sh);
+        if (rc != CD_OK) return rc;
+    }
+    if (get) *get = to_abi(B->shapes);
+    return CD_OK;
+}
+
+int cd_job_shapes(cd_job *J, cd_shapes *get, const cd_shapes *set) {
+    if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
+    if (set) {
+        const cd::Shapes sh = from_abi(*set);
+        if (!cd::shapes_valid(sh)) return fail(CD_E_INVALID, "invalid launch shapes");
+        if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch");
+        const int rc = job_apply_shapes(J, sh);
+        if (rc != CD_OK) return rc;
+    }
+    if (get) *get = to_abi(J->shapes);
+    return CD_OK;
+}
+
+int cd_batch_tune(cd_batch *B, int32_t gens, cd_shapes *chosen, int32_t *generations_run) {
+    if (!B || gens < 1) return fail(CD_E_INVALID, "bad tune arguments");
+    cd::Shapes best;
+    int run = 0;
+    const int rc = tune_shapes(
+        B->shapes, (int)B->jobs.size(), gens, B->stream,
+        [&](const cd::Shapes &sh, Launch *w) {
+            const int r = batch_apply_shapes(B, sh);
+            *w = batch_launch(B);
+            return r;
+        },
+        &best, &run);
+    if (generations_run) *generations_run = run;
+    if (rc != CD_OK) return rc;
+    cdhost::store_shapes(cdhost::work_signature(B->jobs.data(), (int)B->jobs.size()), best);
+    if (chosen) *chosen = to_abi(best);
+    return CD_OK;
+}
+
+int cd_job_tune(cd_job *J, int32_t gens, cd_shapes *chosen, int32_t *generations_run) {
+    if (!J || J->selection_only || gens < 1) return fail(CD_E_INVALID, "bad tune arguments");
+    if (!J->inited) return fail(CD_E_STATE, "cd_job_tune before cd_job_init");
+    if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: tune the batch");
+    cd::Shapes best;
+    int run = 0;
+    const int rc = tune_shapes(
+        J->shapes, 1, gens, J->stream,
+        [&](const cd::Shapes &sh, Launch *w) {
+            const int r = job_apply_shapes(J, sh);
+            *w = job_launch(J);
+            return r;
+        },
+        &best, &run);
+    if (generations_run) *generations_run = run;
+    if (rc != CD_OK) return rc;
+    cd_job *self = J;
+    cdhost::store_shapes(cdhost::work_signature(&self, 1), best);
+    if (chosen) *chosen = to_abi(best);
     return CD_OK;
 }
 

@@ -26,11 +26,12 @@ struct cd_job {
     size_t mem_bytes = 0;
     hipStream_t stream = nullptr;
     int2 *d_off = nullptr;
-    int4 *d_dom = nullptr;
-    int2 *d_asc = nullptr;
+    int4 *d_dom = nullptr;        // domination tiles, room for the smallest tile (64 rows)
+    int2 *d_asc = nullptr;        // association groups, room for the smallest group (4 designs)
     uint32_t *d_slist = nullptr;  // changed CDS of a generation (N * m entries) for k_stem_list
     int32_t *d_sctl = nullptr;    // its count, zero between generations (k_dominance clears it)
     int n_dom = 0, n_asc = 0;
+    cd::Shapes shapes = cd::SHAPES_JOB;  // launch shapes of its generations (the lists above follow them)
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int stem_parts = 1;  // k_stem workgroups per CDS (initialisation and evaluation)
     hipGraph_t graph = nullptr;
@@ -41,10 +42,11 @@ struct cd_job {
 
 struct cd_batch {
     std::vector<cd_job *> jobs;
+    cd::Shapes shapes = cd::SHAPES_BATCH;  // launch shapes of its ticks (the lists below follow them)
     cd::DevJob *d_jobs = nullptr;
     int2 *d_off = nullptr;
-    int4 *d_dom = nullptr;
-    int2 *d_asc = nullptr;
+    int4 *d_dom = nullptr;  // room for 64-row tiles
+    int2 *d_asc = nullptr;  // room for 4-design groups
     uint32_t *d_slist = nullptr;  // changed CDS of a generation (n_cds entries)
     int32_t *d_sctl = nullptr;
     int n_off = 0, n_dom = 0, n_asc = 0, n_cds = 0;
@@ -79,9 +81,9 @@ struct Launch {
     int njobs;
     const int2 *off;  // (job, child) of every child
     int n_off;
-    const int4 *dom;  // (job, first row, word, 0) domination tiles of dom_rows rows
-    int n_dom, dom_rows;
-    const int2 *asc;  // (job, first design) association groups
+    const int4 *dom;  // (job, first row, word, 0) domination tiles of sh.dom_rows rows
+    int n_dom;
+    const int2 *asc;  // (job, first design) association groups of sh.assoc_designs designs
     int n_asc;
     size_t lds, stem_lds, niche_lds;
     int mg;          // largest m
@@ -90,10 +92,11 @@ struct Launch {
     int32_t *stem_ctl;    // its count, zero between generations (k_dominance clears it)
     int n_cds;            // CDS of all children: sum of N * m
     // the first n_small children (0: none apart) are of jobs whose k_design
-    // LDS, at most lds_small, fits six workgroups per CU; they are launched
+    // LDS, at most lds_small (at most sh.design_split), they are launched
     // apart from the others, which take lds
     int n_small;
     size_t lds_small;
+    cd::Shapes sh;
 };
 
 // Host images of the work lists of `jobs` (job k = index k of the list)
@@ -104,14 +107,24 @@ struct WorkLists {
     std::vector<int4> dom;
     size_t lds = 0, stem_lds = 0, niche_lds = 0;
     int mg = 0, stem_parts = 0, n_cds = 0;
-    int n_small = 0;  // children (first in off) of jobs with lds at most cd::DESIGN_LDS_SIX
+    int n_small = 0;  // children (first in off) of jobs with lds at most shapes.design_split
     size_t lds_small = 0;
 };
 
-void build_lists(cd_job *const *jobs, int count, WorkLists &wl);
+// The work lists of a launch sequence over `jobs` with the given shapes.
+void build_lists(cd_job *const *jobs, int count, const cd::Shapes &shapes, WorkLists &wl);
 
-// Mutation, evaluation and selection of one generation of every job of `w`.
-hipError_t enqueue_generation(const Launch &w, hipStream_t s);
+// Autotuner.  Shapes picked for a kind of work are cached per device under
+// a coarse signature of that work (job count, children, largest codons per
+// child, largest pool, each as a power-of-two bucket), so later batches,
+// jobs and served ticks of the same kind start from them.
+std::string work_signature(cd_job *const *jobs, int count);
+bool cached_shapes(const std::string &sig, cd::Shapes *out);
+void store_shapes(const std::string &sig, const cd::Shapes &sh);
+
+// Mutation, evaluation and selection of one generation of every job of `w`;
+// ev (or null): CD_PROFILE_KERNELS + 1 events recorded around the phases.
+hipError_t enqueue_generation(const Launch &w, hipStream_t s, hipEvent_t *ev);
 
 // Raise the dynamic LDS limits of the kernels to what `wl` needs.
 hipError_t set_lds_limits(const WorkLists &wl);

@@ -11,13 +11,26 @@ namespace cd {
 enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
 
 constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
-constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (one wave each)
-constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup of a lone job (16 per wave)
-constexpr int DOM_ROWS_BATCH = 128;        // rows per k_dominance workgroup of a batch (32 per wave)
+constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup of the standalone sort and selection
+constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (a wave each), the smallest shape
 constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
 constexpr int STEM_CUS = 256;              // compute units of an MI355X: k_stem_list runs a resident grid
-constexpr int STEM_WAVES = 6;              // k_stem_list waves per SIMD (workgroups per CU) for a lone job
-constexpr int STEM_WAVES_BATCH = 7;        // the same for a batch (fewer registers, more searches at once)
+
+// Launch shapes of one generation's kernels.  Every shape gives the same
+// results -- a shape only changes how the work is split over workgroups and
+// lanes -- so the autotuner may pick any of them while a job runs.
+struct Shapes {
+    int design_split;    // k_design LDS bytes up to which children launch apart from the rest (0: one launch)
+    int stem_waves;      // k_stem_list waves per SIMD (workgroups per CU): 5, 6, 7 or 8
+    int dom_rows;        // rows per k_dominance workgroup: 64, 128 or 256 (a quarter per wave)
+    int fronts_threads;  // k_fronts workgroup: 256, 512 or 1024 threads
+    int assoc_designs;   // designs per k_associate workgroup, a wave each: 4, 8 or 16
+    int niche_threads;   // k_niche workgroup: 256, 512 or 1024 threads
+};
+// Untuned shapes: a lone job, and a batch or served tick.
+constexpr Shapes SHAPES_JOB = {0, 6, 64, 512, 4, 1024};
+constexpr Shapes SHAPES_BATCH = {24 * 1024, 7, 128, 1024, 4, 1024};
+bool shapes_valid(const Shapes &s);
 
 size_t design_lds_bytes(int L, int m);
 size_t stem_lds_bytes(int L);
@@ -45,27 +58,27 @@ hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, i
 // offspring generations: stem of each CDS the offspring k_design listed
 // (items as given to it), capacity the list's size, on as many workgroups
 // as are resident at once (STEM_CUS x the workgroups per CU that `waves`,
-// STEM_WAVES or STEM_WAVES_BATCH, and the LDS allow), at most capacity;
-// launch_dominance with the same stem_ctl clears the count
+// 5 to 8, and the LDS allow), at most capacity; launch_dominance with the
+// same stem_ctl clears the count
 hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
                             const int32_t *stem_ctl, int capacity, int waves, size_t lds, hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
-// items: (job, first row, word, 0), rows (DOM_ROWS or DOM_ROWS_BATCH) rows
-// per item.  finish: the offspring generation, whose children's stem
+// items: (job, first row, word, 0), rows (64, 128 or 256) rows per item.  finish: the offspring generation, whose children's stem
 // objective (and stem records) k_dominance completes in place of
 // launch_stem_finish; stem_ctl (or null): the changed-CDS count of that
 // generation's k_design, cleared for the next one
 hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
                             hipStream_t s);
-// one workgroup per job
-hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s);
-// items: (job, first design)
-hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s);
-// one workgroup per job; lds = the largest niche_lds_bytes of the jobs
+// one workgroup of `threads` (256, 512 or 1024) per job
+hipError_t launch_fronts(const DevJob *jobs, int njobs, int threads, hipStream_t s);
+// items: (job, first design), `designs` (4, 8 or 16) designs per item
+hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int designs, hipStream_t s);
+// one workgroup of `threads` (256, 512 or 1024) per job; lds = the largest
+// niche_lds_bytes of the jobs
 size_t niche_lds_bytes(int P, int R);
 hipError_t set_niche_lds_limit(size_t bytes);
-hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, hipStream_t s);
+hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, int threads, hipStream_t s);
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s);
 

@@ -1584,10 +1584,10 @@ __device__ __forceinline__ float perp_dist(const float *f, const float *w) {  //
 // Nearest reference line of every ranked design (sorting.cuh:1105-1180): one
 // wave per design, lanes take reference lines r = lane, lane + 64, ... and the
 // wave keeps the smallest distance, ties to the lowest index (the first
-// minimum of a sweep in index order).  item = (job, first of ASSOC_DESIGNS
-// designs).
-__global__ __launch_bounds__(64 * ASSOC_DESIGNS) void k_associate(const DevJob *__restrict__ jobs,
-                                                                  const int2 *__restrict__ items) {
+// minimum of a sweep in index order).  item = (job, first of D designs).
+template <int D>
+__global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__ jobs,
+                                                      const int2 *__restrict__ items) {
     const int2 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
     const SelState *st = J.st;
@@ -1907,19 +1907,19 @@ hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, i
 hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
                             const int32_t *stem_ctl, int capacity, int waves, size_t lds, hipStream_t s) {
     if (capacity <= 0) return hipSuccess;
-    if (!stem_list || !stem_ctl || (waves != STEM_WAVES && waves != STEM_WAVES_BATCH)) return hipErrorInvalidValue;
+    if (!stem_list || !stem_ctl || waves < 5 || waves > 8) return hipErrorInvalidValue;
     // workgroups take the entries round-robin, so every one must be resident
     // from the start: per CU, the waves per SIMD or what the LDS (160 KiB,
     // with the kernel's static 256 bytes) holds
     const int by_lds = (int)((160u * 1024u) / (lds + 256u));
     const int per_cu = by_lds < waves ? (by_lds > 1 ? by_lds : 1) : waves;
     const int grid = capacity < STEM_CUS * per_cu ? capacity : STEM_CUS * per_cu;
-    if (waves == STEM_WAVES)
-        hipLaunchKernelGGL(k_stem_list<STEM_WAVES>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl,
-                           capacity);
-    else
-        hipLaunchKernelGGL(k_stem_list<STEM_WAVES_BATCH>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list,
-                           stem_ctl, capacity);
+    switch (waves) {
+    case 5: hipLaunchKernelGGL(k_stem_list<5>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
+    case 6: hipLaunchKernelGGL(k_stem_list<6>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
+    case 7: hipLaunchKernelGGL(k_stem_list<7>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
+    default: hipLaunchKernelGGL(k_stem_list<8>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
+    }
     return hipGetLastError();
 }
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
@@ -1930,25 +1930,35 @@ hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, i
 hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (rows == DOM_ROWS)
-        hipLaunchKernelGGL(k_dominance<DOM_ROWS>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl);
-    else if (rows == DOM_ROWS_BATCH)
-        hipLaunchKernelGGL(k_dominance<DOM_ROWS_BATCH>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl);
-    else
-        return hipErrorInvalidValue;
+    switch (rows) {
+    case 64: hipLaunchKernelGGL(k_dominance<64>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl); break;
+    case 128: hipLaunchKernelGGL(k_dominance<128>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl); break;
+    case 256: hipLaunchKernelGGL(k_dominance<256>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
-hipError_t launch_fronts(const DevJob *jobs, int njobs, hipStream_t s) {
-    // a lone job's workgroup is the only work on its CU, where the waves'
-    // repeated reductions bound it: 8 waves (measured 22.7 -> 21.6 us at
-    // L = 501, N = 1024); a batch's slowest job gains from 16
-    hipLaunchKernelGGL(k_fronts, dim3(njobs), dim3(njobs == 1 ? 512 : 1024), 0, s, jobs);
+hipError_t launch_fronts(const DevJob *jobs, int njobs, int threads, hipStream_t s) {
+    if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fronts, dim3(njobs), dim3(threads), 0, s, jobs);
     return hipGetLastError();
 }
-hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, hipStream_t s) {
+hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int designs, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_associate, dim3(n), dim3(64 * ASSOC_DESIGNS), 0, s, jobs, items);
+    switch (designs) {
+    case 4: hipLaunchKernelGGL(k_associate<4>, dim3(n), dim3(256), 0, s, jobs, items); break;
+    case 8: hipLaunchKernelGGL(k_associate<8>, dim3(n), dim3(512), 0, s, jobs, items); break;
+    case 16: hipLaunchKernelGGL(k_associate<16>, dim3(n), dim3(1024), 0, s, jobs, items); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+bool shapes_valid(const Shapes &h) {
+    return h.design_split >= 0 && h.stem_waves >= 5 && h.stem_waves <= 8 &&
+           (h.dom_rows == 64 || h.dom_rows == 128 || h.dom_rows == 256) &&
+           (h.fronts_threads == 256 || h.fronts_threads == 512 || h.fronts_threads == 1024) &&
+           (h.assoc_designs == 4 || h.assoc_designs == 8 || h.assoc_designs == 16) &&
+           (h.niche_threads == 256 || h.niche_threads == 512 || h.niche_threads == 1024);
 }
 size_t niche_lds_bytes(int P, int R) { return niche_lds_need(P, R); }
 
@@ -1973,8 +1983,9 @@ hipError_t set_niche_lds_limit(size_t bytes) {
         return hipFuncSetAttribute((const void *)k_niche, hipFuncAttributeMaxDynamicSharedMemorySize, b);
     });
 }
-hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(1024), lds, s, jobs);
+hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, int threads, hipStream_t s) {
+    if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(threads), lds, s, jobs);
     return hipGetLastError();
 }
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s) {
@@ -1997,12 +2008,13 @@ hipError_t set_design_lds_limit(size_t bytes) {
 }
 hipError_t set_stem_lds_limit(size_t bytes) {
     return raise_lds_limit(g_stem_lds, bytes, [](int b) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_stem, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-        if (e != hipSuccess) return e;
-        e = hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-        if (e != hipSuccess) return e;
-        return hipFuncSetAttribute((const void *)k_stem_list<STEM_WAVES_BATCH>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        const void *fns[] = {(const void *)k_stem, (const void *)k_stem_list<5>, (const void *)k_stem_list<6>,
+                             (const void *)k_stem_list<7>, (const void *)k_stem_list<8>};
+        for (const void *f : fns) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     });
 }
 

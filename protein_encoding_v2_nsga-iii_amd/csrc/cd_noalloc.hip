@@ -105,7 +105,7 @@ extern "C" int cd_nds_noalloc(const float *obj, int32_t P, int32_t N, int32_t *f
     hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(1024), 0, s, h, dj, tiles, n_tiles, W);
     HIPCHK(hipGetLastError());
     HIPCHK(cd::launch_dominance(dj, tiles, n_tiles, false, nullptr, cd::DOM_ROWS, s));
-    HIPCHK(cd::launch_fronts(dj, 1, s));
+    HIPCHK(cd::launch_fronts(dj, 1, cd::SHAPES_JOB.fronts_threads, s));
     hipLaunchKernelGGL(k_noalloc_finish, dim3(1), dim3(64), 0, s, (const cd::SelState *)h.st, n_fronts);
     HIPCHK(hipGetLastError());
     return CD_OK;

@@ -122,7 +122,9 @@ int collect(Entry *E) {
 hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res, cdhost::WorkLists &wl, cdhost::Launch &L) {
     std::vector<cd_job *> jobs;
     for (Entry *E : res) jobs.push_back(E->job);
-    cdhost::build_lists(jobs.data(), (int)jobs.size(), wl);
+    cd::Shapes sh = cd::SHAPES_BATCH;  // the tuned shapes of this kind of resident set, if any
+    (void)cdhost::cached_shapes(cdhost::work_signature(jobs.data(), (int)jobs.size()), &sh);
+    cdhost::build_lists(jobs.data(), (int)jobs.size(), sh, wl);
     hipError_t e = ensure(S->d_jobs, S->cap_jobs, wl.descs.size());
     if (e == hipSuccess) e = ensure(S->d_off, S->cap_off, wl.off.size());
     if (e == hipSuccess) e = ensure(S->d_dom, S->cap_dom, wl.dom.size());
@@ -138,10 +140,10 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res, cdhost::WorkLi
     if (e == hipSuccess)
         e = hipMemcpyAsync(S->d_asc, wl.asc.data(), sizeof(int2) * wl.asc.size(), hipMemcpyHostToDevice, S->stream);
     if (e == hipSuccess) e = cdhost::set_lds_limits(wl);
-    L = cdhost::Launch{S->d_jobs,   (int)wl.descs.size(), S->d_off,    (int)wl.off.size(), S->d_dom,
-                       (int)wl.dom.size(), cd::DOM_ROWS_BATCH, S->d_asc,     (int)wl.asc.size(), wl.lds,       wl.stem_lds,
-                       wl.niche_lds, wl.mg, wl.stem_parts, S->d_slist, S->d_sctl, wl.n_cds,
-                       wl.n_small,   wl.lds_small};
+    L = cdhost::Launch{S->d_jobs,   (int)wl.descs.size(), S->d_off,      (int)wl.off.size(), S->d_dom,
+                       (int)wl.dom.size(), S->d_asc,      (int)wl.asc.size(), wl.lds,         wl.stem_lds,
+                       wl.niche_lds, wl.mg,               wl.stem_parts,      S->d_slist,     S->d_sctl,
+                       wl.n_cds,     wl.n_small,          wl.lds_small,       sh};
     return e;
 }
 
@@ -206,7 +208,7 @@ void run(cd_server *S) {
             e = rebuild(S, res, wl, launch);
             dirty = false;
         }
-        if (e == hipSuccess) e = cdhost::enqueue_generation(launch, S->stream);
+        if (e == hipSuccess) e = cdhost::enqueue_generation(launch, S->stream, nullptr);
         if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
         if (e != hipSuccess) {  // a device error ends every resident job and the worker
             const std::string msg = std::string("serving loop: ") + hipGetErrorString(e);
