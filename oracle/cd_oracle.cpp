@@ -61,7 +61,6 @@ const int8_t SYN_NUM[21] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4
 constexpr uint32_t TAG_INIT = 0x494E4954u;
 constexpr uint32_t TAG_TYPE = 0x54595045u;
 constexpr uint32_t TAG_MUT = 0x4D555441u;
-constexpr uint32_t TAG_MUTB = 0x4D555442u;
 constexpr uint32_t TAG_SL = 0x534C5452u;
 constexpr uint32_t TAG_NICHE = 0x4E494348u;
 
@@ -393,38 +392,42 @@ inline int gidx(const MutCtx &c, const int8_t *s, int cds, int k) {
     return SYN_START[(int)c.aa[k]] + s[(size_t)cds * c.L + k];
 }
 
-// Fisher-Yates shuffle of 0..n-1 (indexArrayShuffling, utils.cuh:203-218)
-void shuffle(const uint32_t *draws, int n, int8_t *arr) {
+// Fisher-Yates shuffle of 0..n-1 (indexArrayShuffling, utils.cuh:203-218):
+// for i = n-1 down to 1, swap entries i and j_i, with the j_i the
+// mixed-radix digits (radix i+1) of uniform_int(x, n!) -- one draw gives a
+// uniformly random order of the n <= 6 synonyms.
+void shuffle(uint32_t x, int n, int8_t *arr) {
+    static const uint32_t FACT[7] = {1, 1, 2, 6, 24, 120, 720};
     for (int i = 0; i < 6; ++i) arr[i] = (int8_t)(i < n ? i : -1);
-    int d = 0;
+    uint32_t r = uniform_int(x, FACT[n]);
     for (int i = n - 1; i > 0; --i) {
-        int j = (int)uniform_int(draws[d++], (uint32_t)(i + 1));
+        const uint32_t j = r % (uint32_t)(i + 1);
+        r /= (uint32_t)(i + 1);
         std::swap(arr[i], arr[j]);
     }
 }
 
-void draws_for_slot(const MutCtx &c, uint32_t slot, uint32_t out[6]) {
-    uint32_t a[4], b[4];
-    philox(c.gen, c.child, slot, TAG_MUT, c.seed, a);
-    philox(c.gen, c.child, slot, TAG_MUTB, c.seed, b);
-    out[0] = a[0];
-    out[1] = a[1];
-    out[2] = a[2];
-    out[3] = a[3];
-    out[4] = b[0];
-    out[5] = b[1];
+// R1 draws of the operators below: the i-th slot an operator visits in CDS c
+// (every codon, or every other codon for the parity operators) takes words
+// 2(i&1) -- the mutation test -- and 2(i&1)+1 -- the choice or the shuffle --
+// of Philox(gen, child, c*L + (i >> 1), TAG_MUT).
+void slot_draws(const MutCtx &c, int cds, int i, uint32_t &test, uint32_t &pick) {
+    uint32_t r[4];
+    philox(c.gen, c.child, (uint32_t)(cds * c.L + (i >> 1)), TAG_MUT, c.seed, r);
+    test = r[2 * (i & 1)];
+    pick = r[2 * (i & 1) + 1];
 }
 
 // mutationRandom, mutation.cuh:16-63: every codon of every CDS
 void mut_random(const MutCtx &c, int8_t *s) {
     for (int cds = 0; cds < c.m; ++cds)
         for (int k = 0; k < c.L; ++k) {
-            uint32_t slot = (uint32_t)(cds * c.L + k), r[4];
-            philox(c.gen, c.child, slot, TAG_MUT, c.seed, r);
+            uint32_t test, pick;
+            slot_draws(c, cds, k, test, pick);
             int ns = syn_of(c, k);
-            if (!bern(r[0], c.thr_pm) || ns == 1) continue;
+            if (!bern(test, c.thr_pm) || ns == 1) continue;
             int8_t &cur = s[(size_t)cds * c.L + k];
-            int nw = (int)uniform_int(r[1], (uint32_t)(ns - 1));
+            int nw = (int)uniform_int(pick, (uint32_t)(ns - 1));
             if (nw >= cur) ++nw;
             cur = (int8_t)nw;
         }
@@ -433,12 +436,12 @@ void mut_random(const MutCtx &c, int8_t *s) {
 // mutationCAI, SELECT_UPPER_RANDOM, mutation.cuh:65-142 (main.cu:187)
 void mut_cai(const MutCtx &c, int8_t *s, int cds) {
     for (int k = 0; k < c.L; ++k) {
-        uint32_t r[4];
-        philox(c.gen, c.child, (uint32_t)(cds * c.L + k), TAG_MUT, c.seed, r);
+        uint32_t test, pick;
+        slot_draws(c, cds, k, test, pick);
         int ns = syn_of(c, k);
-        if (!bern(r[0], c.thr_pm) || ns == 1) continue;
+        if (!bern(test, c.thr_pm) || ns == 1) continue;
         int8_t &cur = s[(size_t)cds * c.L + k];
-        if (cur != ns - 1) cur = (int8_t)(cur + 1 + (int)uniform_int(r[1], (uint32_t)(ns - 1 - cur)));
+        if (cur != ns - 1) cur = (int8_t)(cur + 1 + (int)uniform_int(pick, (uint32_t)(ns - 1 - cur)));
     }
 }
 
@@ -448,10 +451,10 @@ void mut_cbp(const MutCtx &c, int8_t *s, int cds) {
     const int parity = (int)(c.gen & 1u);
     for (int k = 0; k < c.L - 1; ++k) {
         if ((k % 2) != parity) continue;
-        uint32_t r[4];
-        philox(c.gen, c.child, (uint32_t)(cds * c.L + k), TAG_MUT, c.seed, r);
+        uint32_t test, pick;
+        slot_draws(c, cds, k >> 1, test, pick);
         int ns = syn_of(c, k);
-        if (!bern(r[0], c.thr_2pm) || ns == 1) continue;
+        if (!bern(test, c.thr_2pm) || ns == 1) continue;
         const int base = SYN_START[(int)c.aa[k]];
         const int cur = s[(size_t)cds * c.L + k];
         const int left = (k > 0) ? gidx(c, s, cds, k - 1) : -1;  // R7: no left neighbour at k=0
@@ -466,7 +469,7 @@ void mut_cbp(const MutCtx &c, int8_t *s, int cds) {
         int cand[6], nc = 0;
         for (int j = 0; j < ns; ++j)
             if (j != cur && score(base + j) > curv) cand[nc++] = j;
-        if (nc > 0) s[(size_t)cds * c.L + k] = (int8_t)cand[uniform_int(r[1], (uint32_t)nc)];
+        if (nc > 0) s[(size_t)cds * c.L + k] = (int8_t)cand[uniform_int(pick, (uint32_t)nc)];
     }
 }
 
@@ -476,10 +479,10 @@ void mut_hsc(const MutCtx &c, int8_t *s, int cds) {
     const int parity = (int)(c.gen & 1u);
     for (int k = 0; k < c.L; ++k) {
         if ((k % 2) != parity) continue;
-        uint32_t r[4];
-        philox(c.gen, c.child, (uint32_t)(cds * c.L + k), TAG_MUT, c.seed, r);
+        uint32_t test, pick;
+        slot_draws(c, cds, k >> 1, test, pick);
         int ns = syn_of(c, k);
-        if (!bern(r[0], c.thr_2pm) || ns == 1) continue;
+        if (!bern(test, c.thr_2pm) || ns == 1) continue;
         const int base = SYN_START[(int)c.aa[k]];
         const int cur = s[(size_t)cds * c.L + k];
         const int left = (k > 0) ? gidx(c, s, cds, k - 1) : -1;  // R7
@@ -494,7 +497,7 @@ void mut_hsc(const MutCtx &c, int8_t *s, int cds) {
         int cand[6], nc = 0;
         for (int j = 0; j < ns; ++j)
             if (j != cur && count(base + j) > curv) cand[nc++] = j;
-        if (nc > 0) s[(size_t)cds * c.L + k] = (int8_t)cand[uniform_int(r[1], (uint32_t)nc)];
+        if (nc > 0) s[(size_t)cds * c.L + k] = (int8_t)cand[uniform_int(pick, (uint32_t)nc)];
     }
 }
 
@@ -502,12 +505,12 @@ void mut_hsc(const MutCtx &c, int8_t *s, int cds) {
 void mut_hd(const MutCtx &c, int8_t *s, int cds1, int cds2) {
     const Tables &T = *c.T;
     for (int k = 0; k < c.L; ++k) {
-        uint32_t d[6];
-        draws_for_slot(c, (uint32_t)(cds1 * c.L + k), d);
+        uint32_t test, pick;
+        slot_draws(c, cds1, k, test, pick);
         int ns = syn_of(c, k);
-        if (!bern(d[0], c.thr_pm) || ns == 1) continue;
+        if (!bern(test, c.thr_pm) || ns == 1) continue;
         int8_t order[6];
-        shuffle(d + 1, ns, order);
+        shuffle(pick, ns, order);
         const int base = SYN_START[(int)c.aa[k]];
         const int cur = s[(size_t)cds1 * c.L + k];
         const int g2 = gidx(c, s, cds2, k);
@@ -545,12 +548,12 @@ void mut_gc3(const MutCtx &c, int8_t *s, int cds, int dir, float dev) {
     const Tables &T = *c.T;
     const uint64_t thr = bern_threshold(dev);
     for (int k = 0; k < c.L; ++k) {
-        uint32_t d[6];
-        draws_for_slot(c, (uint32_t)(cds * c.L + k), d);
+        uint32_t test, pick;
+        slot_draws(c, cds, k, test, pick);
         int ns = syn_of(c, k);
-        if (!bern(d[0], thr) || ns == 1) continue;
+        if (!bern(test, thr) || ns == 1) continue;
         int8_t order[6];
-        shuffle(d + 1, ns, order);
+        shuffle(pick, ns, order);
         const int base = SYN_START[(int)c.aa[k]];
         const int cur = s[(size_t)cds * c.L + k];
         const int curg = T.gc3[base + cur];

@@ -39,53 +39,50 @@ char translate(const uint8_t *c3) {  // codes A0 C1 G2 U3 -> index in U C A G or
     return CODE[ucag[c3[0]] * 16 + ucag[c3[1]] * 4 + ucag[c3[2]]];
 }
 
+// Tables of a job indexed by codon code (first base << 4 | second << 2 |
+// third, A=0 C=1 G=2 U=3), the form k_design works in.
 struct HostTables {
-    uint8_t nt[192];
-    int64_t lw[64];
-    int64_t cpfx[61 * 61];
-    float cps[61 * 61];
-    uint8_t hs[64 * 64];
-    uint8_t nd[64 * 64];
-    uint8_t gc3[64];
+    uint8_t code[64];        // code of each codon of the table
+    int64_t lw6[64];         // 2^40-scaled log weights
+    int64_t cpfx6[64 * 64];  // 2^40-scaled pair scores, 0 where either codon is a stop
+    float cps6[64 * 64];     // pair scores, 0 where either codon is a stop
+    uint64_t stop6;          // bit per code: the table's last three codons
 };
 
 int build_tables(const cd_tables &t, HostTables &h) {
     if (!t.codons || !t.weights || !t.cps) return fail(CD_E_INVALID, "null table pointer");
+    uint8_t nt[192];
     for (int i = 0; i < 192; ++i) {
         const int c = nt_code(t.codons[i]);
         if (c < 0) return fail(CD_E_INVALID, "codon table letters must be A, C, G or U");
-        h.nt[i] = (uint8_t)c;
+        nt[i] = (uint8_t)c;
     }
     for (int a = 0; a < 21; ++a)
         for (int g = SYN_START[a]; g < SYN_START[a] + SYN_NUM[a]; ++g)
-            if (translate(&h.nt[3 * g]) != AMINO[a])
+            if (translate(&nt[3 * g]) != AMINO[a])
                 return fail(CD_E_INVALID, "codon table block does not encode its amino acid");
+    uint64_t seen = 0;
+    for (int g = 0; g < 64; ++g) {
+        h.code[g] = (uint8_t)(nt[3 * g] << 4 | nt[3 * g + 1] << 2 | nt[3 * g + 2]);
+        seen |= 1ull << h.code[g];
+    }
+    if (seen != ~0ull) return fail(CD_E_INVALID, "codon table lists a codon twice");
+    h.stop6 = 0;
+    for (int g = 61; g < 64; ++g) h.stop6 |= 1ull << h.code[g];
     for (int g = 0; g < 64; ++g) {
         const float w = t.weights[g];
         if (!(w > 0.f) || w > 1.f) return fail(CD_E_INVALID, "CAI weights must lie in (0, 1]");
-        h.lw[g] = llrint(std::log((double)w) * 0x1p40);
+        h.lw6[h.code[g]] = llrint(std::log((double)w) * 0x1p40);
     }
-    for (int i = 0; i < 61 * 61; ++i) {
-        const float v = t.cps[i];
-        if (!(std::fabs(v) <= 1000.f)) return fail(CD_E_INVALID, "codon-pair scores must be finite and |v| <= 1000");
-        h.cps[i] = v;
-        h.cpfx[i] = llrint((double)v * 0x1p40);
-    }
-    auto is_stop = [&](int a, int b, int c) {  // the table's last three codons (utils.cuh:101-112)
-        for (int s = 61; s < 64; ++s)
-            if (h.nt[3 * s] == a && h.nt[3 * s + 1] == b && h.nt[3 * s + 2] == c) return true;
-        return false;
-    };
-    for (int a = 0; a < 64; ++a) {
-        for (int b = 0; b < 64; ++b) {
-            h.hs[a * 64 + b] = (uint8_t)(is_stop(h.nt[3 * a + 1], h.nt[3 * a + 2], h.nt[3 * b]) +
-                                         is_stop(h.nt[3 * a + 2], h.nt[3 * b], h.nt[3 * b + 1]));
-            int d = 0;
-            for (int k = 0; k < 3; ++k) d += h.nt[3 * a + k] != h.nt[3 * b + k];
-            h.nd[a * 64 + b] = (uint8_t)d;
+    std::memset(h.cpfx6, 0, sizeof(h.cpfx6));
+    std::memset(h.cps6, 0, sizeof(h.cps6));
+    for (int a = 0; a < 61; ++a)
+        for (int b = 0; b < 61; ++b) {
+            const float v = t.cps[a * 61 + b];
+            if (!(std::fabs(v) <= 1000.f)) return fail(CD_E_INVALID, "codon-pair scores must be finite and |v| <= 1000");
+            h.cps6[h.code[a] * 64 + h.code[b]] = v;
+            h.cpfx6[h.code[a] * 64 + h.code[b]] = llrint((double)v * 0x1p40);
         }
-        h.gc3[a] = (h.nt[3 * a + 2] == 1 || h.nt[3 * a + 2] == 2);
-    }
     return CD_OK;
 }
 
@@ -140,7 +137,7 @@ namespace {
 
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
-    size_t aa, nt, lw, cpfx, cps, hs, nd, gc3, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
+    size_t aa, code, lw6, cpfx6, cps6, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
         sctl, sorted,
         front,
         np, dom,
@@ -153,15 +150,12 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
     Layout o{};
     if (!sel_only) {
         o.aa = s.add(L);
-        o.nt = s.add(192);
-        o.lw = s.add(64 * 8);
-        o.cpfx = s.add(61 * 61 * 8);
-        o.cps = s.add(61 * 61 * 4);
-        o.hs = s.add(4096);
-        o.nd = s.add(4096);
-        o.gc3 = s.add(64);
+        o.code = s.add(64);
+        o.lw6 = s.add(64 * 8);
+        o.cpfx6 = s.add(64 * 64 * 8);
+        o.cps6 = s.add(64 * 64 * 4);
         for (int b = 0; b < 2; ++b) {
-            o.genome[b] = s.add((size_t)P * m * L);
+            o.genome[b] = s.add((size_t)P * m * cd::row_stride(L));
             o.oidx[b] = s.add((size_t)P * 12);
             o.pql[b] = s.add((size_t)P * 16);
             o.cst[b] = s.add((size_t)P * m * sizeof(cd::CdsStat));
@@ -256,6 +250,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.R = J->R;
     h.W = J->W;
     h.den_sl = sel_only ? 1 : (3 * J->L - 4) / 2;
+    h.Ls = sel_only ? 0 : cd::row_stride(J->L);
     h.seed = d->seed;
     h.thr_pm = bern_threshold_host(d->Pm);
     h.thr_2pm = bern_threshold_host(d->Pm * 2.0f);
@@ -269,21 +264,16 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
             return rc;
         }
         std::memcpy(&host[o.aa], d->aa, J->L);
-        std::memcpy(&host[o.nt], t.nt, sizeof(t.nt));
-        std::memcpy(&host[o.lw], t.lw, sizeof(t.lw));
-        std::memcpy(&host[o.cpfx], t.cpfx, sizeof(t.cpfx));
-        std::memcpy(&host[o.cps], t.cps, sizeof(t.cps));
-        std::memcpy(&host[o.hs], t.hs, sizeof(t.hs));
-        std::memcpy(&host[o.nd], t.nd, sizeof(t.nd));
-        std::memcpy(&host[o.gc3], t.gc3, sizeof(t.gc3));
+        std::memcpy(&host[o.code], t.code, sizeof(t.code));
+        std::memcpy(&host[o.lw6], t.lw6, sizeof(t.lw6));
+        std::memcpy(&host[o.cpfx6], t.cpfx6, sizeof(t.cpfx6));
+        std::memcpy(&host[o.cps6], t.cps6, sizeof(t.cps6));
         h.aa = (const int8_t *)(base + o.aa);
-        h.nt = (const uint8_t *)(base + o.nt);
-        h.lw = (const int64_t *)(base + o.lw);
-        h.cpfx = (const int64_t *)(base + o.cpfx);
-        h.cps = (const float *)(base + o.cps);
-        h.hs = (const uint8_t *)(base + o.hs);
-        h.nd = (const uint8_t *)(base + o.nd);
-        h.gc3 = (const uint8_t *)(base + o.gc3);
+        h.code = (const uint8_t *)(base + o.code);
+        h.lw6 = (const int64_t *)(base + o.lw6);
+        h.cpfx6 = (const int64_t *)(base + o.cpfx6);
+        h.cps6 = (const float *)(base + o.cps6);
+        h.stop6 = t.stop6;
         for (int b = 0; b < 2; ++b) {
             h.genome[b] = (int8_t *)(base + o.genome[b]);
             h.oidx[b] = (int8_t *)(base + o.oidx[b]);
@@ -677,7 +667,8 @@ int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql
     cd::SelState st;
     HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
     const int b = (int)(st.gen & 1u), P = J->P;
-    if (genome) HIPCHK(hipMemcpy(genome, J->h.genome[b], (size_t)P * J->m * J->L, hipMemcpyDeviceToHost));
+    if (genome)  // CDS rows of L codons from rows of stride Ls
+        HIPCHK(hipMemcpy2D(genome, J->L, J->h.genome[b], J->h.Ls, J->L, (size_t)P * J->m, hipMemcpyDeviceToHost));
     if (obj) HIPCHK(hipMemcpy(obj, J->h.obj[b], (size_t)P * 6 * 4, hipMemcpyDeviceToHost));
     if (idx) HIPCHK(hipMemcpy(idx, J->h.oidx[b], (size_t)P * 12, hipMemcpyDeviceToHost));
     if (pql) {
@@ -737,13 +728,13 @@ int cd_eval_batch(const cd_job_desc *desc, const int8_t *genome, int32_t count, 
     cd_job *J = nullptr;
     rc = create_job(&d, false, 0, &J);
     if (rc != CD_OK) return rc;
-    const size_t G = (size_t)J->m * J->L;
     std::vector<int2> items(count);
     for (int i = 0; i < count; ++i) items[i] = make_int2(0, i);
     int2 *d_items = nullptr;
     hipError_t e = hipMalloc(&d_items, sizeof(int2) * count);
     if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), sizeof(int2) * count, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(J->h.genome[0], genome, G * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess)  // CDS rows of L codons into rows of stride Ls
+        e = hipMemcpy2D(J->h.genome[0], J->h.Ls, genome, J->L, J->L, (size_t)count * J->m, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = enqueue_designs(job_launch(J), cd::MODE_EVAL, d_items, count, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
     if (e == hipSuccess && obj) e = hipMemcpy(obj, J->h.obj[0], (size_t)count * 24, hipMemcpyDeviceToHost);

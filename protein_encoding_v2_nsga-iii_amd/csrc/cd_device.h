@@ -125,6 +125,10 @@ __device__ __forceinline__ void gstore(T *p, T v) {
     *(__attribute__((address_space(1))) T *)p = v;
 }
 
+// 16-byte words (a native 4 x u32 vector, which the global address space
+// takes as it is; HIP's vector class does not bind through that cast).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // Workgroup barrier that orders LDS only.  __syncthreads also waits for the
 // workgroup's outstanding global stores; code whose threads never read what
 // other threads of the workgroup stored to global memory can let those

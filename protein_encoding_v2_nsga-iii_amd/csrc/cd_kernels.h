@@ -47,10 +47,6 @@ hipError_t set_stem_lds_limit(size_t bytes);
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
                          uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, int stem_cap = 0,
                          int item_base = 0);
-// Dynamic LDS of k_design at which six workgroups fit in one CU's LDS (with
-// its static LDS); the batched and served forms launch the children of such
-// jobs apart from the others
-constexpr size_t DESIGN_LDS_SIX = 24 * 1024;
 // initialisation and evaluation: parts workgroups per (item, CDS c < mg),
 // parts the largest stem_parts_of of the jobs: stem of each CDS k_design marked
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,

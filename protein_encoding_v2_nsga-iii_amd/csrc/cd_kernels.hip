@@ -29,20 +29,19 @@ namespace cd {
 
 __host__ __device__ constexpr int align16(int x) { return (x + 15) & ~15; }
 
-// Table block at the start of k_design's dynamic LDS, staged from the job's
-// tables in 16-byte words: log weights lw [64] (int64), nucleotide
-// mismatches nd [64*64], hidden stops hs [64*64], nucleotide codes nt [64*3],
-// GC3 flags gc3 [64].
-constexpr int DT_LW = 0, DT_ND = 512, DT_HS = DT_ND + 4096, DT_NT = DT_HS + 4096, DT_GC3 = DT_NT + 192,
-              DT_BYTES = DT_GC3 + 64;
-static_assert(DT_BYTES % 16 == 0, "table block");
+// k_design's dynamic LDS: log weights by codon code lw6 [64] (int64), the
+// code of every codon of the table [64], per position the first codon and
+// the number of synonyms of its amino acid [Ls] each, then the design's CDS
+// rows (stride Ls = row_stride(L)): synonym indices syn [m][Ls] and codon
+// codes cd6 [m][Ls], then the CDS statistics [m].  Rows are 16-byte aligned,
+// so the genome moves between HBM and LDS in 16-byte words and four codes
+// share a 32-bit word in the Hamming sums.
+constexpr int DL_LW6 = 0, DL_CODE = 512, DL_POS = 576;
+static_assert(DL_POS % 16 == 0, "design LDS rows are 16-byte aligned");
 
-// dynamic LDS of k_design: the table block, codons [m*L], per position the
-// first global codon and the number of synonyms of its amino acid [L] each,
-// nucleotide codes [3L], CDS statistics [m]
 size_t design_lds_bytes(int L, int m) {
-    return (size_t)DT_BYTES + align16(m * L) + 2 * (size_t)align16(L) + align16(3 * L) +
-           align16(m * (int)sizeof(CdsStat)) + 16;
+    const size_t Ls = row_stride(L);
+    return (size_t)DL_POS + 2 * Ls + 2 * (size_t)m * Ls + align16(m * (int)sizeof(CdsStat)) + 16;
 }
 
 // one CDS's bit planes: the sequence and its reverse complement, each
@@ -94,17 +93,33 @@ __device__ __forceinline__ void gstore_stat(CdsStat *p, const CdsStat &s) {
     gstore(&p->key, s.key);
 }
 
-// k_design's LDS copies of the tables and the per-position synonym ranges.
+// k_design's LDS tables and the per-position synonym ranges.
 struct DesignLds {
-    const int64_t *lw;     // [64]
-    const uint8_t *nd;     // [64*64]
-    const uint8_t *hs;     // [64*64]
-    const uint8_t *nt;     // [64*3]
-    const uint8_t *gc3;    // [64]
-    const uint8_t *pbase;  // [L] first global codon of position k's amino acid
-    const uint8_t *pnum;   // [L] synonyms of position k's amino acid
+    const int64_t *lw6;    // [64] log weights by code
+    const uint8_t *code;   // [64] code of each codon of the table
+    const uint8_t *pbase;  // [Ls] first codon of position k's amino acid
+    const uint8_t *pnum;   // [Ls] synonyms of position k's amino acid
+    uint64_t stop6;        // stop codes
+    int Ls;                // row stride
 };
 
+// Facts of codons given by their codes (first base << 4 | second << 2 | third).
+// Third base C (1) or G (2).
+__device__ __forceinline__ int code_gc3(int a) { return (a ^ (a >> 1)) & 1; }
+// Hidden stop codons read across the junction a|b: a[1] a[2] b[0] and
+// a[2] b[0] b[1] (countRightSideHSC, utils.cuh:138-161).
+__device__ __forceinline__ int code_hidden(uint64_t stop6, int a, int b) {
+    return (int)((stop6 >> (((a & 15) << 2) | (b >> 4))) & 1u) + (int)((stop6 >> (((a & 3) << 4) | (b >> 2))) & 1u);
+}
+// Nucleotide mismatches of the codons whose codes are the bytes of x ^ y:
+// a byte's three 2-bit fields differ when either bit of the field is set.
+__device__ __forceinline__ int code_mismatch4(uint32_t x) { return __popc((x | (x >> 1)) & 0x15151515u); }
+__device__ __forceinline__ int code_mismatch(int a, int b) { return code_mismatch4((uint32_t)(a ^ b)); }
+// Base p of a CDS given as a row of codes.
+__device__ __forceinline__ int code_base(const uint8_t *row, int p) {
+    const unsigned k = (unsigned)p / 3u;
+    return (row[k] >> (4 - 2 * (p - 3 * (int)k))) & 3;
+}
 // ---------------------------------------------------------------------------
 // Longest stem of one CDS held as bit planes in LDS (calMaximumSL,
 // utils.cuh:712-888; the planes are built by k_design).  A changed CDS of an
@@ -303,16 +318,17 @@ __host__ __device__ inline int stem_parts(int n) {
 
 // ---------------------------------------------------------------------------
 // Evaluation of the CDS flagged dirty, then objectives 0-4
-// (calMinimumCAI/CBP/HSC/HD, calMaximumGC3, utils.cuh:271-626).  A changed
-// CDS is marked for k_stem (pad = 1); its stem fields keep the parent's stem,
-// which seeds the scan, or are cleared when there is no parent.  Up to four
-// dirty CDS share one pass over the positions, and up to eight CDS pairs one
-// pass of the Hamming distances, so the table reads of a pass overlap; the
-// sums are wave sums (DPP) added over the waves by one thread per value.
+// (calMinimumCAI/CBP/HSC/HD, calMaximumGC3, utils.cuh:271-626), on codon
+// codes.  A changed CDS is marked for k_stem (pad = 1); its stem fields keep
+// the parent's stem, which seeds the scan, or are cleared when there is no
+// parent.  Up to four dirty CDS share one pass over the positions; the
+// Hamming distances take four codons per 32-bit word of two code rows, eight
+// CDS pairs per pass; the sums are wave sums (DPP) added over the waves by
+// one thread per value.
 // ---------------------------------------------------------------------------
-__device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod, CdsStat *cst, bool have_parent,
+__device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6, CdsStat *cst, bool have_parent,
                          DesignShared &sh, float *obj, int32_t *oidx) {
-    const int L = J.L, m = J.m, n = 3 * L;
+    const int L = J.L, m = J.m, n = 3 * L, Ls = D.Ls;
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     uint32_t dmask = 0;  // the same in every thread
     for (int c = 0; c < m; ++c) dmask |= (uint32_t)(sh.dirty[c] != 0) << c;
@@ -329,14 +345,14 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (cs[u] < 0) continue;
-                const uint8_t *g = cod + cs[u] * L;
+                const uint8_t *g = cd6 + cs[u] * Ls;
                 const int a = g[k];
-                cai[u] += D.lw[a];
-                gcn[u] += D.gc3[a];
+                cai[u] += D.lw6[a];
+                gcn[u] += code_gc3(a);
                 if (k < L - 1) {
                     const int b = g[k + 1];
-                    hsc[u] += D.hs[a * NCODON + b];
-                    if (k < L - 2) cpb[u] += gload(J.cpfx + a * NSENSE + b);
+                    hsc[u] += code_hidden(D.stop6, a, b);
+                    if (k < L - 2) cpb[u] += gload(J.cpfx6 + a * 64 + b);
                 }
             }
         }
@@ -379,8 +395,10 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
         }
         lds_barrier();
     }
-    // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order
-    const int npair = m * (m - 1) / 2;
+    // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order;
+    // the rows' padding codes are 0 in every CDS and add nothing
+    const int npair = m * (m - 1) / 2, nwd = Ls >> 2;
+    const uint32_t *w6 = (const uint32_t *)cd6;
     int pi = 0, pj = 1;  // first pair of the pass
     for (int p0 = 0; p0 < npair; p0 += 8) {
         int ai[8], aj[8];
@@ -389,8 +407,8 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const bool ok = p0 + u < npair;
-                ai[u] = ok ? i : -1;
-                aj[u] = j;
+                ai[u] = ok ? i * nwd : -1;
+                aj[u] = j * nwd;
                 if (ok && ++j == m) {
                     ++i;
                     j = i + 1;
@@ -400,10 +418,10 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
             pj = j;
         }
         int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int k = tid; k < L; k += NT)
+        for (int q = tid; q < nwd; q += NT)
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                if (ai[u] >= 0) acc[u] += D.nd[cod[ai[u] * L + k] * NCODON + cod[aj[u] * L + k]];
+                if (ai[u] >= 0) acc[u] += code_mismatch4(w6[ai[u] + q] ^ w6[aj[u] + q]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             if (ai[u] < 0) continue;
@@ -473,9 +491,13 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cod
 }
 
 // ---------------------------------------------------------------------------
-// Mutation operators (include/mutation.cuh) on global codon indices in LDS.
-// Each thread owns whole codons; operators that read neighbours only touch
-// codons of the other parity or of other CDS, as in the reference.
+// Mutation operators (include/mutation.cuh) on the synonym indices (syn) and
+// codon codes (cd6) in LDS.  Each thread owns whole codons; operators that
+// read neighbours only touch codons of the other parity or of other CDS, as
+// in the reference.  Draws: the i-th slot an operator visits in CDS c takes
+// words 2(i&1) (the mutation test) and 2(i&1)+1 (the choice, or the
+// shuffle) of Philox(gen, child, c*L + (i >> 1), TAG_MUT), so one call serves
+// two slots and a thread takes the two slots of one call.
 // ---------------------------------------------------------------------------
 // The pick-th (0-based) set bit of a candidate mask.
 __device__ __forceinline__ int nth_bit(uint32_t mask, uint32_t pick) {
@@ -486,148 +508,171 @@ __device__ __forceinline__ int nth_bit(uint32_t mask, uint32_t pick) {
 struct MutArgs {
     const DevJob *J;
     const DesignLds *D;
-    uint8_t *cod;
+    uint8_t *syn, *cd6;
     uint32_t gen, child;
 };
 
-// mutationRandom, mutation.cuh:16-63
+__device__ __forceinline__ void set_syn(const MutArgs &A, int c, int k, int s) {
+    const int i = c * A.D->Ls + k;
+    A.syn[i] = (uint8_t)s;
+    A.cd6[i] = A.D->code[A.D->pbase[k] + s];
+}
+
+__device__ __forceinline__ void pair_draws(const MutArgs &A, int c, int q, uint32_t r[4]) {
+    philox(A.gen, A.child, (uint32_t)(c * A.J->L + q), TAG_MUT, A.J->seed, r);
+}
+
+// mutationRandom, mutation.cuh:16-63: every codon of every CDS; a thread
+// takes codons 2q and 2q+1 of CDS c for its (c, q) pairs
 __device__ void mut_random(const MutArgs &A, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L, mL = J.m * L;
-    for (int t = threadIdx.x; t < mL; t += blockDim.x) {
-        const int c = t / L, k = t - c * L;
+    const int L = J.L, Ls = A.D->Ls, hL = (L + 1) >> 1, total = J.m * hL;
+    int c = (int)((unsigned)threadIdx.x / (unsigned)hL), q = (int)threadIdx.x - c * hL;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
         uint32_t r[4];
-        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int ns = A.D->pnum[k];
-        if (!bern(r[0], J.thr_pm) || ns == 1) continue;
-        const int base = A.D->pbase[k], cur = A.cod[t] - base;
-        int nw = (int)uniform_int(r[1], (uint32_t)(ns - 1));
-        if (nw >= cur) ++nw;
-        A.cod[t] = (uint8_t)(base + nw);
-        dirty[c] = 1;
+        pair_draws(A, c, q, r);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * q + h;
+            if (k >= L) break;
+            const int ns = A.D->pnum[k];
+            if (!bern(r[2 * h], J.thr_pm) || ns == 1) continue;
+            const int cur = A.syn[c * Ls + k];
+            int nw = (int)uniform_int(r[2 * h + 1], (uint32_t)(ns - 1));
+            if (nw >= cur) ++nw;
+            set_syn(A, c, k, nw);
+            dirty[c] = 1;
+        }
+        q += blockDim.x;
+        while (q >= hL) {
+            q -= hL;
+            ++c;
+        }
     }
 }
 
 // mutationCAI with SELECT_UPPER_RANDOM, mutation.cuh:65-142
 __device__ void mut_cai(const MutArgs &A, int cds, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L;
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const int t = cds * L + k;
+    const int L = J.L, Ls = A.D->Ls;
+    for (int q = threadIdx.x; 2 * q < L; q += blockDim.x) {
         uint32_t r[4];
-        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int ns = A.D->pnum[k];
-        if (!bern(r[0], J.thr_pm) || ns == 1) continue;
-        const int base = A.D->pbase[k], cur = A.cod[t] - base;
-        if (cur != ns - 1) {
-            A.cod[t] = (uint8_t)(base + cur + 1 + (int)uniform_int(r[1], (uint32_t)(ns - 1 - cur)));
-            dirty[cds] = 1;
+        pair_draws(A, cds, q, r);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * q + h;
+            if (k >= L) break;
+            const int ns = A.D->pnum[k];
+            if (!bern(r[2 * h], J.thr_pm) || ns == 1) continue;
+            const int cur = A.syn[cds * Ls + k];
+            if (cur != ns - 1) {
+                set_syn(A, cds, k, cur + 1 + (int)uniform_int(r[2 * h + 1], (uint32_t)(ns - 1 - cur)));
+                dirty[cds] = 1;
+            }
         }
     }
 }
 
-// mutationCBP with SELECT_UPPER_RANDOM, mutation.cuh:144-262
+// mutationCBP with SELECT_UPPER_RANDOM, mutation.cuh:144-262: the codons
+// k = 2i + parity below L-1; a thread takes i = 2q, 2q+1 (k = 4q + parity,
+// 4q + parity + 2)
 __device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L, parity = (int)(A.gen & 1u);
-    for (int k = threadIdx.x; k < L - 1; k += blockDim.x) {
-        if ((k & 1) != parity) continue;
-        const int t = cds * L + k;
+    const int L = J.L, Ls = A.D->Ls, parity = (int)(A.gen & 1u);
+    const uint8_t *row = A.cd6 + cds * Ls;
+    for (int q = threadIdx.x; 4 * q + parity < L - 1; q += blockDim.x) {
         uint32_t r[4];
-        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int ns = A.D->pnum[k];
-        if (!bern(r[0], J.thr_2pm) || ns == 1) continue;
-        const int base = A.D->pbase[k], cur = A.cod[t] - base;
-        const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
-        const int right = A.cod[t + 1];
-        // every synonym's score, the pair-score loads issued together
-        // (addresses clamped into the table, values masked: the same sums)
-        const bool hl = left >= 0, hr = right < NSENSE;
-        const float *rowl = J.cps + (hl ? left : 0) * NSENSE, *colr = J.cps + (hr ? right : 0);
-        float sc[MAX_SYN];
+        pair_draws(A, cds, q, r);
 #pragma unroll
-        for (int j = 0; j < MAX_SYN; ++j) {
-            const int g = base + (j < ns ? j : 0);
-            const float a = gload(rowl + g), b = gload(colr + g * NSENSE);
-            float v = 0.f;
-            v += hl ? a : 0.f;
-            v += hr ? b : 0.f;
-            sc[j] = v;
-        }
-        float curv = sc[0];
+        for (int h = 0; h < 2; ++h) {
+            const int k = 4 * q + parity + 2 * h;
+            if (k >= L - 1) break;
+            const int ns = A.D->pnum[k];
+            if (!bern(r[2 * h], J.thr_2pm) || ns == 1) continue;
+            const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
+            const int left = (k > 0) ? (int)row[k - 1] : 0;
+            const int right = row[k + 1];
+            // every synonym's score, the pair-score loads issued together (a
+            // missing neighbour adds nothing; stop codes score 0 in the table)
+            const bool hl = k > 0, hr = !((A.D->stop6 >> right) & 1u);
+            const float *rowl = J.cps6 + left * 64, *colr = J.cps6 + right;
+            float sc[MAX_SYN];
 #pragma unroll
-        for (int j = 1; j < MAX_SYN; ++j) curv = j == cur ? sc[j] : curv;
-        uint32_t cand = 0;  // synonyms scoring above the current codon, in order
+            for (int j = 0; j < MAX_SYN; ++j) {
+                const int g = A.D->code[base + (j < ns ? j : 0)];
+                const float a = gload(rowl + g), b = gload(colr + g * 64);
+                float v = 0.f;
+                v += hl ? a : 0.f;
+                v += hr ? b : 0.f;
+                sc[j] = v;
+            }
+            float curv = sc[0];
 #pragma unroll
-        for (int j = 0; j < MAX_SYN; ++j)
-            if (j < ns && j != cur && sc[j] > curv) cand |= 1u << j;
-        if (cand) {
-            A.cod[t] = (uint8_t)(base + nth_bit(cand, uniform_int(r[1], (uint32_t)__popc(cand))));
-            dirty[cds] = 1;
+            for (int j = 1; j < MAX_SYN; ++j) curv = j == cur ? sc[j] : curv;
+            uint32_t cand = 0;  // synonyms scoring above the current codon, in order
+#pragma unroll
+            for (int j = 0; j < MAX_SYN; ++j)
+                if (j < ns && j != cur && sc[j] > curv) cand |= 1u << j;
+            if (cand) {
+                set_syn(A, cds, k, nth_bit(cand, uniform_int(r[2 * h + 1], (uint32_t)__popc(cand))));
+                dirty[cds] = 1;
+            }
         }
     }
 }
 
-// mutationHSC with SELECT_UPPER_RANDOM, mutation.cuh:264-388
+// mutationHSC with SELECT_UPPER_RANDOM, mutation.cuh:264-388: the codons
+// k = 2i + parity below L, two per thread as in mut_cbp
 __device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L, parity = (int)(A.gen & 1u);
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        if ((k & 1) != parity) continue;
-        const int t = cds * L + k;
+    const int L = J.L, Ls = A.D->Ls, parity = (int)(A.gen & 1u);
+    const uint8_t *row = A.cd6 + cds * Ls;
+    const uint64_t stop6 = A.D->stop6;
+    for (int q = threadIdx.x; 4 * q + parity < L; q += blockDim.x) {
         uint32_t r[4];
-        philox(A.gen, A.child, (uint32_t)t, TAG_MUT, J.seed, r);
-        const int ns = A.D->pnum[k];
-        if (!bern(r[0], J.thr_2pm) || ns == 1) continue;
-        const int base = A.D->pbase[k], cur = A.cod[t] - base;
-        const int left = (k > 0) ? (int)A.cod[t - 1] : -1;
-        const int right = (k < L - 1) ? (int)A.cod[t + 1] : -1;
-        auto count = [&](int g) -> int {
-            int v = 0;
-            if (left >= 0) v += A.D->hs[left * NCODON + g];
-            if (right >= 0) v += A.D->hs[g * NCODON + right];
-            return v;
-        };
-        const int curv = count(base + cur);
-        uint32_t cand = 0;
-        for (int j = 0; j < ns; ++j)
-            if (j != cur && count(base + j) > curv) cand |= 1u << j;
-        if (cand) {
-            A.cod[t] = (uint8_t)(base + nth_bit(cand, uniform_int(r[1], (uint32_t)__popc(cand))));
-            dirty[cds] = 1;
+        pair_draws(A, cds, q, r);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 4 * q + parity + 2 * h;
+            if (k >= L) break;
+            const int ns = A.D->pnum[k];
+            if (!bern(r[2 * h], J.thr_2pm) || ns == 1) continue;
+            const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
+            const int left = (k > 0) ? (int)row[k - 1] : -1;
+            const int right = (k < L - 1) ? (int)row[k + 1] : -1;
+            auto count = [&](int j) -> int {
+                const int g = A.D->code[base + j];
+                int v = 0;
+                if (left >= 0) v += code_hidden(stop6, left, g);
+                if (right >= 0) v += code_hidden(stop6, g, right);
+                return v;
+            };
+            const int curv = count(cur);
+            uint32_t cand = 0;
+            for (int j = 0; j < ns; ++j)
+                if (j != cur && count(j) > curv) cand |= 1u << j;
+            if (cand) {
+                set_syn(A, cds, k, nth_bit(cand, uniform_int(r[2 * h + 1], (uint32_t)__popc(cand))));
+                dirty[cds] = 1;
+            }
         }
     }
-}
-
-// The draws of one codon slot: d[0] the mutation test (TAG_MUT word 0),
-// d[1..5] the shuffle (TAG_MUT words 1-3, TAG_MUTB words 0-1).  The second
-// block is drawn only for slots that pass the test (draws are functions of
-// their counters, so drawing late changes nothing).
-__device__ __forceinline__ uint32_t slot_test_draw(const MutArgs &A, uint32_t slot, uint32_t a[4]) {
-    philox(A.gen, A.child, slot, TAG_MUT, A.J->seed, a);
-    return a[0];
-}
-__device__ __forceinline__ void slot_shuffle_draws(const MutArgs &A, uint32_t slot, const uint32_t a[4], uint32_t d[5]) {
-    uint32_t b[4];
-    philox(A.gen, A.child, slot, TAG_MUTB, A.J->seed, b);
-    d[0] = a[1];
-    d[1] = a[2];
-    d[2] = a[3];
-    d[3] = b[0];
-    d[4] = b[1];
 }
 
 // indexArrayShuffling, utils.cuh:203-218: Fisher-Yates over the n <= 6
-// synonyms, draw d swapping entries n-1-d and uniform_int(draws[d], n-d).
-// The order is packed in 4-bit fields (entry i at bits 4i) so that it stays
-// in registers.
-__device__ __forceinline__ uint32_t shuffle(const uint32_t *draws, int n) {
+// synonyms, entries i = n-1 .. 1 swapped with j_i, the mixed-radix digits
+// (radix i+1) of uniform_int(x, n!).  Digits by multiplication: for r < 720,
+// (r * (65536/d + 1)) >> 16 is r / d for d = 2..6.  The order is packed in
+// 4-bit fields (entry i at bits 4i) so that it stays in registers.
+__device__ __forceinline__ uint32_t shuffle(uint32_t x, int n) {
+    const uint32_t fact = n == 6 ? 720u : n == 5 ? 120u : n == 4 ? 24u : n == 3 ? 6u : n == 2 ? 2u : 1u;
+    uint32_t r = uniform_int(x, fact);
     uint32_t perm = 0x543210u;
 #pragma unroll
-    for (int d = 0; d < 5; ++d) {
-        const int i = n - 1 - d;
-        if (i <= 0) break;
-        const int j = (int)uniform_int(draws[d], (uint32_t)(i + 1));
+    for (int i = 5; i > 0; --i) {
+        if (i >= n) continue;
+        const uint32_t d = (uint32_t)(i + 1), q = (r * (65536u / d + 1u)) >> 16, j = r - q * d;
+        r = q;
         const uint32_t vi = (perm >> (4 * i)) & 0xFu, vj = (perm >> (4 * j)) & 0xFu;
         perm &= ~((0xFu << (4 * i)) | (0xFu << (4 * j)));
         perm |= (vj << (4 * i)) | (vi << (4 * j));
@@ -636,56 +681,58 @@ __device__ __forceinline__ uint32_t shuffle(const uint32_t *draws, int n) {
 }
 __device__ __forceinline__ int perm_at(uint32_t perm, int j) { return (int)((perm >> (4 * j)) & 0xFu); }
 
-// mutationHD, mutation.cuh:390-534
+// mutationHD, mutation.cuh:390-534: codons of cds1, scored against all CDS
 __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L, m = J.m;
-    const uint8_t *nd = A.D->nd;
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const uint32_t slot = (uint32_t)(cds1 * L + k);
-        uint32_t a[4], d[5];
-        const int ns = A.D->pnum[k];
-        if (!bern(slot_test_draw(A, slot, a), J.thr_pm) || ns == 1) continue;
-        slot_shuffle_draws(A, slot, a, d);
-        const uint32_t order = shuffle(d, ns);
-        const int base = A.D->pbase[k], cur = A.cod[cds1 * L + k] - base;
-        const int g2 = A.cod[cds2 * L + k];
-        // The table rows of every candidate (the synonyms in shuffled order,
-        // slot MAX_SYN the current codon) are read before any is compared,
-        // so the loads of a CDS overlap: the smallest distance to the other
-        // CDS's codons at k (only cds1's codons change here, each by its own
-        // thread) and the distance to cds2's codon.
-        int row[MAX_SYN + 1], mn[MAX_SYN + 1], pr[MAX_SYN + 1];
+    const int L = J.L, m = J.m, Ls = A.D->Ls;
+    for (int q = threadIdx.x; 2 * q < L; q += blockDim.x) {
+        uint32_t r[4];
+        pair_draws(A, cds1, q, r);
 #pragma unroll
-        for (int j = 0; j <= MAX_SYN; ++j) {
-            row[j] = (base + (j == MAX_SYN ? cur : perm_at(order, j < ns ? j : ns - 1))) * NCODON;
-            mn[j] = 127;
-            pr[j] = nd[row[j] + g2];
-        }
-        for (int o = 0; o < m; ++o) {
-            if (o == cds1) continue;
-            const int oc = A.cod[o * L + k];
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * q + h;
+            if (k >= L) break;
+            const int ns = A.D->pnum[k];
+            if (!bern(r[2 * h], J.thr_pm) || ns == 1) continue;
+            const uint32_t order = shuffle(r[2 * h + 1], ns);
+            const int base = A.D->pbase[k], cur = A.syn[cds1 * Ls + k];
+            const int g2 = A.cd6[cds2 * Ls + k];
+            // every candidate's codes (the synonyms in shuffled order, slot
+            // MAX_SYN the current codon) first, then the smallest distance
+            // to the other CDS's codons at k (only cds1's codons change here,
+            // each by its own thread) and the distance to cds2's codon
+            int gc[MAX_SYN + 1], mn[MAX_SYN + 1], pr[MAX_SYN + 1];
 #pragma unroll
-            for (int j = 0; j <= MAX_SYN; ++j) mn[j] = min(mn[j], (int)nd[row[j] + oc]);
-        }
-        const int cur_pair = pr[MAX_SYN], cur_mhd = mn[MAX_SYN];
-        int best_pair = -1, best_mhd = -1, best = -1;
-#pragma unroll
-        for (int j = 0; j < MAX_SYN; ++j) {
-            const int sh = perm_at(order, j);
-            if (j >= ns || sh == cur) continue;
-            const int np_ = pr[j], nm = mn[j];
-            if (nm > cur_mhd && nm > best_mhd) {
-                best_mhd = nm;
-                best = sh;
-            } else if (best_mhd == -1 && nm == cur_mhd && np_ > cur_pair && np_ > best_pair) {
-                best_pair = np_;
-                best = sh;
+            for (int j = 0; j <= MAX_SYN; ++j) {
+                gc[j] = A.D->code[base + (j == MAX_SYN ? cur : perm_at(order, j < ns ? j : ns - 1))];
+                mn[j] = 127;
+                pr[j] = code_mismatch(gc[j], g2);
             }
-        }
-        if (best_mhd != -1 || best_pair != -1) {
-            A.cod[cds1 * L + k] = (uint8_t)(base + best);
-            dirty[cds1] = 1;
+            for (int o = 0; o < m; ++o) {
+                if (o == cds1) continue;
+                const int oc = A.cd6[o * Ls + k];
+#pragma unroll
+                for (int j = 0; j <= MAX_SYN; ++j) mn[j] = min(mn[j], code_mismatch(gc[j], oc));
+            }
+            const int cur_pair = pr[MAX_SYN], cur_mhd = mn[MAX_SYN];
+            int best_pair = -1, best_mhd = -1, best = -1;
+#pragma unroll
+            for (int j = 0; j < MAX_SYN; ++j) {
+                const int sh = perm_at(order, j);
+                if (j >= ns || sh == cur) continue;
+                const int np_ = pr[j], nm = mn[j];
+                if (nm > cur_mhd && nm > best_mhd) {
+                    best_mhd = nm;
+                    best = sh;
+                } else if (best_mhd == -1 && nm == cur_mhd && np_ > cur_pair && np_ > best_pair) {
+                    best_pair = np_;
+                    best = sh;
+                }
+            }
+            if (best_mhd != -1 || best_pair != -1) {
+                set_syn(A, cds1, k, best);
+                dirty[cds1] = 1;
+            }
         }
     }
 }
@@ -693,25 +740,29 @@ __device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
 // mutationGC3, mutation.cuh:536-653 (probability = the parent's GC3 deviation)
 __device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L;
+    const int L = J.L, Ls = A.D->Ls;
     const uint64_t thr = bern_threshold(dev);
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const uint32_t slot = (uint32_t)(cds * L + k);
-        uint32_t a[4], d[5];
-        const int ns = A.D->pnum[k];
-        if (!bern(slot_test_draw(A, slot, a), thr) || ns == 1) continue;
-        slot_shuffle_draws(A, slot, a, d);
-        const uint32_t order = shuffle(d, ns);
-        const int base = A.D->pbase[k], cur = A.cod[cds * L + k] - base;
-        const int curg = A.D->gc3[base + cur];
-        for (int j = 0; j < ns; ++j) {
-            const int sh = perm_at(order, j);
-            if (sh == cur) continue;
-            const int ng = A.D->gc3[base + sh];
-            if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
-                A.cod[cds * L + k] = (uint8_t)(base + sh);
-                dirty[cds] = 1;
-                break;
+    for (int q = threadIdx.x; 2 * q < L; q += blockDim.x) {
+        uint32_t r[4];
+        pair_draws(A, cds, q, r);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * q + h;
+            if (k >= L) break;
+            const int ns = A.D->pnum[k];
+            if (!bern(r[2 * h], thr) || ns == 1) continue;
+            const uint32_t order = shuffle(r[2 * h + 1], ns);
+            const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
+            const int curg = code_gc3(A.D->code[base + cur]);
+            for (int j = 0; j < ns; ++j) {
+                const int sh = perm_at(order, j);
+                if (sh == cur) continue;
+                const int ng = code_gc3(A.D->code[base + sh]);
+                if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
+                    set_syn(A, cds, k, sh);
+                    dirty[cds] = 1;
+                    break;
+                }
             }
         }
     }
@@ -721,11 +772,12 @@ __device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *
 // (mutation.cuh:843,896,947) compares the parent's SL objective with the
 // integer quotient l_new / ((n-4)/2); it only needs to know whether the CDS
 // still has any stem and whether it is one full-length hairpin, which two
-// O(n) workgroup checks decide.
-__device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t *dirty) {
+// O(n) workgroup checks over the CDS's codes decide.
+__device__ void mut_sl(const MutArgs &A, DesignShared &sh, int32_t *dirty) {
     const DevJob &J = *A.J;
-    const int L = J.L, n = 3 * L, tid = threadIdx.x, NT = blockDim.x;
+    const int L = J.L, n = 3 * L, Ls = A.D->Ls, tid = threadIdx.x, NT = blockDim.x;
     const int cds = sh.pidx[10];
+    const uint8_t *row = A.cd6 + cds * Ls;
     const int P0 = sh.ppql[0], Lst = sh.ppql[2];
     const int l_idx = P0 / 3, r_idx = (P0 + Lst - 1) / 3;
     const int mid = (r_idx + l_idx) / 2;
@@ -750,23 +802,16 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
         if (tid == 0) {
             uint32_t r[4];
             philox(A.gen, A.child, (uint32_t)t, TAG_SL, J.seed, r);
-            const int ns = A.D->pnum[pos], base = A.D->pbase[pos];
+            const int ns = A.D->pnum[pos];
             ctl[0] = 0;
-            ctl[1] = A.cod[cds * L + pos];
+            ctl[1] = A.syn[cds * Ls + pos];
             if (bern(r[0], J.thr_pm) && ns > 1) {
-                const int cur = A.cod[cds * L + pos] - base;
+                const int cur = A.syn[cds * Ls + pos];
                 int nw = (int)uniform_int(r[1], (uint32_t)(ns - 1));
                 if (nw == cur) ++nw;  // mutation.cuh:827 compares with ==
-                A.cod[cds * L + pos] = (uint8_t)(base + nw);
+                set_syn(A, cds, pos, nw);
                 ctl[0] = 1;
             }
-        }
-        lds_barrier();
-        for (int k = tid; k < L; k += NT) {
-            const int g = A.cod[cds * L + k];
-            ntb[3 * k] = A.D->nt[3 * g];
-            ntb[3 * k + 1] = A.D->nt[3 * g + 1];
-            ntb[3 * k + 2] = A.D->nt[3 * g + 2];
         }
         lds_barrier();
         // first and last position of each base seen by this thread (positions
@@ -774,7 +819,7 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
         int f[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, l4[4] = {-1, -1, -1, -1};
         int okA = 1, okB = 1, okC = 1;
         for (int i = tid; i < n; i += NT) {
-            const int b = ntb[i];
+            const int b = code_base(row, i);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (b == q) {
@@ -783,10 +828,10 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
                 }
         }
         for (int i = tid; i < den; i += NT) {
-            okA &= ((ntb[i] ^ ntb[n - 1 - i]) == 3);
+            okA &= ((code_base(row, i) ^ code_base(row, n - 1 - i)) == 3);
             if (n & 1) {
-                okB &= ((ntb[i] ^ ntb[n - 2 - i]) == 3);
-                okC &= ((ntb[1 + i] ^ ntb[n - 1 - i]) == 3);
+                okB &= ((code_base(row, i) ^ code_base(row, n - 2 - i)) == 3);
+                okC &= ((code_base(row, 1 + i) ^ code_base(row, n - 1 - i)) == 3);
             }
         }
         // wave minima, maxima and conjunctions (DPP, ballots), combined over
@@ -836,7 +881,7 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
             if (any) ctl[3] = is_full ? den : 0;
             const float rhs = (float)(den > 0 ? ctl[3] / den : 0);
             ctl[2] = parent_sl > rhs;
-            if (!ctl[2] && ctl[0]) A.cod[cds * L + pos] = (uint8_t)ctl[1];
+            if (!ctl[2] && ctl[0]) set_syn(A, cds, pos, ctl[1]);
             if (ctl[2] && ctl[0]) dirty[cds] = 1;
         }
         lds_barrier();
@@ -844,21 +889,20 @@ __device__ void mut_sl(const MutArgs &A, uint8_t *ntb, DesignShared &sh, int32_t
     }
 }
 
-// Bit planes of one CDS (codons g [L] as global codon indices, nucleotide
-// codes nt [64*3], both in LDS) into the job's plane buffer gp: bit p of the
-// sequence plane is position p, of the reverse-complement plane position
-// n-1-p complemented; positions past n are zero.  One wave ballots 64
-// positions; its lane 0 stores the four words.  Every lane must be active.
-__device__ void build_planes(const uint8_t *g, const uint8_t *nt, int L, uint2 *gp) {
+// Bit planes of one CDS (its row of codon codes in LDS) into the job's plane
+// buffer gp: bit p of the sequence plane is position p, of the
+// reverse-complement plane position n-1-p complemented; positions past n
+// are zero.  One wave ballots 64 positions; its lane 0 stores the four
+// words.  Every lane must be active.
+__device__ void build_planes(const uint8_t *row, int L, uint2 *gp) {
     const int n = 3 * L, W = scan_words(n), lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     uint2 *sp = gp, *rp = gp + W;
     for (int p0 = wid * 64; p0 < W * 32; p0 += nw * 64) {  // W is even: words w, w+1 < W
         const int pos = p0 + lane;
         int c = 0, rc = 0;
         if (pos < n) {
-            const int k = pos / 3, q = n - 1 - pos, kq = q / 3;
-            c = nt[3 * g[k] + (pos - 3 * k)];
-            rc = nt[3 * g[kq] + (q - 3 * kq)] ^ 3;
+            c = code_base(row, pos);
+            rc = code_base(row, n - 1 - pos) ^ 3;
         }
         const uint64_t bl = __ballot(c & 1), bh = __ballot(c >> 1), rl = __ballot(rc & 1), rh = __ballot(rc >> 1);
         if (lane == 0) {
@@ -877,6 +921,8 @@ __device__ void build_planes(const uint8_t *g, const uint8_t *nt, int L, uint2 *
 //              utils.cuh:1047-1104, then evaluation into buffer 0
 //   EVAL       evaluation of genome[0][d] into buffer 0
 //   OFFSPRING  survivor copy, mutation, evaluation (mutationKernel)
+// A design's genome row is m CDS rows of stride Ls; it moves between HBM and
+// LDS in 16-byte words.
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
@@ -886,36 +932,26 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     __shared__ DesignShared sh;
     const int2 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
-    const int L = J.L, m = J.m, mL = m * L, N = J.N;
+    const int L = J.L, m = J.m, Ls = J.Ls, N = J.N;
     const int tid = threadIdx.x, NT = blockDim.x;
-    uint8_t *cod = dyn + DT_BYTES;
-    uint8_t *pbase = cod + align16(mL);
-    uint8_t *pnum = pbase + align16(L);
-    uint8_t *ntb = pnum + align16(L);
-    CdsStat *cst = (CdsStat *)(ntb + align16(3 * L));
-    const DesignLds D{(const int64_t *)(dyn + DT_LW), dyn + DT_ND, dyn + DT_HS, dyn + DT_NT, dyn + DT_GC3, pbase, pnum};
+    const int G16 = (m * Ls) >> 4;  // 16-byte words of a design row
+    uint8_t *pbase = dyn + DL_POS;
+    uint8_t *pnum = pbase + Ls;
+    uint8_t *syn = pnum + Ls;
+    uint8_t *cd6 = syn + m * Ls;
+    CdsStat *cst = (CdsStat *)(cd6 + m * Ls);
+    const DesignLds D{(const int64_t *)(dyn + DL_LW6), dyn + DL_CODE, pbase, pnum, J.stop6, Ls};
 
-    // stage the tables (the layout of the table block matches the job's
-    // tables, each a multiple of 8 bytes) and the synonym ranges
-    for (int i = tid; i < DT_BYTES / 8; i += NT) {
-        int o = i;
-        const uint64_t *src;
-        if (o < DT_ND / 8)
-            src = (const uint64_t *)J.lw + o;
-        else if ((o -= DT_ND / 8) < 4096 / 8)
-            src = (const uint64_t *)J.nd + o;
-        else if ((o -= 4096 / 8) < 4096 / 8)
-            src = (const uint64_t *)J.hs + o;
-        else if ((o -= 4096 / 8) < 192 / 8)
-            src = (const uint64_t *)J.nt + o;
-        else
-            src = (const uint64_t *)J.gc3 + (o - 192 / 8);
-        ((uint64_t *)dyn)[i] = gload(src);
-    }
-    for (int k = tid; k < L; k += NT) {
-        const int a = gload(J.aa + k);
-        pbase[k] = (uint8_t)c_syn_start[a];
-        pnum[k] = (uint8_t)c_syn_num[a];
+    // stage the tables and the synonym ranges (padding positions: one
+    // synonym at codon 0)
+    if (tid < 64)
+        ((int64_t *)(dyn + DL_LW6))[tid] = gload(J.lw6 + tid);
+    else if (tid < 128)
+        dyn[DL_CODE + tid - 64] = gload(J.code + tid - 64);
+    for (int k = tid; k < Ls; k += NT) {
+        const int a = k < L ? (int)gload(J.aa + k) : -1;
+        pbase[k] = a >= 0 ? (uint8_t)c_syn_start[a] : 0;
+        pnum[k] = a >= 0 ? (uint8_t)c_syn_num[a] : 1;
     }
     if (tid < MAX_M) sh.dirty[tid] = (MODE != 2);
     uint32_t gen = 0;
@@ -927,38 +963,38 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         out_slot = N + it.y;
         parent = J.sorted[it.y];
     }
-    // The genome is read in the same phase as the tables: position k's
-    // synonym range was written above by this same thread (k = tid + j*NT in
-    // both loops), so no barrier is needed before it, and the loads overlap.
+    u32x4 *syn4 = (u32x4 *)syn;
     if (MODE == 0) {
+        // position k's synonym count was written above by this same thread
+        // (k = tid + j*NT in both loops), so no barrier is needed before it
         const int d = it.y;
         for (int c = 0; c < m; ++c)
-            for (int k = tid; k < L; k += NT) {
-                const int t = c * L + k, ns = pnum[k];
-                int s;
-                if (d == 0) {
+            for (int k = tid; k < Ls; k += NT) {
+                const int ns = pnum[k];
+                int s = 0;
+                if (k >= L) {
+                    s = 0;
+                } else if (d == 0) {
                     s = ns - 1;
                 } else {
                     uint32_t r[4];
-                    philox(0u, (uint32_t)d, (uint32_t)t, TAG_INIT, J.seed, r);
+                    philox(0u, (uint32_t)d, (uint32_t)(c * L + k), TAG_INIT, J.seed, r);
                     s = (int)uniform_int(r[0], (uint32_t)ns);
                 }
-                cod[t] = (uint8_t)(pbase[k] + s);
+                syn[c * Ls + k] = (uint8_t)s;
             }
     } else if (MODE == 1) {
-        const int8_t *sg = J.genome[0] + (size_t)it.y * mL;
-        for (int c = 0; c < m; ++c)
-            for (int k = tid; k < L; k += NT) cod[c * L + k] = (uint8_t)(pbase[k] + gload(sg + c * L + k));
+        const u32x4 *sg = (const u32x4 *)(J.genome[0] + (size_t)it.y * m * Ls);
+        for (int i = tid; i < G16; i += NT) syn4[i] = gload(sg + i);
     } else {
         const int c = it.y;
-        const int8_t *pg = J.genome[src] + (size_t)parent * mL;
-        int8_t *sg = J.genome[dst] + (size_t)c * mL;
-        for (int q = 0; q < m; ++q)
-            for (int k = tid; k < L; k += NT) {
-                const int8_t v = gload(pg + q * L + k);
-                gstore(sg + q * L + k, v);
-                cod[q * L + k] = (uint8_t)(pbase[k] + v);
-            }
+        const u32x4 *pg = (const u32x4 *)(J.genome[src] + (size_t)parent * m * Ls);
+        u32x4 *sg = (u32x4 *)(J.genome[dst] + (size_t)c * m * Ls);
+        for (int i = tid; i < G16; i += NT) {
+            const u32x4 v = gload(pg + i);
+            gstore(sg + i, v);
+            syn4[i] = v;
+        }
         // survivor records (copySolution, utils.cuh:1106-1147)
         if (tid < NOBJ) {
             const float v = gload(J.obj[src] + (size_t)parent * NOBJ + tid);
@@ -985,11 +1021,37 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
             gstore(J.np + c, 0);
             gstore(J.np + N + c, 0);
         }
-        lds_barrier();  // tables, codons and records
+    }
+    lds_barrier();  // tables, synonym ranges, genome row and records
+    // codon codes of the row, four per thread step; the padding positions
+    // are forced to synonym 0 and code 0
+    {
+        const int nwd = Ls >> 2;
+        const uint32_t *pb4 = (const uint32_t *)pbase;
+        uint32_t *s4 = (uint32_t *)syn, *c4 = (uint32_t *)cd6;
+        for (int c = 0; c < m; ++c)
+            for (int q = tid; q < nwd; q += NT) {
+                const uint32_t g = s4[c * nwd + q] + pb4[q];  // bytewise: synonyms < 6, first codons < 64
+                uint32_t s = s4[c * nwd + q], code = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (4 * q + b < L) {
+                        code |= (uint32_t)D.code[(g >> (8 * b)) & 0xFFu] << (8 * b);
+                    } else {
+                        s &= ~(0xFFu << (8 * b));
+                    }
+                }
+                s4[c * nwd + q] = s;
+                c4[c * nwd + q] = code;
+            }
+    }
+    lds_barrier();
+    if (MODE == 2) {
+        const int c = it.y;
         uint32_t r[4];
         philox(gen, (uint32_t)c, 0u, TAG_TYPE, J.seed, r);
         const int type = (int)uniform_int(r[0], (uint32_t)(NOBJ + 1));  // main.cu:172-178
-        const MutArgs A{&J, &D, cod, gen, (uint32_t)c};
+        const MutArgs A{&J, &D, syn, cd6, gen, (uint32_t)c};
         switch (type) {  // main.cu:181-211
         case 0: mut_random(A, sh.dirty); break;
         case 1: mut_cai(A, sh.pidx[0], sh.dirty); break;
@@ -999,20 +1061,19 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         case 5:
             if (sh.pidx[9] == GC_UP || sh.pidx[9] == GC_DOWN) mut_gc3(A, sh.pidx[8], sh.pidx[9], sh.pobj[4], sh.dirty);
             break;
-        default: mut_sl(A, ntb, sh, sh.dirty); break;
+        default: mut_sl(A, sh, sh.dirty); break;
         }
+        lds_barrier();
     }
-    lds_barrier();
 
     float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
     __shared__ float s_obj[NOBJ];
     __shared__ int32_t s_idx[2 * NOBJ];
-    evaluate(J, D, cod, cst, MODE == 2, sh, s_obj, s_idx);
-    int8_t *og = J.genome[dst] + (size_t)out_slot * mL;
-    for (int c = 0; c < m; ++c)
-        for (int k = tid; k < L; k += NT) gstore(og + c * L + k, (int8_t)(cod[c * L + k] - pbase[k]));
+    evaluate(J, D, cd6, cst, MODE == 2, sh, s_obj, s_idx);
+    u32x4 *og = (u32x4 *)(J.genome[dst] + (size_t)out_slot * m * Ls);
+    for (int i = tid; i < G16; i += NT) gstore(og + i, syn4[i]);
     for (int c = 0; c < m; ++c)  // planes of the changed CDS for k_stem
-        if (sh.dirty[c]) build_planes(cod + c * L, D.nt, L, J.planes + ((size_t)out_slot * m + c) * 2 * scan_words(3 * L));
+        if (sh.dirty[c]) build_planes(cd6 + c * Ls, L, J.planes + ((size_t)out_slot * m + c) * 2 * scan_words(3 * L));
     if (tid < NOBJ - 1) gstore(oobj + tid, s_obj[tid]);  // objective 5: k_stem_finish
     if (tid < 2 * NOBJ && tid != 10) gstore(J.oidx[dst] + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
     if (tid < m) gstore_stat(J.cst[dst] + (size_t)out_slot * m + tid, cst[tid]);

@@ -78,20 +78,25 @@ __host__ __device__ inline size_t niche_lds_need(int P, int R) {
 
 // Device view of one job.  Pools ping-pong on generation parity: generation
 // g reads buffer g&1 and writes buffer (g+1)&1.
+// Stride of a CDS row of L codons in the genome pools and in k_design's LDS:
+// L rounded up to 16 bytes, so rows move in 16-byte words.
+__host__ __device__ constexpr int row_stride(int L) { return (L + 15) & ~15; }
+
+// A codon's code is its three nucleotides A=0 C=1 G=2 U=3 in six bits,
+// first << 4 | second << 2 | third; the 64 codons have the 64 codes.
 struct DevJob {
-    int32_t L, m, N, P, R, W, den_sl, pad0;
+    int32_t L, m, N, P, R, W, den_sl;
+    int32_t Ls;            // stride of a CDS row in the genome pools: L rounded up to 16 bytes
     uint64_t seed, thr_pm, thr_2pm;
+    uint64_t stop6;        // bit c set when code c is a stop codon (the table's last three codons)
     float gc3_ref, pad1;
     const int8_t *aa;      // [L]
-    const uint8_t *nt;     // [64*3] nucleotide codes A=0 C=1 G=2 U=3
-    const int64_t *lw;     // [64]
-    const int64_t *cpfx;   // [61*61]
-    const float *cps;      // [61*61]
-    const uint8_t *hs;     // [64*64] hidden stops across junction a|b
-    const uint8_t *nd;     // [64*64] nucleotide mismatches
-    const uint8_t *gc3;    // [64]
+    const uint8_t *code;   // [64] code of each codon of the table
+    const int64_t *lw6;    // [64] 2^40-scaled log weights by code
+    const int64_t *cpfx6;  // [64*64] 2^40-scaled pair scores by codes (0 where either is a stop)
+    const float *cps6;     // [64*64] pair scores by codes (0 where either is a stop)
     const float *refpts;   // [R*6]
-    int8_t *genome[2];     // [P][m][L]
+    int8_t *genome[2];     // [P][m][Ls] synonym indices, CDS rows padded with zeros to Ls
     float *obj[2];         // [P][6]
     int8_t *oidx[2];       // [P][12]
     int32_t *pql[2];       // [P][4] (p, q, l, 0)

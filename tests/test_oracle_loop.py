@@ -126,7 +126,7 @@ P_MIN = 1e-6  # chi-square p-value bound of the uniformity checks
 # deliberate mistakes: (function, text, replacement) in oracle/cd_oracle.cpp
 PLANTS = {
     "random_keeps_equal_draw": [("void mut_random(", "if (nw >= cur) ++nw;", "if (nw > cur) ++nw;")],
-    "cai_always_top": [("void mut_cai(", "cur = (int8_t)(cur + 1 + (int)uniform_int(r[1], (uint32_t)(ns - 1 - cur)));",
+    "cai_always_top": [("void mut_cai(", "cur = (int8_t)(cur + 1 + (int)uniform_int(pick, (uint32_t)(ns - 1 - cur)));",
                         "cur = (int8_t)(ns - 1);")],
     "cbp_rate_pm": [("void mut_cbp(", "c.thr_2pm", "c.thr_pm")],
     "hsc_rate_pm": [("void mut_hsc(", "c.thr_2pm", "c.thr_pm")],
