@@ -36,12 +36,23 @@ __host__ __device__ constexpr int align16(int x) { return (x + 15) & ~15; }
 // codes cd6 [m][Ls], then the CDS statistics [m].  Rows are 16-byte aligned,
 // so the genome moves between HBM and LDS in 16-byte words and four codes
 // share a 32-bit word in the Hamming sums.
+// After the genome row is stored, the syn area stages the forward bit planes
+// of the changed CDS (plane_stage_bytes each), so its stride per CDS is the
+// larger of the two.
 constexpr int DL_LW6 = 0, DL_CODE = 512, DL_POS = 576;
 static_assert(DL_POS % 16 == 0, "design LDS rows are 16-byte aligned");
 
+// forward planes of one CDS staged in LDS: scan_words(3L) words and one zero
+// word past them (the reverse-complement windows read one word ahead)
+__host__ __device__ inline int plane_stage_bytes(int L) { return align16((scan_words(3 * L) + 1) * 8); }
+__host__ __device__ inline int syn_stride(int L) {
+    const int a = row_stride(L), b = plane_stage_bytes(L);
+    return a > b ? a : b;
+}
+
 size_t design_lds_bytes(int L, int m) {
-    const size_t Ls = row_stride(L);
-    return (size_t)DL_POS + 2 * Ls + 2 * (size_t)m * Ls + align16(m * (int)sizeof(CdsStat)) + 16;
+    const size_t Ls = row_stride(L), SA = syn_stride(L);
+    return (size_t)DL_POS + 2 * Ls + (size_t)m * (SA + Ls) + align16(m * (int)sizeof(CdsStat)) + 16;
 }
 
 // one CDS's bit planes: the sequence and its reverse complement, each
@@ -64,6 +75,7 @@ struct DesignShared {
     uint64_t redu[16];
     int32_t misc[16];
     int32_t slw[4 * 11];  // mut_sl: per wave, first [4], last [4], full [3]
+    int32_t dlist[MAX_M];  // the changed CDS, in order
 };
 
 // CdsStat records in global memory, field by field through the global
@@ -889,29 +901,65 @@ __device__ void mut_sl(const MutArgs &A, DesignShared &sh, int32_t *dirty) {
     }
 }
 
-// Bit planes of one CDS (its row of codon codes in LDS) into the job's plane
-// buffer gp: bit p of the sequence plane is position p, of the
-// reverse-complement plane position n-1-p complemented; positions past n
-// are zero.  One wave ballots 64 positions; its lane 0 stores the four
-// words.  Every lane must be active.
-__device__ void build_planes(const uint8_t *row, int L, uint2 *gp) {
-    const int n = 3 * L, W = scan_words(n), lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint2 *sp = gp, *rp = gp + W;
-    for (int p0 = wid * 64; p0 < W * 32; p0 += nw * 64) {  // W is even: words w, w+1 < W
-        const int pos = p0 + lane;
-        int c = 0, rc = 0;
-        if (pos < n) {
-            c = code_base(row, pos);
-            rc = code_base(row, n - 1 - pos) ^ 3;
+// Bit planes of the changed CDS into the job's plane buffer: bit p of the
+// sequence plane is position p, of the reverse-complement plane position
+// n-1-p complemented; positions past n are zero; a word is (low bits, high
+// bits) of 32 positions.  The forward words are built 32 codons (three words)
+// per lane from the codes, four codons per 32-bit step, and staged in LDS
+// (stage + c * stride for CDS c); the reverse-complement word j is the
+// forward window of bits n-32-32j .. n-1-32j, bit-reversed and inverted.
+// rows: the CDS's code rows (stride Ls); dl: the changed CDS, nd of them.
+__device__ void build_planes(const uint8_t *rows, int Ls, int L, const int *dl, int nd, uint2 *stage, int stride,
+                             uint2 *gbase) {
+    const int n = 3 * L, W = scan_words(n), ng = (W + 2) / 3, tid = threadIdx.x, NT = blockDim.x;
+    const int nwd = Ls >> 2;
+    for (int i = tid; i < nd * ng; i += NT) {
+        const int e = i / ng, g = i - e * ng, c = dl[e];
+        const uint32_t *row = (const uint32_t *)(rows + c * Ls);
+        uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {  // codons 32g + 4t .. 32g + 4t + 3 (zero past the row)
+            const int q = 8 * g + t;
+            const uint32_t d = q < nwd ? row[q] : 0u;
+            // per byte the three bases' low (high) bits in position order
+            const uint32_t l = ((d >> 4) & 0x01010101u) | ((d >> 1) & 0x02020202u) | ((d << 2) & 0x04040404u);
+            const uint32_t h = ((d >> 5) & 0x01010101u) | ((d >> 2) & 0x02020202u) | ((d << 1) & 0x04040404u);
+            const uint32_t l12 = (l & 7u) | ((l >> 5) & 0x38u) | ((l >> 10) & 0x1C0u) | ((l >> 15) & 0xE00u);
+            const uint32_t h12 = (h & 7u) | ((h >> 5) & 0x38u) | ((h >> 10) & 0x1C0u) | ((h >> 15) & 0xE00u);
+            const int b = 12 * t, w = b >> 5, o = b & 31;  // compile-time after unrolling
+            lo[w] |= l12 << o;
+            hi[w] |= h12 << o;
+            if (o > 20) {
+                lo[w + 1] |= l12 >> (32 - o);
+                hi[w + 1] |= h12 >> (32 - o);
+            }
         }
-        const uint64_t bl = __ballot(c & 1), bh = __ballot(c >> 1), rl = __ballot(rc & 1), rh = __ballot(rc >> 1);
-        if (lane == 0) {
-            const int w = p0 >> 5;
-            gstore((uint64_t *)(sp + w), (uint64_t)((bl & 0xFFFFFFFFull) | (bh << 32)));
-            gstore((uint64_t *)(sp + w + 1), (uint64_t)((bl >> 32) | (bh & 0xFFFFFFFF00000000ull)));
-            gstore((uint64_t *)(rp + w), (uint64_t)((rl & 0xFFFFFFFFull) | (rh << 32)));
-            gstore((uint64_t *)(rp + w + 1), (uint64_t)((rl >> 32) | (rh & 0xFFFFFFFF00000000ull)));
+        uint2 *st = stage + (size_t)e * stride;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+            if (3 * g + u < W) st[3 * g + u] = make_uint2(lo[u], hi[u]);
+        if (g == 0) st[W] = make_uint2(0u, 0u);
+    }
+    lds_barrier();
+    for (int i = tid; i < nd * W; i += NT) {
+        const int e = i / W, j = i - e * W, c = dl[e];
+        const uint2 *st = stage + (size_t)e * stride;
+        uint2 *sp = gbase + (size_t)c * 2 * W, *rp = sp + W;
+        gstore((uint64_t *)(sp + j), (uint64_t)st[j].x | ((uint64_t)st[j].y << 32));
+        const int s0 = n - 32 - 32 * j;  // first bit of the forward window
+        uint32_t wl = 0, wh = 0;
+        if (s0 >= 0) {
+            const uint2 a = st[s0 >> 5], b = st[(s0 >> 5) + 1];
+            wl = __builtin_amdgcn_alignbit(b.x, a.x, (uint32_t)(s0 & 31));
+            wh = __builtin_amdgcn_alignbit(b.y, a.y, (uint32_t)(s0 & 31));
+        } else if (s0 > -32) {
+            wl = st[0].x << (-s0);
+            wh = st[0].y << (-s0);
         }
+        const int cnt = n - 32 * j;  // positions of word j below n
+        const uint32_t mask = cnt >= 32 ? ~0u : (cnt <= 0 ? 0u : (1u << cnt) - 1u);
+        const uint32_t rl = ~__builtin_bitreverse32(wl) & mask, rh = ~__builtin_bitreverse32(wh) & mask;
+        gstore((uint64_t *)(rp + j), (uint64_t)rl | ((uint64_t)rh << 32));
     }
 }
 
@@ -935,10 +983,11 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     const int L = J.L, m = J.m, Ls = J.Ls, N = J.N;
     const int tid = threadIdx.x, NT = blockDim.x;
     const int G16 = (m * Ls) >> 4;  // 16-byte words of a design row
+    const int SA = syn_stride(L);
     uint8_t *pbase = dyn + DL_POS;
     uint8_t *pnum = pbase + Ls;
-    uint8_t *syn = pnum + Ls;
-    uint8_t *cd6 = syn + m * Ls;
+    uint8_t *syn = pnum + Ls;       // rows at stride Ls; the planes' stage afterwards (stride SA)
+    uint8_t *cd6 = syn + m * SA;
     CdsStat *cst = (CdsStat *)(cd6 + m * Ls);
     const DesignLds D{(const int64_t *)(dyn + DL_LW6), dyn + DL_CODE, pbase, pnum, J.stop6, Ls};
 
@@ -1072,8 +1121,16 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
     evaluate(J, D, cd6, cst, MODE == 2, sh, s_obj, s_idx);
     u32x4 *og = (u32x4 *)(J.genome[dst] + (size_t)out_slot * m * Ls);
     for (int i = tid; i < G16; i += NT) gstore(og + i, syn4[i]);
-    for (int c = 0; c < m; ++c)  // planes of the changed CDS for k_stem
-        if (sh.dirty[c]) build_planes(cd6 + c * Ls, L, J.planes + ((size_t)out_slot * m + c) * 2 * scan_words(3 * L));
+    {  // planes of the changed CDS for k_stem, staged where the syn rows were
+        int nd = 0;
+        for (int c = 0; c < m; ++c) {
+            if (tid == 0 && sh.dirty[c]) sh.dlist[nd] = c;
+            nd += sh.dirty[c] != 0;
+        }
+        lds_barrier();  // the syn rows are stored and the list is written
+        build_planes(cd6, Ls, L, sh.dlist, nd, (uint2 *)syn, SA / 8,
+                     J.planes + (size_t)out_slot * m * 2 * scan_words(3 * L));
+    }
     if (tid < NOBJ - 1) gstore(oobj + tid, s_obj[tid]);  // objective 5: k_stem_finish
     if (tid < 2 * NOBJ && tid != 10) gstore(J.oidx[dst] + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
     if (tid < m) gstore_stat(J.cst[dst] + (size_t)out_slot * m + tid, cst[tid]);
