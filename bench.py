@@ -372,6 +372,7 @@ def bench_served(cdp, args):
         queued.append(r["times"]["queued"])
     wall = time.perf_counter() - t0
     designs = sum(a.spec.N * a.spec.generations for a in arrivals)
+    st = srv.stats()
     srv.close()
     log(f"served: done in {wall:.1f} s")
     # the same jobs one at a time through the per-instance path (a sample)
@@ -396,6 +397,12 @@ def bench_served(cdp, args):
                                 "max": round(float(np.max(ttf)), 3)},
             "queued_s": {"p50": round(float(np.percentile(queued, 50)), 3),
                          "p99": round(float(np.percentile(queued, 99)), 3)},
+            "ticks": st["generations"],
+            "tick_gpu_s": round(st["tick_gpu_s"], 3),
+            "gpu_busy_frac": round(st["tick_gpu_s"] / wall, 4),
+            "resident_set_changes": st["rebuilds"], "rebuild_s": round(st["rebuild_s"], 3),
+            "worker": {"busy_s": round(st["worker_busy_s"], 3), "launch_s": round(st["launch_s"], 3),
+                       "wait_s": round(st["wait_s"], 3)},
             "sequential_per_instance": {"designs_s": round(seq_rate, 1),
                                         "sample": f"every 10th job ({len(range(0, len(arrivals), 10))} jobs) "
                                                   f"run alone, GPU time by HIP events",

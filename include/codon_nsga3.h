@@ -76,7 +76,9 @@ typedef struct cd_job cd_job;
 typedef struct cd_batch cd_batch;
 
 /* Allocate a job's device state and upload its tables and reference points
- * (replaces main.cu:558-678, including combinationRepetition). */
+ * (replaces main.cu:558-678, including combinationRepetition).  The memory
+ * comes from the device's stream-ordered pool, which keeps what
+ * cd_job_destroy returns for later jobs. */
 int cd_job_create(const cd_job_desc *desc, cd_job **out);
 
 /* Random initial population, design 0 with the best-CAI codons everywhere,
@@ -195,13 +197,15 @@ int cd_batch_shapes(cd_batch *batch, cd_shapes *get, const cd_shapes *set);
 
 /* Served form (no counterpart in the reference, which runs one job per
  * process): continuous batching of jobs of different protein lengths,
- * population sizes and CDS counts.  Requests queue in arrival order; a
- * worker thread admits the oldest one whenever its device bytes
- * (cd_job_device_bytes) fit the budget beside the resident jobs and fewer
- * than max_jobs are resident, and advances every resident job one
- * generation per launch sequence.  Populations, fronts and generator state
- * stay on the device until the job's generation budget is spent; then its
- * final pool is copied to host memory and its device memory is freed.  A
+ * population sizes and CDS counts.  Requests queue in arrival order; an
+ * admitting thread admits the oldest one whenever its device bytes
+ * (cd_job_device_bytes) fit the budget beside the jobs holding memory and
+ * fewer than max_jobs hold memory, and initialises it on its own stream
+ * while the resident jobs keep running; a worker thread advances every
+ * resident job one generation per tick (one captured hipGraph per resident
+ * set); a collecting thread copies a finished job's final pool to host
+ * memory and frees its device memory.  Populations, fronts and generator
+ * state stay on the device until the job's generation budget is spent.  A
  * job's result is the same alone and in any mix.  Thread-safe: submit, wait
  * and result may be called from any thread. */
 typedef struct cd_server cd_server;
@@ -218,8 +222,12 @@ int cd_server_wait(cd_server *server, int64_t ticket, double timeout_s, int32_t 
  * seconds from submission to result (time to front). */
 int cd_server_result(cd_server *server, int64_t ticket, int8_t *genome, float *obj, int8_t *idx, int32_t *pql,
                      int32_t *sorted, int32_t *front_of, double *times);
-/* stats [5] = queued requests, resident jobs, device bytes in use,
- * generations run (launch sequences), designs produced. */
+/* stats [11] = queued requests, resident jobs, device bytes in use, ticks
+ * run, designs produced, GPU time of the ticks (microseconds, HIP events
+ * around each tick), worker time while jobs were resident (microseconds),
+ * resident-set changes (each captures the tick again), time spent on them,
+ * worker time launching ticks, worker time waiting for the tick two back
+ * (microseconds). */
 int cd_server_stats(cd_server *server, int64_t *stats);
 /* Stops the worker; unfinished requests are dropped. */
 int cd_server_destroy(cd_server *server);

@@ -322,10 +322,11 @@ class Server:
         return out
 
     def stats(self):
-        s = np.zeros(5, np.int64)
+        s = np.zeros(11, np.int64)
         _check(lib().cd_server_stats(self.h, s))
         return {"queued": int(s[0]), "resident": int(s[1]), "bytes": int(s[2]), "generations": int(s[3]),
-                "designs": int(s[4])}
+                "designs": int(s[4]), "tick_gpu_s": s[5] / 1e6, "worker_busy_s": s[6] / 1e6,
+                "rebuilds": int(s[7]), "rebuild_s": s[8] / 1e6, "launch_s": s[9] / 1e6, "wait_s": s[10] / 1e6}
 
     def close(self):
         if getattr(self, "h", None):

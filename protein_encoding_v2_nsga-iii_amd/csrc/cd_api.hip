@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -221,6 +222,61 @@ int validate(const cd_job_desc *d) {
     return CD_OK;
 }
 
+// Pinned host staging for genome rows, which the device holds at a padded
+// stride Ls: reads copy the padded block in one transfer and drop the pads on
+// the host, writes add them on the host and copy in one transfer (a pitched
+// copy to or from pageable memory runs row by row).  Grown on demand and kept
+// for the life of the process; callers serialise on its lock.
+struct Staging {
+    std::mutex mu;
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {  // with mu held
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t c = std::max(n + n / 4, (size_t)1 << 20);
+        hipError_t e = hipHostMalloc((void **)&p, c, hipHostMallocDefault);
+        if (e == hipSuccess) cap = c;
+        return e;
+    }
+};
+Staging g_rows;
+
+// Memory a job frees stays in the device's stream-ordered pool for the next
+// job instead of going back to the system at each synchronisation (set once
+// per device).
+hipError_t keep_pool_memory() {
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[dev]) return hipSuccess;
+    hipMemPool_t pool;
+    e = hipDeviceGetDefaultMemPool(&pool, dev);
+    uint64_t keep = ~0ull;
+    if (e == hipSuccess) e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    if (e == hipSuccess) done[dev] = true;
+    return e;
+}
+
+// Regions of a job's allocation to upload at creation; the rest of the
+// allocation is zero (cleared on the device).
+struct Upload {
+    std::vector<char> bytes;
+    std::vector<std::pair<size_t, std::pair<size_t, size_t>>> parts;  // (device offset, (host offset, size))
+    template <class T>
+    void add(size_t dev_off, const T *src, size_t count) {
+        const size_t at = bytes.size(), n = count * sizeof(T);
+        bytes.resize(at + n);
+        std::memcpy(bytes.data() + at, src, n);
+        parts.push_back({dev_off, {at, n}});
+    }
+};
+
 int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out) {
     cd_job *J = new cd_job();
     J->selection_only = sel_only;
@@ -233,15 +289,31 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     J->W = (J->P + 63) / 64;
     work_counts(J->P, J->W, &J->n_dom, &J->n_asc);
     Layout o = layout(J->L, J->m, J->N, J->P, J->R, J->W, J->n_dom, J->n_asc, sel_only);
-    hipError_t e = hipMalloc(&J->mem, o.total);
+    HostTables t;
+    if (!sel_only) {
+        const int rc = build_tables(d->tables, t);
+        if (rc != CD_OK) {
+            delete J;
+            return rc;
+        }
+    }
+    // the job's own stream does not wait for (or hold up) work on other
+    // streams, such as a served tick that is running while this job is
+    // admitted
+    // The allocation comes from the device's stream-ordered pool: freeing it
+    // (cd_job_destroy) does not wait for every stream of the device, which
+    // hipFree does, so jobs can come and go while others run.
+    hipError_t e = hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = keep_pool_memory();
+    if (e == hipSuccess) e = hipMallocAsync(&J->mem, o.total, J->stream);
     if (e != hipSuccess) {
+        if (J->stream) (void)hipStreamDestroy(J->stream);
         delete J;
-        return fail(CD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return fail(CD_E_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
     }
     J->mem_bytes = o.total;
     char *base = (char *)J->mem;
-    e = hipMemset(J->mem, 0, o.total);
-    std::vector<char> host(o.total, 0);  // staged image of the tables and descriptors
+    Upload up;
     cd::DevJob &h = J->h;
     h.L = J->L;
     h.m = J->m;
@@ -256,18 +328,11 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.thr_2pm = bern_threshold_host(d->Pm * 2.0f);
     h.gc3_ref = d->tables.gc3_ref;
     if (!sel_only) {
-        HostTables t;
-        int rc = build_tables(d->tables, t);
-        if (rc != CD_OK) {
-            (void)hipFree(J->mem);
-            delete J;
-            return rc;
-        }
-        std::memcpy(&host[o.aa], d->aa, J->L);
-        std::memcpy(&host[o.code], t.code, sizeof(t.code));
-        std::memcpy(&host[o.lw6], t.lw6, sizeof(t.lw6));
-        std::memcpy(&host[o.cpfx6], t.cpfx6, sizeof(t.cpfx6));
-        std::memcpy(&host[o.cps6], t.cps6, sizeof(t.cps6));
+        up.add(o.aa, d->aa, J->L);
+        up.add(o.code, t.code, 64);
+        up.add(o.lw6, t.lw6, 64);
+        up.add(o.cpfx6, t.cpfx6, 64 * 64);
+        up.add(o.cps6, t.cps6, 64 * 64);
         h.aa = (const int8_t *)(base + o.aa);
         h.code = (const uint8_t *)(base + o.code);
         h.lw6 = (const int64_t *)(base + o.lw6);
@@ -280,17 +345,18 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
             h.pql[b] = (int32_t *)(base + o.pql[b]);
             h.cst[b] = (cd::CdsStat *)(base + o.cst[b]);
         }
-        int2 *off = (int2 *)&host[o.off];
+        std::vector<int2> off(J->N);
         for (int c = 0; c < J->N; ++c) off[c] = make_int2(0, c);
+        up.add(o.off, off.data(), off.size());
         J->d_off = (int2 *)(base + o.off);
-        J->d_slist = (uint32_t *)(base + o.slist);  // the count in d_sctl starts at zero with the image
+        J->d_slist = (uint32_t *)(base + o.slist);  // the count in d_sctl starts at zero with the cleared allocation
         J->d_sctl = (int32_t *)(base + o.sctl);
         J->lds = cd::design_lds_bytes(J->L, J->m);
         J->stem_lds = cd::stem_lds_bytes(J->L);
         J->stem_parts = cd::stem_parts_of(J->L);
     }
-    auto ref = das_dennis(d->H);
-    std::memcpy(&host[o.ref], ref.data(), ref.size() * sizeof(float));
+    const auto ref = das_dennis(d->H);
+    up.add(o.ref, ref.data(), ref.size());
     h.refpts = (const float *)(base + o.ref);
     h.obj[0] = (float *)(base + o.obj[0]);
     h.obj[1] = (float *)(base + o.obj[1]);
@@ -305,7 +371,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.niche = (int32_t *)(base + o.niche);
     h.flag = (uint8_t *)(base + o.flag);
     h.st = (cd::SelState *)(base + o.st);
-    std::memcpy(&host[o.self], &h, sizeof(h));
+    up.add(o.self, &h, 1);
     J->d_self = (cd::DevJob *)(base + o.self);
     if (!sel_only) {  // the tuned shapes of this kind of job, if any
         cd_job *self = J;
@@ -318,21 +384,27 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
         std::vector<int4> dom;
         std::vector<int2> asc;
         job_lists(J->P, J->W, J->shapes, dom, asc);
-        std::memcpy(&host[o.domi], dom.data(), dom.size() * sizeof(int4));
-        std::memcpy(&host[o.asc], asc.data(), asc.size() * sizeof(int2));
+        up.add(o.domi, dom.data(), dom.size());
+        up.add(o.asc, asc.data(), asc.size());
         J->n_dom = (int)dom.size();
         J->n_asc = (int)asc.size();
     }
     J->d_dom = (int4 *)(base + o.domi);
     J->d_asc = (int2 *)(base + o.asc);
-    if (e == hipSuccess) e = hipMemcpy(J->mem, host.data(), o.total, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipStreamCreate(&J->stream);
+    e = hipMemsetAsync(J->mem, 0, o.total, J->stream);
+    for (const auto &pt : up.parts)
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(base + pt.first, up.bytes.data() + pt.second.first, pt.second.second,
+                               hipMemcpyHostToDevice, J->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(J->stream);  // the staged bytes are released below
     if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
     if (e == hipSuccess && J->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(J->stem_lds);
     J->niche_lds = cd::niche_lds_bytes(J->P, J->R);
     if (e == hipSuccess && J->niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(J->niche_lds);
     if (e != hipSuccess) {
-        (void)hipFree(J->mem);
+        (void)hipFreeAsync(J->mem, J->stream);
+        (void)hipStreamSynchronize(J->stream);
+        (void)hipStreamDestroy(J->stream);
         delete J;
         return fail(CD_E_DEVICE, std::string("job setup: ") + hipGetErrorString(e));
     }
@@ -650,11 +722,21 @@ int cd_job_sync(cd_job *J) {
     return CD_OK;
 }
 
+// Reads of a job's device state go through its own (non-blocking) stream,
+// never the legacy null stream, so that they neither wait for nor disturb
+// other streams -- for instance a served tick being captured in another
+// thread.
+static hipError_t read_state(cd_job *J, cd::SelState &st) {
+    hipError_t e = sync_job(J);
+    if (e == hipSuccess) e = hipMemcpyAsync(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost, J->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+    return e;
+}
+
 int cd_job_generation(cd_job *J, int32_t *generation) {
     if (!J || !generation) return fail(CD_E_INVALID, "null argument");
-    HIPCHK(sync_job(J));
     cd::SelState st;
-    HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(read_state(J, st));
     *generation = (int32_t)st.gen;
     return CD_OK;
 }
@@ -663,22 +745,29 @@ int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql
                 float *state) {
     if (!J || J->selection_only) return fail(CD_E_INVALID, "null job");
     if (!J->inited) return fail(CD_E_STATE, "cd_job_read before cd_job_init");
-    HIPCHK(sync_job(J));
     cd::SelState st;
-    HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(read_state(J, st));
     const int b = (int)(st.gen & 1u), P = J->P;
+    const size_t rows = (size_t)P * J->m, L = J->L, Ls = J->h.Ls;
+    hipStream_t s = J->stream;
+    std::unique_lock<std::mutex> stage(g_rows.mu, std::defer_lock);
+    if (genome) {  // the padded rows in one transfer, compacted below
+        stage.lock();
+        HIPCHK(g_rows.reserve(rows * Ls));
+        HIPCHK(hipMemcpyAsync(g_rows.p, J->h.genome[b], rows * Ls, hipMemcpyDeviceToHost, s));
+    }
+    if (obj) HIPCHK(hipMemcpyAsync(obj, J->h.obj[b], (size_t)P * 6 * 4, hipMemcpyDeviceToHost, s));
+    if (idx) HIPCHK(hipMemcpyAsync(idx, J->h.oidx[b], (size_t)P * 12, hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> t(pql ? (size_t)P * 4 : 0);
+    if (pql) HIPCHK(hipMemcpyAsync(t.data(), J->h.pql[b], t.size() * 4, hipMemcpyDeviceToHost, s));
+    if (sorted) HIPCHK(hipMemcpyAsync(sorted, J->h.sorted, (size_t)J->N * 4, hipMemcpyDeviceToHost, s));
+    if (front_of) HIPCHK(hipMemcpyAsync(front_of, J->h.front_of, (size_t)P * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     if (genome)  // CDS rows of L codons from rows of stride Ls
-        HIPCHK(hipMemcpy2D(genome, J->L, J->h.genome[b], J->h.Ls, J->L, (size_t)P * J->m, hipMemcpyDeviceToHost));
-    if (obj) HIPCHK(hipMemcpy(obj, J->h.obj[b], (size_t)P * 6 * 4, hipMemcpyDeviceToHost));
-    if (idx) HIPCHK(hipMemcpy(idx, J->h.oidx[b], (size_t)P * 12, hipMemcpyDeviceToHost));
-    if (pql) {
-        std::vector<int32_t> t((size_t)P * 4);
-        HIPCHK(hipMemcpy(t.data(), J->h.pql[b], t.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t r = 0; r < rows; ++r) std::memcpy(genome + r * L, g_rows.p + r * Ls, L);
+    if (pql)
         for (int i = 0; i < P; ++i)
             for (int k = 0; k < 3; ++k) pql[i * 3 + k] = t[(size_t)i * 4 + k];
-    }
-    if (sorted) HIPCHK(hipMemcpy(sorted, J->h.sorted, (size_t)J->N * 4, hipMemcpyDeviceToHost));
-    if (front_of) HIPCHK(hipMemcpy(front_of, J->h.front_of, (size_t)P * 4, hipMemcpyDeviceToHost));
     if (state) {
         std::memcpy(state, st.ext, sizeof(st.ext));
         std::memcpy(state + 36, st.ideal, sizeof(st.ideal));
@@ -697,9 +786,8 @@ int cd_job_profile(cd_job *J, int32_t generations, float *kernel_ms) {
 
 int cd_job_cds_scanned(cd_job *J, int64_t *count) {
     if (!J || !count) return fail(CD_E_INVALID, "null argument");
-    HIPCHK(sync_job(J));
     cd::SelState st;
-    HIPCHK(hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(read_state(J, st));
     *count = (int64_t)st.cds_scanned;
     return CD_OK;
 }
@@ -710,8 +798,11 @@ int cd_job_destroy(cd_job *J) {
     if (J->stream) (void)hipStreamSynchronize(J->stream);
     if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
     if (J->graph) (void)hipGraphDestroy(J->graph);
-    if (J->stream) (void)hipStreamDestroy(J->stream);
-    if (J->mem) (void)hipFree(J->mem);
+    if (J->mem) (void)hipFreeAsync(J->mem, J->stream);  // back to the pool, after the job's last work
+    if (J->stream) {
+        (void)hipStreamSynchronize(J->stream);
+        (void)hipStreamDestroy(J->stream);
+    }
     delete J;
     return CD_OK;
 }
@@ -733,8 +824,18 @@ int cd_eval_batch(const cd_job_desc *desc, const int8_t *genome, int32_t count, 
     int2 *d_items = nullptr;
     hipError_t e = hipMalloc(&d_items, sizeof(int2) * count);
     if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), sizeof(int2) * count, hipMemcpyHostToDevice);
-    if (e == hipSuccess)  // CDS rows of L codons into rows of stride Ls
-        e = hipMemcpy2D(J->h.genome[0], J->h.Ls, genome, J->L, J->L, (size_t)count * J->m, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {  // CDS rows of L codons into rows of stride Ls (zero pads), one transfer
+        const size_t rows = (size_t)count * J->m, L = J->L, Ls = J->h.Ls;
+        std::lock_guard<std::mutex> lk(g_rows.mu);
+        e = g_rows.reserve(rows * Ls);
+        if (e == hipSuccess) {
+            for (size_t r = 0; r < rows; ++r) {
+                std::memcpy(g_rows.p + r * Ls, genome + r * L, L);
+                std::memset(g_rows.p + r * Ls + L, 0, Ls - L);
+            }
+            e = hipMemcpy(J->h.genome[0], g_rows.p, rows * Ls, hipMemcpyHostToDevice);
+        }
+    }
     if (e == hipSuccess) e = enqueue_designs(job_launch(J), cd::MODE_EVAL, d_items, count, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
     if (e == hipSuccess && obj) e = hipMemcpy(obj, J->h.obj[0], (size_t)count * 24, hipMemcpyDeviceToHost);
