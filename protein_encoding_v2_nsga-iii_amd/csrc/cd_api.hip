@@ -139,10 +139,10 @@ namespace {
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
     size_t aa, code, lw6, cpfx6, cps6, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
-        sctl, sorted,
+        sctl,
         front,
         np, dom,
-        ref_of, dist, niche, flag, st, self, off, domi, asc;
+        ref_of, dist, niche, st, acc, ranked, selm, prio, cutl, cutr, self, off, domi, asc;
     size_t total;
 };
 
@@ -170,15 +170,19 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
     for (int b = 0; b < (sel_only ? 1 : 2); ++b) o.obj[b] = s.add((size_t)P * 6 * 4);
     if (sel_only) o.obj[1] = o.obj[0];
     o.ref = s.add((size_t)R * 6 * 4);
-    o.sorted = s.add((size_t)N * 4);
     o.front = s.add((size_t)P * 4);
     o.np = s.add((size_t)P * 4);
     o.dom = s.add((size_t)P * W * 8);
     o.ref_of = s.add((size_t)P * 4);
     o.dist = s.add((size_t)P * 4);
-    o.niche = s.add((size_t)8 * R * 4);
-    o.flag = s.add((size_t)P);
+    o.niche = s.add((size_t)cd::NICHE_INTS * R * 4);
     o.st = s.add(sizeof(cd::SelState));
+    o.acc = s.add(sizeof(cd::SelAcc));
+    o.ranked = s.add((size_t)W * 8);
+    o.selm = s.add((size_t)W * 8);
+    o.prio = s.add((size_t)P * 4);
+    o.cutl = s.add((size_t)P * 8);
+    o.cutr = s.add((size_t)P * 4);
     o.self = s.add(sizeof(cd::DevJob));
     o.domi = s.add((size_t)n_dom * sizeof(int4));
     o.asc = s.add((size_t)n_asc * sizeof(int2));
@@ -360,7 +364,6 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.refpts = (const float *)(base + o.ref);
     h.obj[0] = (float *)(base + o.obj[0]);
     h.obj[1] = (float *)(base + o.obj[1]);
-    h.sorted = (int32_t *)(base + o.sorted);
     h.front_of = (int32_t *)(base + o.front);
     h.np = (int32_t *)(base + o.np);
     h.planes = sel_only ? nullptr : (uint2 *)(base + o.planes);
@@ -369,8 +372,13 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.ref_of = (int32_t *)(base + o.ref_of);
     h.dist = (float *)(base + o.dist);
     h.niche = (int32_t *)(base + o.niche);
-    h.flag = (uint8_t *)(base + o.flag);
     h.st = (cd::SelState *)(base + o.st);
+    h.acc = (cd::SelAcc *)(base + o.acc);
+    h.ranked = (uint64_t *)(base + o.ranked);
+    h.selm = (uint64_t *)(base + o.selm);
+    h.prio = (uint32_t *)(base + o.prio);
+    h.cutl = (uint64_t *)(base + o.cutl);
+    h.cutr = (int32_t *)(base + o.cutr);
     up.add(o.self, &h, 1);
     J->d_self = (cd::DevJob *)(base + o.self);
     if (!sel_only) {  // the tuned shapes of this kind of job, if any
@@ -399,8 +407,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);  // the staged bytes are released below
     if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
     if (e == hipSuccess && J->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(J->stem_lds);
-    J->niche_lds = cd::niche_lds_bytes(J->P, J->R);
-    if (e == hipSuccess && J->niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(J->niche_lds);
+    if (e == hipSuccess && cd::niche_lds_bytes(J->P) > 48 * 1024) e = cd::set_niche_lds_limit(cd::niche_lds_bytes(J->P));
     if (e != hipSuccess) {
         (void)hipFreeAsync(J->mem, J->stream);
         (void)hipStreamSynchronize(J->stream);
@@ -448,11 +455,11 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s, hipEvent_t *ev) {
     if (e == hipSuccess) e = mark(2);
     if (e == hipSuccess) e = cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, w.stem_ctl, w.sh.dom_rows, s);
     if (e == hipSuccess) e = mark(3);
-    if (e == hipSuccess) e = cd::launch_fronts(w.jobs, w.njobs, w.sh.fronts_threads, s);
+    if (e == hipSuccess) e = cd::launch_fronts(w.jobs, w.njobs, w.max_pool, w.sh.fronts_threads, s);
     if (e == hipSuccess) e = mark(4);
     if (e == hipSuccess) e = cd::launch_associate(w.jobs, w.asc, w.n_asc, w.sh.assoc_designs, s);
     if (e == hipSuccess) e = mark(5);
-    if (e == hipSuccess) e = cd::launch_niche(w.jobs, w.njobs, w.niche_lds, w.sh.niche_threads, s);
+    if (e == hipSuccess) e = cd::launch_niche(w.jobs, w.njobs, w.max_pool, w.max_refs, w.sh.niche_threads, s);
     if (e == hipSuccess) e = mark(6);
     return e;
 }
@@ -486,7 +493,8 @@ void build_lists(cd_job *const *jobs, int count, const cd::Shapes &sh, WorkLists
         job_lists(J->P, J->W, sh, wl.dom, wl.asc, j);
         wl.lds = std::max(wl.lds, J->lds);
         wl.stem_lds = std::max(wl.stem_lds, J->stem_lds);
-        wl.niche_lds = std::max(wl.niche_lds, J->niche_lds);
+        wl.max_pool = std::max(wl.max_pool, J->P);
+        wl.max_refs = std::max(wl.max_refs, J->R);
         wl.mg = std::max(wl.mg, J->m);
         wl.stem_parts = std::max(wl.stem_parts, J->stem_parts);
     }
@@ -496,7 +504,8 @@ hipError_t set_lds_limits(const WorkLists &wl) {
     hipError_t e = hipSuccess;
     if (wl.lds > 48 * 1024) e = cd::set_design_lds_limit(wl.lds);
     if (e == hipSuccess && wl.stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(wl.stem_lds);
-    if (e == hipSuccess && wl.niche_lds > 48 * 1024) e = cd::set_niche_lds_limit(wl.niche_lds);
+    if (e == hipSuccess && cd::niche_lds_bytes(wl.max_pool) > 48 * 1024)
+        e = cd::set_niche_lds_limit(cd::niche_lds_bytes(wl.max_pool));
     return e;
 }
 
@@ -520,7 +529,8 @@ int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
 
 Launch job_launch(cd_job *J) {
     return Launch{J->d_self, 1,          J->d_off,     J->N,          J->d_dom,   J->n_dom,   J->d_asc,
-                  J->n_asc,  J->lds,     J->stem_lds,  J->niche_lds,  J->m,       J->stem_parts, J->d_slist,
+                  J->n_asc,  J->lds,     J->stem_lds,  J->P,          J->R,       J->m,          J->stem_parts,
+                  J->d_slist,
                   J->d_sctl, J->N * J->m, 0,           0,             J->shapes};
 }
 
@@ -726,6 +736,15 @@ int cd_job_sync(cd_job *J) {
 // never the legacy null stream, so that they neither wait for nor disturb
 // other streams -- for instance a served tick being captured in another
 // thread.
+// The survivors in ascending pool index: the first N set bits of the
+// survivor mask.
+static void survivors_of(const std::vector<uint64_t> &mask, int N, int32_t *out) {
+    int k = 0;
+    for (size_t w = 0; w < mask.size() && k < N; ++w)
+        for (uint64_t x = mask[w]; x && k < N; x &= x - 1) out[k++] = (int32_t)(64 * w + __builtin_ctzll(x));
+    for (; k < N; ++k) out[k] = -1;
+}
+
 static hipError_t read_state(cd_job *J, cd::SelState &st) {
     hipError_t e = sync_job(J);
     if (e == hipSuccess) e = hipMemcpyAsync(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost, J->stream);
@@ -760,11 +779,13 @@ int cd_job_read(cd_job *J, int8_t *genome, float *obj, int8_t *idx, int32_t *pql
     if (idx) HIPCHK(hipMemcpyAsync(idx, J->h.oidx[b], (size_t)P * 12, hipMemcpyDeviceToHost, s));
     std::vector<int32_t> t(pql ? (size_t)P * 4 : 0);
     if (pql) HIPCHK(hipMemcpyAsync(t.data(), J->h.pql[b], t.size() * 4, hipMemcpyDeviceToHost, s));
-    if (sorted) HIPCHK(hipMemcpyAsync(sorted, J->h.sorted, (size_t)J->N * 4, hipMemcpyDeviceToHost, s));
+    std::vector<uint64_t> mask(sorted ? J->W : 0);
+    if (sorted) HIPCHK(hipMemcpyAsync(mask.data(), J->h.selm, (size_t)J->W * 8, hipMemcpyDeviceToHost, s));
     if (front_of) HIPCHK(hipMemcpyAsync(front_of, J->h.front_of, (size_t)P * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (genome)  // CDS rows of L codons from rows of stride Ls
         for (size_t r = 0; r < rows; ++r) std::memcpy(genome + r * L, g_rows.p + r * Ls, L);
+    if (sorted) survivors_of(mask, J->N, sorted);
     if (pql)
         for (int i = 0; i < P; ++i)
             for (int k = 0; k < 3; ++k) pql[i * 3 + k] = t[(size_t)i * 4 + k];
@@ -870,9 +891,9 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
     const cd::Shapes &sh = J->shapes;
     if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, sh.dom_rows, J->stream);
-    if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, sh.fronts_threads, J->stream);
+    if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->P, sh.fronts_threads, J->stream);
     if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, sh.assoc_designs, J->stream);
-    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->niche_lds, sh.niche_threads, J->stream);
+    if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->P, J->R, sh.niche_threads, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
     if (e != hipSuccess) {
         cd_job_destroy(J);
@@ -902,7 +923,9 @@ int cd_select(const float *obj, int32_t N, int32_t H, uint32_t gen, uint64_t see
     if (rc != CD_OK) return rc;
     cd::SelState st;
     hipError_t e = hipMemcpy(&st, J->h.st, sizeof(st), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && sorted) e = hipMemcpy(sorted, J->h.sorted, (size_t)N * 4, hipMemcpyDeviceToHost);
+    std::vector<uint64_t> mask(J->W);
+    if (e == hipSuccess && sorted) e = hipMemcpy(mask.data(), J->h.selm, (size_t)J->W * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sorted) survivors_of(mask, N, sorted);
     if (e == hipSuccess && front_of) e = hipMemcpy(front_of, J->h.front_of, (size_t)2 * N * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess) {
         if (extremes) std::memcpy(extremes, st.ext, sizeof(st.ext));
@@ -951,7 +974,8 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     }
     B->lds = wl.lds;
     B->stem_lds = wl.stem_lds;
-    B->niche_lds = wl.niche_lds;
+    B->max_pool = wl.max_pool;
+    B->max_refs = wl.max_refs;
     B->mg = wl.mg;
     B->stem_parts = wl.stem_parts;
     B->n_cds = wl.n_cds;
@@ -985,7 +1009,7 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
 static Launch batch_launch(cd_batch *B) {
     return Launch{B->d_jobs,    (int)B->jobs.size(), B->d_off,      B->n_off,   B->d_dom,
                   B->n_dom,     B->d_asc,            B->n_asc,      B->lds,     B->stem_lds,
-                  B->niche_lds, B->mg,               B->stem_parts, B->d_slist, B->d_sctl,
+                  B->max_pool,  B->max_refs,         B->mg,         B->stem_parts, B->d_slist, B->d_sctl,
                   B->n_cds,     B->n_small,          B->lds_small,  B->shapes};
 }
 

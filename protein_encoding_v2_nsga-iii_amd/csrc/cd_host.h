@@ -32,7 +32,7 @@ struct cd_job {
     int32_t *d_sctl = nullptr;    // its count, zero between generations (k_dominance clears it)
     int n_dom = 0, n_asc = 0;
     cd::Shapes shapes = cd::SHAPES_JOB;  // launch shapes of its generations (the lists above follow them)
-    size_t lds = 0, stem_lds = 0, niche_lds = 0;
+    size_t lds = 0, stem_lds = 0;
     int stem_parts = 1;  // k_stem workgroups per CDS (initialisation and evaluation)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -52,7 +52,9 @@ struct cd_batch {
     int n_off = 0, n_dom = 0, n_asc = 0, n_cds = 0;
     int n_small = 0;        // children first in d_off whose k_design fits six per CU
     size_t lds_small = 0;   // their k_design LDS
-    size_t lds = 0, stem_lds = 0, niche_lds = 0;
+    size_t lds = 0, stem_lds = 0;
+    int max_pool = 0;    // largest pool of the batch's jobs
+    int max_refs = 0;    // most reference points of the batch's jobs
     int mg = 0;          // largest m of the batch's jobs
     int stem_parts = 0;  // largest k_stem workgroups per CDS of the batch's jobs
     hipStream_t stream = nullptr;
@@ -85,7 +87,9 @@ struct Launch {
     int n_dom;
     const int2 *asc;  // (job, first design) association groups of sh.assoc_designs designs
     int n_asc;
-    size_t lds, stem_lds, niche_lds;
+    size_t lds, stem_lds;
+    int max_pool;    // largest pool
+    int max_refs;    // most reference points
     int mg;          // largest m
     int stem_parts;  // largest k_stem workgroups per CDS (initialisation and evaluation)
     uint32_t *stem_list;  // changed CDS of a generation, n_cds entries at most
@@ -105,8 +109,8 @@ struct WorkLists {
     std::vector<cd::DevJob> descs;
     std::vector<int2> off, asc;
     std::vector<int4> dom;
-    size_t lds = 0, stem_lds = 0, niche_lds = 0;
-    int mg = 0, stem_parts = 0, n_cds = 0;
+    size_t lds = 0, stem_lds = 0;
+    int max_pool = 0, max_refs = 0, mg = 0, stem_parts = 0, n_cds = 0;
     int n_small = 0;  // children (first in off) of jobs with lds at most shapes.design_split
     size_t lds_small = 0;
 };

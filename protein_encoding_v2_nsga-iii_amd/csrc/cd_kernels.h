@@ -66,15 +66,21 @@ hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, i
 // generation's k_design, cleared for the next one
 hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
                             hipStream_t s);
-// one workgroup of `threads` (256, 512 or 1024) per job
-hipError_t launch_fronts(const DevJob *jobs, int njobs, int threads, hipStream_t s);
-// items: (job, first design), `designs` (4, 8 or 16) designs per item
+// k_front0 (grid: jobs x chunks of FRONT0_THREADS designs of the largest
+// pool, max_pool), then k_fronts (one workgroup of `threads` (256, 512 or
+// 1024) per job)
+constexpr int FRONT0_THREADS = 256;
+hipError_t launch_fronts(const DevJob *jobs, int njobs, int max_pool, int threads, hipStream_t s);
+// items: (job, first design), `designs` (4, 8 or 16) designs per item, a wave each
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int designs, hipStream_t s);
-// one workgroup of `threads` (256, 512 or 1024) per job; lds = the largest
-// niche_lds_bytes of the jobs
-size_t niche_lds_bytes(int P, int R);
+// k_levels (one workgroup per job) when some job has more reference points
+// than NICHE_STAGE (max_refs), then k_place and k_niche (grid: jobs x chunks
+// of `threads` (256, 512 or 1024) designs of the largest pool, max_pool;
+// k_niche takes niche_lds_bytes(max_pool) of dynamic LDS, set_niche_lds_limit
+// above 48 KiB)
+size_t niche_lds_bytes(int P);
 hipError_t set_niche_lds_limit(size_t bytes);
-hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, int threads, hipStream_t s);
+hipError_t launch_niche(const DevJob *jobs, int njobs, int max_pool, int max_refs, int threads, hipStream_t s);
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s);
 

@@ -1,19 +1,29 @@
 // cd_kernels.hip -- HIP kernels of the codon-design engine for gfx950.
 //
-// One generation is six launches over every resident job:
-//   k_design<OFFSPRING>  one workgroup per child: survivor copy, mutation,
-//                        objectives 0-4 (mutationKernel, main.cu:121-240)
-//   k_stem               1..STEM_PARTS workgroups per (child, CDS): longest
-//                        stem of each CDS the mutation changed (calMaximumSL)
+// One generation is these launches over every resident job:
+//   k_design<OFFSPRING>  one workgroup per child: parent from the survivor
+//                        mask, survivor copy, mutation, objectives 0-4
+//                        (mutationKernel, main.cu:121-240)
+//   k_stem_list          longest stem of each CDS the mutation changed
+//                        (calMaximumSL)
 //   k_dominance          objective 5 of the children, bit-packed
-//                        eps-domination matrix (sorting.cuh:994-1023)
-//   k_fronts             one workgroup per job: front peeling, ideal/worst,
-//                        extreme points, hyperplane/MNDF nadir
-//                        (sorting.cuh:974-1089, 132-972)
-//   k_associate          nearest reference line per ranked design
-//                        (sorting.cuh:1105-1180)
-//   k_niche              one workgroup per job: niche counts, level sweeps,
-//                        survivor list (sorting.cuh:1182-1279)
+//                        eps-domination matrix (sorting.cuh:994-1023), the
+//                        pool's ideal and worst points
+//   k_front0             the first front, its maxima and ASF minima, many
+//                        workgroups per job
+//   k_fronts             one workgroup per job: further fronts, extreme
+//                        points, hyperplane/MNDF nadir (sorting.cuh:974-1089,
+//                        132-972)
+//   k_associate          nearest reference line per ranked design and the
+//                        niche counts (sorting.cuh:1105-1180)
+//   [k_levels]           the niche levels, when the reference points do not
+//                        fit in LDS (else k_place settles them)
+//   k_place              level-0 picks, whole buckets, the cut list
+//   k_niche              the cut buckets ranked by priority
+//                        (sorting.cuh:1182-1279); ends the generation
+// Hand-offs between workgroups happen at kernel boundaries; atomics that
+// fold per-job quantities are reduced over the workgroup first (same-address
+// atomics serialise at the device's coherence point).
 // Work lists map blockIdx.x to (job, item) so the same kernels advance one
 // job or a batch of uneven jobs.
 #include <cfloat>
@@ -963,6 +973,45 @@ __device__ void build_planes(const uint8_t *rows, int Ls, int L, const int *dl, 
     }
 }
 
+// The index of the c-th survivor (0-based, ascending pool index) in the
+// survivor mask of W words: wave-uniform, every lane of the wave takes part.
+__device__ int survivor_at(const uint64_t *selm, int W, int c) {
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    for (int w0 = 0; w0 < W; w0 += 64) {
+        const int w = w0 + lane;
+        const uint64_t x = w < W ? gload(selm + w) : 0ull;
+        int incl = (int)__popcll(x);  // inclusive prefix of the words' counts
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        const uint64_t hit = __ballot(base + incl > c);
+        if (hit) {
+            const int q = (int)__builtin_ctzll(hit);
+            uint64_t y = ((uint64_t)(uint32_t)__shfl((int)(x >> 32), q, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)x, q, 64);
+            int k = c - (base + __shfl(incl, q, 64) - (int)__popcll(y));  // rank within the word
+            int pos = 0;
+#pragma unroll
+            for (int h = 32; h > 0; h >>= 1) {  // the k-th set bit of y
+                const uint64_t low = (1ull << h) - 1ull;
+                const int n = (int)__popcll(y & low);
+                if (k >= n) {
+                    k -= n;
+                    y >>= h;
+                    pos += h;
+                } else {
+                    y &= low;
+                }
+            }
+            return (w0 + q) * 64 + pos;
+        }
+        base += __shfl(incl, 63, 64);
+    }
+    return 0;  // fewer than c + 1 survivors: not reached
+}
+
 // ---------------------------------------------------------------------------
 // k_design: one workgroup per design.
 //   INIT       random genome (design 0: best-CAI codons), genPopulation,
@@ -1010,7 +1059,7 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
         src = (int)(gen & 1u);
         dst = src ^ 1;
         out_slot = N + it.y;
-        parent = J.sorted[it.y];
+        parent = survivor_at(J.selm, J.W, it.y);
     }
     u32x4 *syn4 = (u32x4 *)syn;
     if (MODE == 0) {
@@ -1066,10 +1115,12 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
             gstore_stat(J.cst[dst] + (size_t)c * m + tid, s);
             cst[tid] = s;
         }
+        // the sort of this generation starts from cleared counts
         if (tid == 0) {
             gstore(J.np + c, 0);
             gstore(J.np + N + c, 0);
         }
+        if (c == 0 && tid < ACC_WORDS) gstore((uint32_t *)J.acc + tid, acc_clear_word(tid));
     }
     lds_barrier();  // tables, synonym ranges, genome row and records
     // codon codes of the row, four per thread step; the padding positions
@@ -1300,6 +1351,38 @@ __device__ __forceinline__ bool dominates(const float *a, const float *b) {
     return any && !bad;
 }
 
+// The pool's extrema over rows row0 .. row0 + rows - 1 (updateIdealValue /
+// updateWorstValue, sorting.cuh:132-426; fminf / fmaxf from FLT_MAX /
+// -FLT_MAX), folded into the job's accumulators with one atomic minimum and
+// maximum per objective.  One wave; `vals` holds the rows' objectives.
+__device__ void pool_extrema(const DevJob &J, int row0, int rows, const float *vals) {
+    const int lane = threadIdx.x & 63, P = J.P;
+    float mn[NOBJ], mx[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        mn[k] = FLT_MAX;
+        mx[k] = -FLT_MAX;
+    }
+    for (int t = lane; t < rows && row0 + t < P; t += 64)
+#pragma unroll
+        for (int k = 0; k < NOBJ; ++k) {
+            const float v = vals[t * NOBJ + k];
+            mn[k] = fminf(mn[k], v);
+            mx[k] = fmaxf(mx[k], v);
+        }
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        mn[k] = wave_min_f32(mn[k]);
+        mx[k] = wave_max_f32(mx[k]);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NOBJ; ++k) {
+            atomicMin(&J.acc->lo[k], ord_bits(mn[k]));
+            atomicMax(&J.acc->hi[k], ord_bits(mx[k]));
+        }
+}
+
 // Bit i of word w of row j (i = 64w + bit) is set when design i dominates
 // design j; the matrix is stored word-major, dom[w * P + j], so that the
 // front peeling reads one word of consecutive rows per wave.  np[j] counts
@@ -1339,6 +1422,8 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         if (r >= N && r < P) stem_finish(J, buf, r);
     }
     __syncthreads();
+    // and fold their rows into the pool's extrema (one workgroup per row block)
+    if (word == 0 && wv == 0) pool_extrema(J, it.y, ROWS, rows);
     const int r0 = it.y + wv * WR;
     uint64_t mine = 0;  // lane q < WR: the word of row r0 + q
 #pragma unroll 4
@@ -1415,11 +1500,13 @@ __device__ void gauss(const float *E, float *sol) {
     for (int r = 0; r < NOBJ; ++r) sol[r] = AB[r][NOBJ];
 }
 
+// (v, id) becomes (ov, oid) when that pair is smaller: a smaller value, or
+// the same value with a smaller index (a NaN value is never taken).  Written
+// without short-circuit evaluation so that it compiles to selects.
 __device__ __forceinline__ void argmin_pair(float &v, int &id, float ov, int oid) {
-    if (ov < v || (ov == v && oid < id)) {
-        v = ov;
-        id = oid;
-    }
+    const bool take = (ov < v) | ((ov == v) & (oid < id));
+    v = take ? ov : v;
+    id = take ? oid : id;
 }
 
 // argmin_pair over a full wave (DPP row shifts and broadcasts, lane 63 read
@@ -1442,106 +1529,235 @@ __device__ __forceinline__ void wave_argmin(float &v, int &id) {
     id = __builtin_amdgcn_readlane(id, 63);
 }
 
-// asf() for all six axes of one design: the clamped differences are divided
-// by F_PRECISION once each (the axis itself divides by 1, which is exact),
-// and each axis takes the same strict running maximum in k order.
+// asf() for all six axes of one design.  The clamped differences t[k] are +0
+// or at least 0.001 (or NaN, which no comparison takes), and axis a takes the
+// strict running maximum of t[a] and t[k] / F_PRECISION over k != a.  A
+// correctly rounded division by a positive constant is nondecreasing, so the
+// largest of those quotients is the quotient of the largest t[k], k != a: the
+// largest and second-largest differences give every axis with two divisions,
+// the same values as dividing each difference.
 __device__ __forceinline__ void asf_all(const float *x, const float *ideal, float *out) {
-    float t[NOBJ], d[NOBJ];
+    float t[NOBJ];
+    float m1 = -FLT_MAX, m2 = -FLT_MAX;  // largest and second-largest difference
+    int k1 = -1;
 #pragma unroll
     for (int k = 0; k < NOBJ; ++k) {
         t[k] = x[k] - ideal[k];
         if ((double)t[k] < 0.001) t[k] = 0.f;
-        d[k] = t[k] / F_PRECISION;
+        const bool top = t[k] > m1, second = !top && t[k] > m2;
+        m2 = top ? m1 : (second ? t[k] : m2);
+        m1 = top ? t[k] : m1;
+        k1 = top ? k : k1;
     }
+    const float q1 = m1 == -FLT_MAX ? -FLT_MAX : m1 / F_PRECISION;
+    const float q2 = m2 == -FLT_MAX ? -FLT_MAX : m2 / F_PRECISION;
 #pragma unroll
     for (int a = 0; a < NOBJ; ++a) {
-        float mx = -FLT_MAX;
-#pragma unroll
-        for (int k = 0; k < NOBJ; ++k) {
-            const float v = k == a ? t[k] : d[k];
-            if (v > mx) mx = v;
-        }
-        out[a] = mx;
+        const float q = a == k1 ? q2 : q1;
+        out[a] = t[a] > q ? t[a] : q;
     }
 }
 
-// One workgroup per job: nonDominatedSorting's peeling (sorting.cuh:1026-1086),
-// updateIdealValue/updateWorstValue, findExtremePoints, updateNadirValue_HYP
-// with the MNDF fallback (sorting.cuh:132-972).  Three passes over the pool:
-// F0 with ideal/worst/F0 maximum, the peeling of further fronts (only while
-// fewer than N are ranked), and the ASF minima of F0 for all six axes.
+__device__ __forceinline__ unsigned long long asf_key(float v, int id) {
+    return ((unsigned long long)ord_bits(v) << 32) | (uint32_t)id;
+}
+
+// Extreme points (findExtremePoints, sorting.cuh:601-683) settled axis by
+// axis against the previous ones, by one wave.  The candidate rows are the
+// previous extremes (0-5) and the first front's ASF minima of the six axes
+// (6-11), with their ASF values for every axis in casf.  Lane t (0-5) keeps
+// the row extreme slot t holds; per axis the slots' values and the first
+// front's minimum (lane 6) are reduced to the smallest (value, index) pair --
+// the serial fold over the slots in order gives the same pair, NaN values
+// never being taken -- and the winner's row goes to the axis's slot.  Returns
+// the row held by slot `lane` (lanes 0-5).
+__device__ int settle_extremes(const float *pv, const int *pid, const float (*casf)[NOBJ], int P) {
+    const int lane = threadIdx.x & 63;
+    int cur = lane < NOBJ ? lane : 0;
+#pragma unroll
+    for (int axis = 0; axis < NOBJ; ++axis) {
+        const float c = casf[cur][axis];
+        const bool slot = lane < NOBJ && !isnan(c), first = lane == NOBJ;
+        float v = slot ? c : (first ? pv[axis] : FLT_MAX);
+        int id = slot ? P + lane : (first ? pid[axis] : 0x7FFFFFFF);
+        // the smallest pair of lanes 0-7 (prefix minima within the first row)
+        argmin_dpp<0x111, 0xf>(v, id);
+        argmin_dpp<0x112, 0xf>(v, id);
+        argmin_dpp<0x114, 0xf>(v, id);
+        v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 7));
+        id = __builtin_amdgcn_readlane(id, 7);
+        if (id != 0x7FFFFFFF && v < FLT_MAX) {
+            const int t = id - P;  // an extreme slot won: the row it holds
+            const int held = (t >= 0 && t < NOBJ) ? __builtin_amdgcn_readlane(cur, t) : NOBJ + axis;
+            if (lane == axis) cur = held;
+        }
+    }
+    return cur;
+}
+
+// GaussianElimination (sorting.cuh:451-599, E x = 1) by one wave: lane r
+// (0-5) holds row r of AB (a[6] the right-hand side) in registers and reads
+// the pivot row with v_readlane; every element takes the same operations in
+// the same order as the row-by-row elimination, so the solution is the same
+// to the bit.  Returns x[lane] in lanes 0-5.
+__device__ __forceinline__ float lane_of(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ float gauss_wave(const float *E) {
+    const int lane = threadIdx.x & 63;
+    const bool row_lane = lane < NOBJ;
+    const int r = row_lane ? lane : 0;
+    float a[NOBJ + 1];
+#pragma unroll
+    for (int c = 0; c < NOBJ; ++c) a[c] = E[r * NOBJ + c];
+    a[NOBJ] = 1.0f;
+#pragma unroll
+    for (int col = 0; col < NOBJ; ++col) {
+        if (fabsf(lane_of(a[col], col)) <= 1e-6f) {  // add the first row below with a usable entry
+            const uint64_t cand = __ballot(row_lane && lane > col && fabsf(a[col]) > 1e-6f);
+            if (cand) {
+                const int rr = (int)__builtin_ctzll(cand);
+#pragma unroll
+                for (int c = col; c <= NOBJ; ++c) {
+                    const float add = lane_of(a[c], rr);
+                    if (lane == col) a[c] += add;
+                }
+            }
+        }
+        float prow[NOBJ + 1];
+#pragma unroll
+        for (int c = col; c <= NOBJ; ++c) prow[c] = lane_of(a[c], col);
+        if (row_lane && lane > col) {
+            const float f = a[col] / prow[col];
+#pragma unroll
+            for (int c = col; c <= NOBJ; ++c) a[c] -= f * prow[c];
+        }
+    }
+#pragma unroll
+    for (int row = NOBJ - 1; row >= 0; --row) {
+        float s = 0.f;
+#pragma unroll
+        for (int cc = row + 1; cc < NOBJ; ++cc) s += lane_of(a[cc], row);
+        float x = lane_of(a[NOBJ], row);
+        if (row != NOBJ - 1) x -= s;
+        x /= lane_of(a[row], row);
+        if (lane == row) {
+            a[NOBJ] = x;
+            a[row] = 1.0f;
+        }
+        if (lane < row) a[row] *= x;
+    }
+    return a[NOBJ];
+}
+
+// The first front (designs nobody dominates: their dominator counts are
+// complete after k_dominance), one design a thread (grid: jobs x chunks of
+// FRONT0_THREADS designs of the largest pool): front_of 0 or -1, the
+// first-front words of the ranked mask, and the workgroup's share of the
+// first front's size, its maximum per objective (updateNadirValue_MNDF,
+// sorting.cuh:685-884) and its per-axis ASF minima against the ideal point
+// k_dominance folded (findExtremePoints, sorting.cuh:601-683; the smallest
+// value, ties to the lowest index), reduced over the workgroup and folded
+// into the job's accumulators with one atomic per quantity.
+__global__ __launch_bounds__(FRONT0_THREADS) void k_front0(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    const int P = J.P, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    constexpr int NW = FRONT0_THREADS / 64;
+    const int d0 = blockIdx.y * FRONT0_THREADS;
+    if (d0 >= P) return;
+    SelAcc *acc = J.acc;
+    const float *obj = J.obj[pool_buffer(J)];
+    __shared__ float ideal[NOBJ], red_v[NW * 2 * NOBJ];
+    __shared__ int red_i[NW * NOBJ], s_cnt;
+    if (tid < NOBJ) ideal[tid] = ord_float(acc->lo[tid]);
+    if (tid == 0) s_cnt = 0;
+    const int d = d0 + tid, dc = min(d, P - 1);
+    const int npv = gload(J.np + dc);
+    float x[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) x[k] = gload(obj + (size_t)dc * NOBJ + k);
+    __syncthreads();
+    const bool f0 = d < P && npv == 0;
+    if (d < P) gstore(J.front_of + d, f0 ? 0 : -1);
+    const uint64_t b = __ballot(f0);  // a wave's 64 designs are one word of the mask
+    if (lane == 0 && d0 + 64 * wid < P) {
+        gstore(J.ranked + (d0 >> 6) + wid, b);
+        if (b) atomicAdd(&s_cnt, (int)__popcll(b));
+    }
+    float fm[NOBJ], bv[NOBJ];
+    int bi[NOBJ];
+    asf_all(x, ideal, bv);
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        fm[k] = f0 ? x[k] : -FLT_MAX;
+        // an ASF value is never NaN: it is at least -FLT_MAX
+        bi[k] = f0 ? d : 0x7FFFFFFF;
+        bv[k] = f0 ? bv[k] : FLT_MAX;
+        fm[k] = wave_max_f32(fm[k]);
+        wave_argmin(bv[k], bi[k]);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NOBJ; ++k) {
+            red_v[wid * 2 * NOBJ + k] = fm[k];
+            red_v[wid * 2 * NOBJ + NOBJ + k] = bv[k];
+            red_i[wid * NOBJ + k] = bi[k];
+        }
+    __syncthreads();
+    if (tid < NOBJ) {
+        float m = -FLT_MAX, v = FLT_MAX;
+        int id = 0x7FFFFFFF;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            m = fmaxf(m, red_v[w * 2 * NOBJ + tid]);
+            argmin_pair(v, id, red_v[w * 2 * NOBJ + NOBJ + tid], red_i[w * NOBJ + tid]);
+        }
+        atomicMax(&acc->f0hi[tid], ord_bits(m));
+        if (id != 0x7FFFFFFF) atomicMin(&acc->asf[tid], asf_key(v, id));
+    }
+    if (tid == 0 && s_cnt) atomicAdd(&acc->f0, s_cnt);
+}
+
+// One workgroup per job, after k_front0: the selection up to the niching.
+//  * Further fronts while fewer than N designs are ranked (nonDominatedSorting's
+//    peeling, sorting.cuh:1026-1086).
+//  * The extreme points settled against the previous ones and the nadir point
+//    (updateNadirValue_HYP with the MNDF fallback, sorting.cuh:911-972), each
+//    by one wave.
+//  * The survivors of the fronts before the last one, and cleared niche
+//    arrays for k_associate.
 __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs) {
     const DevJob &J = jobs[blockIdx.x];
-    const int P = J.P, W = J.W, N = J.N;
-    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
+    const int P = J.P, W = J.W, N = J.N, R = J.R;
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const float *obj = J.obj[pool_buffer(J)];
-    __shared__ uint64_t ranked[MAX_W], newm[MAX_W];
+    const SelAcc *acc = J.acc;
+    SelState *st = J.st;
+    __shared__ uint64_t ranked[MAX_W], newm[MAX_W], lastm[MAX_W];
     __shared__ int s_cnt, s_new;
-    __shared__ float red_v[16 * 18];
-    __shared__ int red_i[16 * NOBJ];
     __shared__ float ext[NOBJ * NOBJ], ideal[NOBJ], worst[NOBJ], f0max[NOBJ];
-
-    for (int w = tid; w < W; w += NT) {
-        ranked[w] = 0;
-        newm[w] = 0;
+    __shared__ float pv[NOBJ], crow[2 * NOBJ][NOBJ], casf[2 * NOBJ][NOBJ];
+    __shared__ int pid[NOBJ];
+    if (tid < NOBJ) {
+        ideal[tid] = ord_float(acc->lo[tid]);
+        worst[tid] = ord_float(acc->hi[tid]);
+        f0max[tid] = ord_float(acc->f0hi[tid]);
+        const unsigned long long k = acc->asf[tid];
+        pv[tid] = k == ~0ull ? FLT_MAX : ord_float((uint32_t)(k >> 32));
+        pid[tid] = k == ~0ull ? 0x7FFFFFFF : (int)(uint32_t)k;
     }
+    if (tid < NOBJ * NOBJ) ext[tid] = st->ext[tid];
     if (tid == 0) {
-        s_cnt = 0;
+        s_cnt = acc->f0;
         s_new = 0;
     }
-    if (tid < NOBJ * NOBJ) ext[tid] = J.st->ext[tid];
-    __syncthreads();
-    // F0 = designs nobody dominates; ideal / worst over the pool, maximum over F0
-    {
-        float mn[NOBJ], mx[NOBJ], fm[NOBJ];
-        for (int k = 0; k < NOBJ; ++k) {
-            mn[k] = FLT_MAX;
-            mx[k] = -FLT_MAX;
-            fm[k] = -FLT_MAX;
-        }
-        int c = 0;
-        for (int r0 = 0; r0 < P; r0 += NT) {  // a wave's 64 designs are one word of the masks
-            const int r = r0 + tid;
-            const bool f0 = r < P && gload(J.np + r) == 0;
-            const uint64_t b = __ballot(f0);
-            if (r < P) gstore(J.front_of + r, f0 ? 0 : -1);
-            if (lane == 0 && r < P) ranked[r >> 6] = b;
-            c += __popcll(b);
-            if (r < P)
-                for (int k = 0; k < NOBJ; ++k) {
-                    const float v = gload(obj + (size_t)r * NOBJ + k);
-                    mn[k] = fminf(mn[k], v);
-                    mx[k] = fmaxf(mx[k], v);
-                    if (f0) fm[k] = fmaxf(fm[k], v);
-                }
-        }
-        for (int k = 0; k < NOBJ; ++k) {
-            mn[k] = wave_min_f32(mn[k]);
-            mx[k] = wave_max_f32(mx[k]);
-            fm[k] = wave_max_f32(fm[k]);
-        }
-        if (lane == 0) {
-            atomicAdd(&s_cnt, c);
-            for (int k = 0; k < NOBJ; ++k) {
-                red_v[wid * 18 + k] = mn[k];
-                red_v[wid * 18 + 6 + k] = mx[k];
-                red_v[wid * 18 + 12 + k] = fm[k];
-            }
-        }
-        __syncthreads();
-        if (tid < NOBJ) {
-            float a = FLT_MAX, b = -FLT_MAX, d = -FLT_MAX;
-            for (int w = 0; w < nw; ++w) {
-                a = fminf(a, red_v[w * 18 + tid]);
-                b = fmaxf(b, red_v[w * 18 + 6 + tid]);
-                d = fmaxf(d, red_v[w * 18 + 12 + tid]);
-            }
-            ideal[tid] = a;
-            worst[tid] = b;
-            f0max[tid] = d;
-        }
-        __syncthreads();
+    for (int w = tid; w < W; w += NT) {
+        const uint64_t f0w = gload(J.ranked + w);
+        ranked[w] = f0w;
+        lastm[w] = f0w;
+        newm[w] = 0;
     }
+    __syncthreads();
     int f = 0, last_size = s_cnt;
     while (s_cnt < N) {
         ++f;
@@ -1573,6 +1789,7 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
             __syncthreads();
         }
         for (int w = tid; w < W; w += NT) {
+            lastm[w] = newm[w];
             ranked[w] |= newm[w];
             newm[w] = 0;
         }
@@ -1585,100 +1802,68 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
         __syncthreads();
     }
     const int ranked_n = s_cnt;
-    // findExtremePoints: per axis the ASF minimum over F0 (ascending index),
-    // then against the previous extremes, axis by axis
-    {
-        float bv[NOBJ];
-        int bi[NOBJ];
-        for (int a = 0; a < NOBJ; ++a) {
-            bv[a] = FLT_MAX;
-            bi[a] = 0x7FFFFFFF;
-        }
-        for (int r = tid; r < P; r += NT) {
-            if (J.front_of[r] != 0) continue;
-            float x[NOBJ], av[NOBJ];
-            for (int k = 0; k < NOBJ; ++k) x[k] = obj[(size_t)r * NOBJ + k];
-            asf_all(x, ideal, av);
-            for (int a = 0; a < NOBJ; ++a) argmin_pair(bv[a], bi[a], av[a], r);
-        }
-        for (int a = 0; a < NOBJ; ++a) wave_argmin(bv[a], bi[a]);
-        if (lane == 0)
-            for (int a = 0; a < NOBJ; ++a) {
-                red_v[wid * NOBJ + a] = bv[a];
-                red_i[wid * NOBJ + a] = bi[a];
-            }
-        __syncthreads();
-        // The rows an extreme can end up holding are the previous extremes
-        // (candidates 0-5) and the pool minima of the six axes (6-11): their
-        // ASF values for every axis are computed in parallel, then the axes
-        // are settled in order with lookups.
-        __shared__ float pv[NOBJ], crow[2 * NOBJ][NOBJ], casf[2 * NOBJ][NOBJ];
-        __shared__ int pid[NOBJ];
-        if (tid < NOBJ) {
-            float v = red_v[tid];
-            int id = red_i[tid];
-            for (int w = 1; w < nw; ++w) argmin_pair(v, id, red_v[w * NOBJ + tid], red_i[w * NOBJ + tid]);
-            pv[tid] = v;
-            pid[tid] = id;
-        }
-        __syncthreads();
-        if (tid < 2 * NOBJ * NOBJ) {
-            const int c = tid / NOBJ, k = tid % NOBJ;
-            if (c < NOBJ)
-                crow[c][k] = ext[c * NOBJ + k];
-            else
-                crow[c][k] = pid[c - NOBJ] < P ? obj[(size_t)pid[c - NOBJ] * NOBJ + k] : 0.f;
-        }
-        __syncthreads();
-        if (tid < 2 * NOBJ * NOBJ) casf[tid / NOBJ][tid % NOBJ] = asf(crow[tid / NOBJ], ideal, tid % NOBJ);
-        __syncthreads();
-        if (tid == 0) {
-            int cur[NOBJ];  // candidate row now held by each extreme slot
-#pragma unroll
-            for (int t = 0; t < NOBJ; ++t) cur[t] = t;
-#pragma unroll
-            for (int axis = 0; axis < NOBJ; ++axis) {
-                float v = pv[axis];
-                int id = pid[axis];
-#pragma unroll
-                for (int t = 0; t < NOBJ; ++t) argmin_pair(v, id, casf[cur[t]][axis], P + t);
-                if (id != 0x7FFFFFFF && v < FLT_MAX) {
-                    int held = NOBJ + axis;  // cur[id - P] when an extreme slot won, read by a select
-#pragma unroll
-                    for (int t = 0; t < NOBJ; ++t)
-                        if (id == P + t) held = cur[t];
-                    cur[axis] = held;
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < NOBJ; ++t)
-#pragma unroll
-                for (int k = 0; k < NOBJ; ++k) ext[t * NOBJ + k] = crow[cur[t]][k];
+    const bool n_cut = ranked_n == N;
+    // survivors so far: every ranked design when the fronts fill N exactly,
+    // else the fronts before the last one
+    if (J.selm)
+        for (int w = tid; w < W; w += NT) gstore(J.selm + w, n_cut ? ranked[w] : (ranked[w] & ~lastm[w]));
+    if (J.niche && !n_cut) {  // k_associate counts into these
+        const NicheView g = niche_view(J.niche, R);
+        for (int r = tid; r < R; r += NT) {
+            g.cr[r] = make_int2(0, 0);
+            gstore(g.fill + r, 0);
+            gstore(g.nearest + r, ~0ull);
+            gstore(g.lowest + r, ~0ull);
         }
     }
-    if (tid == 0) {
-        SelState *st = J.st;
-        float x[NOBJ], icpt[NOBJ];
-        gauss(ext, x);
+    // candidate rows of the extremes and their ASF values for every axis
+    if (tid < 2 * NOBJ * NOBJ) {
+        const int c = tid / NOBJ, k = tid % NOBJ;
+        if (c < NOBJ)
+            crow[c][k] = ext[c * NOBJ + k];
+        else
+            crow[c][k] = pid[c - NOBJ] < P ? gload(obj + (size_t)pid[c - NOBJ] * NOBJ + k) : 0.f;
+    }
+    __syncthreads();
+    if (tid < 2 * NOBJ * NOBJ) casf[tid / NOBJ][tid % NOBJ] = asf(crow[tid / NOBJ], ideal, tid % NOBJ);
+    __syncthreads();
+    if (wid == 0) {
+        const int held = settle_extremes(pv, pid, casf, P);
+        const int mine = __shfl(held, lane < NOBJ * NOBJ ? lane / NOBJ : 0, 64);
+        if (lane < NOBJ * NOBJ) ext[lane] = crow[mine][lane % NOBJ];
+    }
+    __syncthreads();
+    if (wid == 0) {
+        const float x = gauss_wave(ext);  // x[lane], lanes 0-5
+        float xs[NOBJ];
+#pragma unroll
+        for (int i = 0; i < NOBJ; ++i) xs[i] = lane_of(x, i);
+        // residuals of every extreme and the intercepts (a lane each)
         bool exc = false;
-        for (int t = 0; t < NOBJ; ++t) {
+        float icpt = 0.f;
+        if (lane < NOBJ) {
             float res = 0.f;
-            for (int i = 0; i < NOBJ; ++i) res += x[i] * (ext[t * NOBJ + i] - ideal[i]);
+#pragma unroll
+            for (int i = 0; i < NOBJ; ++i) res += xs[i] * (ext[lane * NOBJ + i] - ideal[i]);
             if (fabsf(res - 1.f) > F_PRECISION || isnan(res) || isinf(res)) exc = true;
-            icpt[t] = 1.f / x[t];
-            if (icpt[t] <= F_PRECISION || isnan(icpt[t]) || isinf(icpt[t])) exc = true;
+            icpt = 1.f / x;
+            if (icpt <= F_PRECISION || isnan(icpt) || isinf(icpt)) exc = true;
         }
-        for (int k = 0; k < NOBJ; ++k) {
-            st->ideal[k] = ideal[k];
-            st->worst[k] = worst[k];
-            st->nadir[k] = exc ? (((f0max[k] - ideal[k]) > F_PRECISION) ? f0max[k] : worst[k]) : ideal[k] + icpt[k];
+        const bool hyp = __ballot(exc) != 0;
+        if (lane < NOBJ) {
+            st->ideal[lane] = ideal[lane];
+            st->worst[lane] = worst[lane];
+            st->nadir[lane] = hyp ? (((f0max[lane] - ideal[lane]) > F_PRECISION) ? f0max[lane] : worst[lane])
+                                  : ideal[lane] + icpt;
         }
-        for (int i = 0; i < NOBJ * NOBJ; ++i) st->ext[i] = ext[i];
-        st->hyp = exc;
-        st->ranked = ranked_n;
-        st->last = f;
-        st->base = (f == 0) ? 0 : ranked_n - last_size;
-        st->n_cut = (ranked_n == N);
+        if (lane < NOBJ * NOBJ) st->ext[lane] = ext[lane];
+        if (lane == 0) {
+            st->hyp = hyp;
+            st->ranked = ranked_n;
+            st->last = f;
+            st->base = (f == 0) ? 0 : ranked_n - last_size;
+            st->n_cut = n_cut;
+        }
     }
 }
 
@@ -1699,284 +1884,307 @@ __device__ __forceinline__ float perp_dist(const float *f, const float *w) {  //
     return sqrtf(d);
 }
 
-// Nearest reference line of every ranked design (sorting.cuh:1105-1180): one
-// wave per design, lanes take reference lines r = lane, lane + 64, ... and the
-// wave keeps the smallest distance, ties to the lowest index (the first
-// minimum of a sweep in index order).  item = (job, first of D designs).
-template <int D>
-__global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__ jobs,
-                                                      const int2 *__restrict__ items) {
-    const int2 it = items[blockIdx.x];
-    const DevJob &J = jobs[it.x];
-    const SelState *st = J.st;
-    if (st->n_cut) return;
-    const int lane = threadIdx.x & 63;
-    const int d = it.y + (int)(threadIdx.x >> 6);
-    if (d >= J.P || J.front_of[d] < 0) return;  // whole wave
-    const float *obj = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
-    float f[NOBJ];
-#pragma unroll
-    for (int k = 0; k < NOBJ; ++k) {
-        const float den = st->nadir[k] - st->ideal[k];
-        f[k] = (den == 0.f) ? 0.f : (obj[k] - st->ideal[k]) / den;
-    }
-    float best = FLT_MAX;
-    int br = 0;
-    for (int r = lane; r < J.R; r += 64) {
-        const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
-        if (dd < best) {
-            best = dd;
-            br = r;
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const float ob = __shfl_xor(best, o, 64);
-        const int orr = __shfl_xor(br, o, 64);
-        if (ob < best || (ob == best && orr < br)) {
-            best = ob;
-            br = orr;
-        }
-    }
-    if (lane == 0) {
-        J.ref_of[d] = br;
-        J.dist[d] = best;
-    }
-}
-
 __device__ __forceinline__ uint64_t niche_prio(const DevJob &J, uint32_t gen, int d) {
     uint32_t r[4];
     philox(gen, (uint32_t)d, 0u, TAG_NICHE, J.seed, r);
     return ((uint64_t)r[0] << 32) | (uint32_t)d;
 }
 
-// Block-wide sum of one int per thread.
-// One workgroup per job: niche counts, the level sweeps of
-// referenceBasedSorting (sorting.cuh:1182-1279) in closed form, and the
-// survivor list in ascending pool index.  Ends the generation.  Dynamic LDS
-// (niche_lds_need) holds the last front bucketed by reference point (design
-// indices [P], high words of their priorities [P]) and, when they fit, the
-// bucket's distances, the per-reference-point arrays and copies of front_of,
-// ref_of and the flags; otherwise those stay in the job's global scratch.
-__global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs) {
-    extern __shared__ __align__(16) uint8_t dyn[];
-    const DevJob &J = jobs[blockIdx.x];
-    SelState *st = J.st;
-    const int P = J.P, N = J.N, R = J.R, tid = threadIdx.x, NT = blockDim.x;
-    const int lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
-    const uint32_t gen = st->gen;
-    const int last = st->last;
-    __shared__ int red[16];
-    __shared__ int red64[16 * 64], s_lo, s_clo, s_hi;
-    __shared__ int s_carry;
-    const int tier = niche_tier(P, R);
-    const int Pa = niche_stride(P);
-    int *lb = (int *)dyn;
-    uint32_t *lp = (uint32_t *)(lb + Pa);
-    float *ld = tier >= 1 ? (float *)(lp + Pa) : nullptr;  // bucket distances when staged
-    int *cnt = tier >= 1 ? (int *)(ld + Pa) : J.niche;
-    int *rem = cnt + R, *off = rem + R, *fill = off + R, *kp = fill + R, *lvl0 = kp + R;
-    uint64_t *thr = (uint64_t *)(lvl0 + R);  // spare u64 [R] (keeps the layout's alignment)
-    const int *fo = J.front_of, *ro = J.ref_of;
-    uint8_t *flag = J.flag;
-    if (tier >= 2) {
-        int *fo2 = (int *)(thr + R), *ro2 = fo2 + Pa;
-        uint8_t *fl2 = (uint8_t *)(ro2 + Pa);
-        for (int d = tid; d < P; d += NT) {
-            fo2[d] = J.front_of[d];
-            ro2[d] = J.ref_of[d];
-        }
-        fo = fo2;
-        ro = ro2;
-        flag = fl2;
-    }
-    if (!st->n_cut) {
+// Nearest reference line of every ranked design (sorting.cuh:1105-1180): one
+// wave per design, D designs a workgroup (item = (job, first design)); lanes
+// take reference lines r = lane, lane + 64, ... and the wave keeps the
+// smallest distance, ties to the lowest index (the first minimum of a sweep
+// in index order).  The niche arrays are folded in: designs of the fronts
+// before the last count toward their niche, those of the last front toward
+// its bucket, with the bucket's nearest and lowest designs as 64-bit minima;
+// the workgroup sums them in LDS first (when R fits) and adds each nonzero
+// entry to the job's arrays once.
+template <int D>
+__global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__ jobs,
+                                                      const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const DevJob &J = jobs[it.x];
+    const SelState *st = J.st;
+    if (st->n_cut) return;  // the whole job: no niching
+    constexpr int NT = 64 * D;
+    const int tid = threadIdx.x, lane = tid & 63, R = J.R, P = J.P, last = st->last;
+    const int d = it.y + (tid >> 6);
+    const NicheView g = niche_view(J.niche, R);
+    const bool agg = R <= NICHE_STAGE;
+    __shared__ int2 l_cr[NICHE_STAGE];
+    __shared__ unsigned long long l_near[NICHE_STAGE], l_low[NICHE_STAGE];
+    if (agg)
         for (int r = tid; r < R; r += NT) {
-            cnt[r] = 0;
-            rem[r] = 0;
-            fill[r] = 0;
+            l_cr[r] = make_int2(0, 0);
+            l_near[r] = ~0ull;
+            l_low[r] = ~0ull;
         }
-        __syncthreads();
-        for (int d = tid; d < P; d += NT) {
-            const int f = fo[d];
-            if (f >= 0) atomicAdd(f < last ? &cnt[ro[d]] : &rem[ro[d]], 1);
+    __syncthreads();
+    const int fr = d < P ? gload(J.front_of + d) : -1;
+    if (fr >= 0) {  // whole wave
+        const float *o = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
+        float f[NOBJ];
+#pragma unroll
+        for (int k = 0; k < NOBJ; ++k) {
+            const float den = st->nadir[k] - st->ideal[k];
+            f[k] = (den == 0.f) ? 0.f : (o[k] - st->ideal[k]) / den;
         }
-        if (tid == 0) s_carry = 0;
-        __syncthreads();
-        for (int r0 = 0; r0 < R; r0 += NT) {  // off = exclusive scan of rem
-            const int r = r0 + tid;
-            int total;
-            const int ex = block_scan_excl(r < R ? rem[r] : 0, red, &total);
-            if (r < R) off[r] = s_carry + ex;
-            __syncthreads();
-            if (tid == 0) s_carry += total;
-            __syncthreads();
-        }
-        const int total_last = s_carry;  // designs of the last front
-        for (int d = tid; d < P; d += NT) {
-            if (fo[d] == last) {
-                const int r = ro[d];
-                const int slot = off[r] + atomicAdd(&fill[r], 1);
-                lb[slot] = d;
-                lp[slot] = (uint32_t)(niche_prio(J, gen, d) >> 32);
-                if (ld) ld[slot] = J.dist[d];
+        float best = FLT_MAX;
+        int br = 0;
+        for (int r = lane; r < R; r += 64) {
+            const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
+            if (dd < best) {
+                best = dd;
+                br = r;
             }
         }
-        __syncthreads();
-        const int need = N - st->base;
-        // picks made through level i: C(i) = sum_r clamp(i + 1 - cnt[r], 0, rem[r]),
-        // nondecreasing in i; the final level is the smallest i with C(i) >= need
-        // (or P).  A 64-way search: each pass evaluates C at 64 evenly spaced
-        // levels of the open interval (lo, hi), lane q of every wave one level
-        // over its wave's share of the reference points, keeping C(lo) < need
-        // and C(hi) >= need or hi = P (two passes for P up to 4224).
-        int lo = -1, clo = 0, hi = P;
-        while (hi - lo > 1) {
-            const int step = (hi - lo + 64) / 65;
-            const int iq = min(lo + step * (lane + 1), hi);
-            int acc = 0;
-            for (int r = wid; r < R; r += nw) acc += min(max(iq + 1 - cnt[r], 0), rem[r]);
-            red64[wid * 64 + lane] = acc;
-            __syncthreads();
-            if (wid == 0) {
-                int C = 0;
-                for (int w = 0; w < nw; ++w) C += red64[w * 64 + lane];
-                const bool valid = lo + step * (lane + 1) < hi;
-                const uint64_t hit = __ballot(valid && C >= need), val = __ballot(valid);
-                int nlo = lo, nclo = clo, nhi = hi;
-                if (hit) {  // the first probe at or past need; the one before it is below
-                    const int qf = __builtin_ctzll(hit);
-                    const int cprev = __shfl(C, qf > 0 ? qf - 1 : 0, 64);
-                    nhi = lo + step * (qf + 1);
-                    if (qf > 0) {
-                        nlo = lo + step * qf;
-                        nclo = cprev;
-                    }
-                } else {  // every probe is below need: the last one becomes lo
-                    const int ql = 63 - __builtin_clzll(val);
-                    nclo = __shfl(C, ql, 64);
-                    nlo = lo + step * (ql + 1);
-                }
-                if (lane == 0) {
-                    s_lo = nlo;
-                    s_clo = nclo;
-                    s_hi = nhi;
-                }
+#pragma unroll
+        for (int o2 = 32; o2 > 0; o2 >>= 1) argmin_pair(best, br, __shfl_xor(best, o2, 64), __shfl_xor(br, o2, 64));
+        if (lane == 0) {
+            J.ref_of[d] = br;
+            J.dist[d] = best;
+            int2 *cr = agg ? &l_cr[br] : g.cr + br;
+            if (fr < last) {
+                atomicAdd(&cr->x, 1);
+            } else {
+                atomicAdd(&cr->y, 1);
+                if (best <= FLT_MAX) atomicMin(agg ? &l_near[br] : g.nearest + br, asf_key(best, d));
+                atomicMin(agg ? &l_low[br] : g.lowest + br,
+                          ((unsigned long long)d << 32) | __float_as_uint(best));
+                J.prio[d] = (uint32_t)(niche_prio(J, st->gen, d) >> 32);
             }
-            __syncthreads();
-            lo = s_lo;
-            clo = s_clo;
-            hi = s_hi;
         }
-        const int istar = hi, prevC = clo;
-        const int need_final = need - prevC;
-        if (tid == 0) s_carry = 0;
-        __syncthreads();
-        for (int r0 = 0; r0 < R; r0 += NT) {  // final level: the first eligible refs in index order
-            const int r = r0 + tid;
-            const int e = (r < R) && cnt[r] <= istar && istar < cnt[r] + rem[r];
-            int total;
-            const int ex = block_scan_excl(e, red, &total);
-            if (r < R) kp[r] = min(max(istar - cnt[r], 0), rem[r]) + ((e && s_carry + ex < need_final) ? 1 : 0);
-            __syncthreads();
-            if (tid == 0) s_carry += total;
-            __syncthreads();
-        }
-        // level 0 of an empty niche takes its nearest candidate, ties to the
-        // lowest index; a sweep in ascending index starts from the lowest index
-        // and keeps it when its distance is NaN or nothing compares below
-        // FLT_MAX.  A group of gw lanes per reference point: 16 (four points
-        // per wave) when the last front's buckets hold 8 designs or fewer on
-        // average, else the whole wave; both minima are folded over the group
-        // by butterflies (the (distance, index) minimum never takes a NaN, so
-        // its order does not matter).
-        const int gw = total_last <= 8 * R ? 16 : 64, gpw = 64 / gw;
-        for (int r0 = wid * gpw; r0 < R; r0 += nw * gpw) {
-            const int r = r0 + lane / gw, gl = lane & (gw - 1);
-            const bool pick = r < R && kp[r] > 0 && cnt[r] == 0;
-            float bd = FLT_MAX, md = 0.f;
-            int bi = 0x7FFFFFFF, mi = 0x7FFFFFFF;
-            if (pick) {
-                const int *b = lb + off[r];
-                const int nb = rem[r];
-                for (int t = gl; t < nb; t += gw) {
-                    const int e = b[t];
-                    const float de = ld ? ld[off[r] + t] : J.dist[e];
-                    argmin_pair(bd, bi, de, e);
-                    if (e < mi) {
-                        mi = e;
-                        md = de;
-                    }
-                }
-            }
-            for (int o = gw >> 1; o > 0; o >>= 1) {  // every lane takes part
-                const float ov = __shfl_xor(bd, o, 64), omd = __shfl_xor(md, o, 64);
-                const int oi = __shfl_xor(bi, o, 64), omi = __shfl_xor(mi, o, 64);
-                argmin_pair(bd, bi, ov, oi);
-                if (omi < mi) {
-                    mi = omi;
-                    md = omd;
-                }
-            }
-            if (gl == 0 && r < R) lvl0[r] = pick ? ((bi == 0x7FFFFFFF || isnan(md)) ? mi : bi) : -1;
-        }
-        for (int d = tid; d < P; d += NT) {
-            const int f = fo[d];
-            flag[d] = (f >= 0 && f < last);
-        }
-        __syncthreads();
-        // later levels take the kk smallest priorities of the rest of the
-        // bucket: one thread per bucket slot (consecutive slots share a bucket,
-        // so a wave's loop is as long as its own buckets)
-        for (int sl = tid; sl < total_last; sl += NT) {
-            const int d = lb[sl], r = ro[d], l0 = lvl0[r];
-            uint8_t sel = (d == l0);
-            if (!sel) {
-                const int kk = kp[r] - (l0 >= 0 ? 1 : 0);
-                const int nb = rem[r] - (l0 >= 0 ? 1 : 0);
-                if (kk >= nb) {
-                    sel = kk > 0;
-                } else if (kk > 0) {
-                    // rank among the bucket without the level-0 pick; the
-                    // design's own key is not below itself
-                    const uint64_t my = ((uint64_t)lp[sl] << 32) | (uint32_t)d;
-                    const int *b = lb + off[r];
-                    const uint32_t *bp = lp + off[r];
-                    const int nr = rem[r];
-                    int rank = 0;
-#pragma unroll 4
-                    for (int u = 0; u < nr; ++u) {
-                        const int e = b[u];
-                        rank += (int)(((((uint64_t)bp[u] << 32) | (uint32_t)e) < my) & (e != l0));
-                    }
-                    sel = rank < kk;
-                }
-            }
-            if (sel) flag[d] = 1;
-        }
-    } else {
-        __syncthreads();
-        for (int d = tid; d < P; d += NT) flag[d] = fo[d] >= 0;
     }
+    if (!agg) return;
     __syncthreads();
-    // survivors in ascending pool index
-    const int ch = (P + NT - 1) / NT, d0 = tid * ch, d1 = min(P, d0 + ch);
-    int c = 0;
-    for (int d = d0; d < d1; ++d) c += flag[d];
-    int total;
-    int pos = block_scan_excl(c, red, &total);
-    for (int d = d0; d < d1; ++d)
-        if (flag[d] && pos < N) J.sorted[pos++] = d;
-    __syncthreads();
-    if (tid == 0) st->gen = gen + 1;
+    for (int r = tid; r < R; r += NT) {
+        const int2 c = l_cr[r];
+        if (c.x) atomicAdd(&g.cr[r].x, c.x);
+        if (c.y) {
+            atomicAdd(&g.cr[r].y, c.y);
+            if (l_near[r] != ~0ull) atomicMin(g.nearest + r, l_near[r]);
+            atomicMin(g.lowest + r, l_low[r]);
+        }
+    }
 }
 
-// Initial state after k_design<INIT>: identity survivor list, extreme points
+// The levels of referenceBasedSorting (sorting.cuh:1182-1279) in closed form,
+// by wave 0 of a workgroup (every thread calls it; the niche arrays are in
+// LDS or global memory).  Picks made through level i: C(i) = sum_r
+// clamp(i + 1 - cnt[r], 0, rem[r]), nondecreasing in i; the final level is
+// the smallest i with C(i) >= need (or P), found by bisection, where the
+// first eligible reference points in index order take one more.  Per
+// reference point: its picks, its level-0 pick (an empty niche takes its
+// nearest design, ties to the lowest index; a sweep in ascending index keeps
+// the lowest index when that design's distance is NaN or nothing compares
+// below FLT_MAX), and, for a bucket only part of whose rest is taken (a cut
+// bucket), its offset in the cut list.  Returns the cut list's size.
+__device__ __forceinline__ int niche_levels(const DevJob &J, const int2 *cr, const unsigned long long *nearest,
+                                            const unsigned long long *lowest, int *kp, int *lvl0, int *coff) {
+    const int P = J.P, N = J.N, R = J.R, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ int s_ncut;
+    if (wid == 0) {
+        const int need = N - J.st->base;
+        // up to 256 reference points: four a lane, in registers
+        const bool small = R <= 256;
+        int2 rc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int r = lane + 64 * k;
+            rc[k] = small && r < R ? cr[r] : make_int2(0, 0);
+        }
+        int lo = -1, clo = 0, hi = P;  // C(lo) < need; C(hi) >= need or hi = P
+        while (hi - lo > 1) {
+            const int mid = lo + ((hi - lo) >> 1);
+            int acc = 0;
+            if (small) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc += min(max(mid + 1 - rc[k].x, 0), rc[k].y);
+            } else {
+                for (int r = lane; r < R; r += 64) {
+                    const int2 c = cr[r];
+                    acc += min(max(mid + 1 - c.x, 0), c.y);
+                }
+            }
+            const int C = wave_sum_i32(acc);
+            if (C >= need) {
+                hi = mid;
+            } else {
+                lo = mid;
+                clo = C;
+            }
+        }
+        const int istar = hi, need_final = need - clo;
+        int carry = 0, cut_carry = 0;
+        for (int r0 = 0; r0 < R; r0 += 64) {
+            const int r = r0 + lane;
+            const int2 c = r < R ? cr[r] : make_int2(0, 0);
+            const bool e = r < R && c.x <= istar && istar < c.x + c.y;
+            const uint64_t eb = __ballot(e);
+            const int ex = carry + (int)__popcll(eb & ((1ull << lane) - 1ull));
+            const int k = min(max(istar - c.x, 0), c.y) + ((e && ex < need_final) ? 1 : 0);
+            int l0 = -1;
+            if (r < R && k > 0 && c.x == 0) {
+                const unsigned long long nk = nearest[r], lk = lowest[r];
+                const int low = (int)(lk >> 32);
+                const int bi = nk == ~0ull ? 0x7FFFFFFF : (int)(uint32_t)nk;
+                l0 = (bi == 0x7FFFFFFF || isnan(__uint_as_float((uint32_t)lk))) ? low : bi;
+            }
+            const int h = l0 >= 0 ? 1 : 0;
+            const int cut = (r < R && k - h > 0 && k - h < c.y - h) ? c.y : 0;
+            const int incl = wave_scan_i32(cut);  // inclusive prefix of the cut sizes over the wave
+            if (r < R) {
+                kp[r] = k;
+                lvl0[r] = l0;
+                coff[r] = cut ? cut_carry + incl - cut : -1;
+            }
+            carry += (int)__popcll(eb);
+            cut_carry += __builtin_amdgcn_readlane(incl, 63);
+        }
+        if (lane == 0) s_ncut = cut_carry;
+    }
+    __syncthreads();
+    return s_ncut;
+}
+
+// niche_levels for jobs whose niche arrays do not fit in LDS (R above
+// NICHE_STAGE): one workgroup per job, on the arrays in global memory.
+__global__ __launch_bounds__(256) void k_levels(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    if (J.st->n_cut || J.R <= NICHE_STAGE) return;
+    const NicheView g = niche_view(J.niche, J.R);
+    const int ncut = niche_levels(J, g.cr, g.nearest, g.lowest, g.kp, g.lvl0, g.coff);
+    if (threadIdx.x == 0) J.acc->ncut = ncut;
+}
+
+// Every design of the last front, one a thread (grid: jobs x chunks of NT
+// designs of the largest pool).  Each workgroup first settles the levels
+// itself (niche_levels on the niche arrays staged in LDS; workgroup 0 also
+// keeps them for k_niche) when R fits, else reads what k_levels wrote.  A
+// design is selected when it is its niche's level-0 pick or when the rest of
+// its bucket is taken whole (one atomic OR per wave and word of the survivor
+// mask); the designs of a cut bucket get slots in the cut list, bucket by
+// bucket, keyed by (priority, design), for k_niche to rank -- the workgroup
+// counts its designs per bucket in LDS and claims each bucket's share with
+// one atomic (when R fits).
+__global__ __launch_bounds__(1024) void k_place(const DevJob *__restrict__ jobs) {
+    const DevJob &J = jobs[blockIdx.x];
+    const SelState *st = J.st;
+    const int P = J.P, R = J.R, NT = blockDim.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int d0 = blockIdx.y * NT;
+    if (st->n_cut || d0 >= P) return;
+    const NicheView g = niche_view(J.niche, R);
+    const bool staged = R <= NICHE_STAGE;
+    __shared__ int2 s_cr[NICHE_STAGE];
+    __shared__ unsigned long long s_near[NICHE_STAGE], s_low[NICHE_STAGE];
+    __shared__ int s_kp[NICHE_STAGE], s_l0[NICHE_STAGE], s_co[NICHE_STAGE], l_cnt[NICHE_STAGE], l_base[NICHE_STAGE];
+    const int *kp = g.kp, *lvl0 = g.lvl0, *coff = g.coff;
+    if (staged) {
+        for (int r = tid; r < R; r += NT) {
+            s_cr[r] = g.cr[r];
+            s_near[r] = gload(g.nearest + r);
+            s_low[r] = gload(g.lowest + r);
+            l_cnt[r] = 0;
+        }
+        __syncthreads();
+        const int ncut = niche_levels(J, s_cr, s_near, s_low, s_kp, s_l0, s_co);
+        if (blockIdx.y == 0) {
+            for (int r = tid; r < R; r += NT) {
+                gstore(g.kp + r, s_kp[r]);
+                gstore(g.lvl0 + r, s_l0[r]);
+                gstore(g.coff + r, s_co[r]);
+            }
+            if (tid == 0) J.acc->ncut = ncut;
+        }
+        kp = s_kp;
+        lvl0 = s_l0;
+        coff = s_co;
+    }
+    const int d = d0 + tid;
+    const bool mine = d < P && gload(J.front_of + d) == st->last;
+    int r = 0, co = -1, loc = 0;
+    bool sel = false;
+    if (mine) {
+        r = gload(J.ref_of + d);
+        const int l0 = lvl0[r];
+        co = coff[r];
+        sel = d == l0;
+        if (co >= 0)
+            loc = staged ? atomicAdd(&l_cnt[r], 1) : 0;
+        else if (!sel)  // not cut: the rest of the bucket is taken whole, or nothing of it
+            sel = kp[r] - (l0 >= 0 ? 1 : 0) > 0;
+    }
+    if (staged) {
+        __syncthreads();
+        for (int q = tid; q < R; q += NT)
+            if (l_cnt[q]) l_base[q] = atomicAdd(g.fill + q, l_cnt[q]);
+        __syncthreads();
+    }
+    if (mine && co >= 0) {
+        const int slot = co + (staged ? l_base[r] + loc : atomicAdd(g.fill + r, 1));
+        gstore(J.cutl + slot, ((uint64_t)gload(J.prio + d) << 32) | (uint32_t)d);
+        gstore(J.cutr + slot, r);
+    }
+    const uint64_t b = __ballot(sel);  // a wave's 64 designs are one word of the mask
+    if (lane == 0 && b) atomicOr((unsigned long long *)J.selm + (d0 >> 6) + wid, b);
+}
+
+// The designs of the cut buckets, one a thread in cut-list order (grid: jobs
+// x chunks of NT designs of the largest pool): a design other than its
+// niche's level-0 pick is selected when fewer than kk of the rest of its
+// bucket (the bucket without the level-0 pick) have a smaller (priority,
+// design) key.  A workgroup stages the keys of every bucket its slots touch
+// in LDS (dynamic, 8 bytes a design of the pool).  Workgroup 0 ends the
+// generation (nothing here reads the generation count).
+__global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t dyn[];
+    uint64_t *keys = (uint64_t *)dyn;
+    const DevJob &J = jobs[blockIdx.x];
+    SelState *st = J.st;
+    const int R = J.R, NT = blockDim.x, tid = threadIdx.x;
+    const int s0 = blockIdx.y * NT;
+    if (blockIdx.y == 0 && tid == 0) st->gen = st->gen + 1u;
+    if (st->n_cut || s0 >= J.acc->ncut) return;
+    const int ncut = J.acc->ncut;
+    const NicheView g = niche_view(J.niche, R);
+    __shared__ int s_a, s_b;
+    const int s = s0 + tid;
+    uint64_t my = 0;
+    int r = 0;
+    if (s < ncut) {
+        my = gload(J.cutl + s);
+        r = gload(J.cutr + s);
+    }
+    if (tid == 0) s_a = gload(g.coff + r);  // the first slot's bucket starts here
+    if (s == min(s0 + NT, ncut) - 1) s_b = gload(g.coff + r) + g.cr[r].y;
+    __syncthreads();
+    const int a = s_a, nk = s_b - s_a;
+    for (int u = tid; u < nk; u += NT) keys[u] = gload(J.cutl + a + u);
+    __syncthreads();
+    if (s < ncut) {
+        const int d = (int)(uint32_t)my, l0 = gload(g.lvl0 + r);
+        if (d != l0) {
+            const int kk = gload(g.kp + r) - (l0 >= 0 ? 1 : 0);
+            const int off = gload(g.coff + r) - a, nr = g.cr[r].y;
+            int rank = 0;
+#pragma unroll 4
+            for (int u = 0; u < nr; ++u) rank += (int)(keys[off + u] < my);
+            // the level-0 pick of the bucket is not part of the rest
+            if (l0 >= 0 && ((((uint64_t)gload(J.prio + l0)) << 32) | (uint32_t)l0) < my) --rank;
+            if (rank < kk) atomicOr((unsigned long long *)J.selm + (d >> 6), 1ull << (d & 63));
+        }
+    }
+}
+
+// Initial state after k_design<INIT>: the first N designs survive, extreme points
 // from the first designs (main.cu:101, 106-113).
 __global__ void k_init_state(const DevJob *__restrict__ jobs) {
     const DevJob &J = jobs[blockIdx.x];
     SelState *st = J.st;
-    for (int i = threadIdx.x; i < J.N; i += blockDim.x) J.sorted[i] = i;
+    for (int w = threadIdx.x; w < J.W; w += blockDim.x) {
+        const int n = J.N - 64 * w;
+        J.selm[w] = n >= 64 ? ~0ull : (n > 0 ? (1ull << n) - 1ull : 0ull);
+    }
     for (int i = threadIdx.x; i < J.P; i += blockDim.x) J.front_of[i] = -1;
     if (threadIdx.x < NOBJ * NOBJ) {
         const int e = threadIdx.x / NOBJ, k = threadIdx.x % NOBJ;
@@ -1989,11 +2197,13 @@ __global__ void k_init_state(const DevJob *__restrict__ jobs) {
     }
 }
 
-// Clear the dominator counts before a standalone sort (the generation loop
-// clears them inside k_design<OFFSPRING>).
+// Clear the dominator counts and selection accumulators before a standalone
+// sort (the generation loop clears them
+// inside k_design<OFFSPRING>).
 __global__ void k_zero_np(const DevJob *__restrict__ jobs) {
     const DevJob &J = jobs[blockIdx.x];
     for (int i = threadIdx.x; i < J.P; i += blockDim.x) J.np[i] = 0;
+    if (threadIdx.x < ACC_WORDS) ((uint32_t *)J.acc)[threadIdx.x] = acc_clear_word(threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2056,8 +2266,12 @@ hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool f
     }
     return hipGetLastError();
 }
-hipError_t launch_fronts(const DevJob *jobs, int njobs, int threads, hipStream_t s) {
+hipError_t launch_fronts(const DevJob *jobs, int njobs, int max_pool, int threads, hipStream_t s) {
     if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_front0, dim3(njobs, (max_pool + FRONT0_THREADS - 1) / FRONT0_THREADS), dim3(FRONT0_THREADS), 0, s,
+                       jobs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fronts, dim3(njobs), dim3(threads), 0, s, jobs);
     return hipGetLastError();
 }
@@ -2078,7 +2292,7 @@ bool shapes_valid(const Shapes &h) {
            (h.assoc_designs == 4 || h.assoc_designs == 8 || h.assoc_designs == 16) &&
            (h.niche_threads == 256 || h.niche_threads == 512 || h.niche_threads == 1024);
 }
-size_t niche_lds_bytes(int P, int R) { return niche_lds_need(P, R); }
+size_t niche_lds_bytes(int P) { return (size_t)8 * P; }
 
 // The dynamic-LDS limit of a kernel is an attribute of the process, shared by
 // every job, batch and captured graph that launches it, so it only ever
@@ -2101,9 +2315,19 @@ hipError_t set_niche_lds_limit(size_t bytes) {
         return hipFuncSetAttribute((const void *)k_niche, hipFuncAttributeMaxDynamicSharedMemorySize, b);
     });
 }
-hipError_t launch_niche(const DevJob *jobs, int njobs, size_t lds, int threads, hipStream_t s) {
+hipError_t launch_niche(const DevJob *jobs, int njobs, int max_pool, int max_refs, int threads, hipStream_t s) {
     if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_niche, dim3(njobs), dim3(threads), lds, s, jobs);
+    hipError_t e = hipSuccess;
+    if (max_refs > NICHE_STAGE) {
+        hipLaunchKernelGGL(k_levels, dim3(njobs), dim3(256), 0, s, jobs);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid(njobs, (max_pool + threads - 1) / threads);
+    hipLaunchKernelGGL(k_place, grid, dim3(threads), 0, s, jobs);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_niche, grid, dim3(threads), niche_lds_bytes(max_pool), s, jobs);
     return hipGetLastError();
 }
 hipError_t launch_init_state(const DevJob *jobs, int njobs, hipStream_t s) {

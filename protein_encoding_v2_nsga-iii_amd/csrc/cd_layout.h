@@ -2,6 +2,7 @@
 // shared by the kernels and the host API.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace cd {
@@ -21,7 +22,6 @@ constexpr int MAX_W = 256;               // dominance words per row: pools up to
 constexpr uint32_t TAG_INIT = 0x494E4954u;
 constexpr uint32_t TAG_TYPE = 0x54595045u;
 constexpr uint32_t TAG_MUT = 0x4D555441u;
-constexpr uint32_t TAG_MUTB = 0x4D555442u;
 constexpr uint32_t TAG_SL = 0x534C5452u;
 constexpr uint32_t TAG_NICHE = 0x4E494348u;
 
@@ -52,29 +52,60 @@ struct SelState {
     float ideal[NOBJ], worst[NOBJ], nadir[NOBJ];
 };
 
-// Dynamic LDS of k_niche: the bucketed last front (design indices and
-// priority high words, 8P bytes) always; with tier >= 1 also the bucket's
-// distances (4P) and the per-reference-point arrays (six int arrays and one
-// u64 array, 32R); with tier 2 also copies of front_of, ref_of and the
-// survivor flags (9P).  What is not in LDS lives in the job's global scratch.
-constexpr size_t NICHE_LDS_CAP = 150 * 1024;
+// Per-generation accumulators of the selection, filled with atomics by the
+// workgroups of k_dominance, k_front0 and k_levels and cleared with the
+// dominator counts before every sort (acc_clear_word).  Float extrema are kept
+// as order-preserving bits (ord_bits) so that integer atomics combine them.
+struct SelAcc {
+    uint32_t lo[NOBJ];    // smallest value of each objective over the pool
+    uint32_t hi[NOBJ];    // largest value over the pool
+    uint32_t f0hi[NOBJ];  // largest value over the first front
+    int32_t f0;           // size of the first front
+    int32_t ncut;         // designs of the cut buckets (those ranked by priority)
+    unsigned long long asf[NOBJ];  // per axis: (ASF bits << 32 | design) minimum over the first front
+};
+constexpr int ACC_WORDS = (int)(sizeof(SelAcc) / 4);
+static_assert(sizeof(SelAcc) % 8 == 0, "SelAcc is cleared in 32-bit words and holds u64 minima");
 
-// P-sized arrays are laid out with an even stride so that the u64 array that
-// follows them stays 8-byte aligned.
-__host__ __device__ inline int niche_stride(int P) { return (P + 1) & ~1; }
-
-__host__ __device__ inline int niche_tier(int P, int R) {
-    const size_t Pa = niche_stride(P);
-    if (21 * Pa + (size_t)32 * R + 16 <= NICHE_LDS_CAP) return 2;
-    if (12 * Pa + (size_t)32 * R <= NICHE_LDS_CAP) return 1;
-    return 0;
+// Order-preserving bits of a float: unsigned comparison of ord_bits(a) and
+// ord_bits(b) orders a and b as floats (-0 below +0; not for NaN).
+__host__ __device__ inline uint32_t ord_bits(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float ord_float(uint32_t u) {
+    return __builtin_bit_cast(float, (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+// The value word i of a cleared SelAcc holds: minima start at all ones,
+// everything else at zero.
+__host__ __device__ inline uint32_t acc_clear_word(int i) {
+    const int lo0 = (int)(offsetof(SelAcc, lo) / 4), asf0 = (int)(offsetof(SelAcc, asf) / 4);
+    return ((i >= lo0 && i < lo0 + NOBJ) || i >= asf0) ? 0xFFFFFFFFu : 0u;
 }
 
-__host__ __device__ inline size_t niche_lds_need(int P, int R) {
-    const int t = niche_tier(P, R);
-    const size_t Pa = niche_stride(P);
-    return 8 * Pa + (t >= 1 ? 4 * Pa + (size_t)32 * R : 0) + (t >= 2 ? 9 * Pa + 16 : 0);
+// Per-reference-point arrays of the niching (DevJob::niche, R entries each):
+// int2 (designs of the earlier fronts, designs of the last front); int picks,
+// level-0 pick, offset of a cut bucket in the cut list (-1 when not cut), its
+// fill count; u64 (distance bits << 32 | design) minimum and (design << 32 |
+// distance bits) minimum over the bucket (its nearest and its lowest design).
+struct NicheView {
+    int2 *cr;
+    int32_t *kp, *lvl0, *coff, *fill;
+    unsigned long long *nearest, *lowest;
+};
+constexpr int NICHE_INTS = 10;  // 32-bit words per reference point
+__host__ __device__ inline NicheView niche_view(int32_t *base, int R) {
+    NicheView v;
+    v.cr = (int2 *)base;
+    v.kp = base + 2 * R;
+    v.lvl0 = base + 3 * R;
+    v.coff = base + 4 * R;
+    v.fill = base + 5 * R;
+    v.nearest = (unsigned long long *)(base + 6 * R);
+    v.lowest = (unsigned long long *)(base + 8 * R);
+    return v;
 }
+constexpr int NICHE_STAGE = 1024;  // reference points whose niche arrays fit in a workgroup's LDS
 
 // Device view of one job.  Pools ping-pong on generation parity: generation
 // g reads buffer g&1 and writes buffer (g+1)&1.
@@ -103,16 +134,20 @@ struct DevJob {
     CdsStat *cst[2];       // [P][m]
     uint2 *planes;         // [P][m][2][scan_words(3L)] bit planes (sequence, reverse complement) of the CDS
                            // k_design evaluated, read by k_stem
-    int32_t *sorted;       // [N]
     int32_t *front_of;     // [P]
     int32_t *np;           // [P] dominator counts
     int32_t *stem_max;     // [P] longest stem of the design (k_design: unchanged CDS, k_stem: atomic maximum)
     uint64_t *dom;         // [P][W]: bit i of row j set when i dominates j
     int32_t *ref_of;       // [P]
     float *dist;           // [P]
-    int32_t *niche;        // [8][R]: count, rem, off, fill, kpick, level-0 pick, then u64 [R] pick threshold
-    uint8_t *flag;         // [P]
+    int32_t *niche;        // NICHE_INTS x R words per reference point (NicheView); null for a sort alone
     SelState *st;
+    SelAcc *acc;
+    uint64_t *ranked;      // [W] first front (bit i of word w: design 64w + i)
+    uint64_t *selm;        // [W] survivors: the next generation's parents, in ascending pool index; null for a sort alone
+    uint32_t *prio;        // [P] high word of a last-front design's niche priority
+    uint64_t *cutl;        // [P] (priority << 32 | design) of the cut buckets' designs, bucket by bucket
+    int32_t *cutr;         // [P] the reference point of each cut-list entry
 };
 
 }  // namespace cd

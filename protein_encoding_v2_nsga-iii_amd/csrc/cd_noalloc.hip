@@ -29,19 +29,22 @@ constexpr size_t OFF_ST = OFF_JOB + a256(sizeof(cd::DevJob));
 constexpr size_t OFF_NP = OFF_ST + a256(sizeof(cd::SelState));
 constexpr size_t OFF_DOM = OFF_NP + a256((size_t)MAX_POOL * 4);
 constexpr size_t OFF_TILES = OFF_DOM + a256((size_t)MAX_POOL * MAX_WORDS * 8);
-constexpr size_t SCRATCH_BYTES = OFF_TILES + a256((size_t)MAX_TILES * sizeof(int4));
+constexpr size_t OFF_ACC = OFF_TILES + a256((size_t)MAX_TILES * sizeof(int4));
+constexpr size_t OFF_RANKED = OFF_ACC + a256(sizeof(cd::SelAcc));
+constexpr size_t SCRATCH_BYTES = OFF_RANKED + a256((size_t)MAX_WORDS * 8);
 static_assert(sizeof(cd::SelState) % 4 == 0, "selection state is cleared in words");
 
-// The scratch of cd_nds_noalloc on every GPU (about 8.3 MiB).  Host code uses
+// The scratch of cd_nds_noalloc on every GPU (about 8.4 MiB).  Host code uses
 // the address hipGetSymbolAddress returns for the current device, never this
 // name.
 static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
 
-// Descriptor, cleared dominator counts and selection state, and the
-// domination tiles (one workgroup).
+// Descriptor, cleared dominator counts, selection state and accumulators,
+// and the domination tiles (one workgroup).
 __global__ __launch_bounds__(1024) void k_noalloc_setup(cd::DevJob h, cd::DevJob *dj, int4 *tiles, int n_tiles,
                                                         int words) {
     for (int i = threadIdx.x; i < h.P; i += blockDim.x) h.np[i] = 0;
+    if (threadIdx.x < cd::ACC_WORDS) ((uint32_t *)h.acc)[threadIdx.x] = cd::acc_clear_word(threadIdx.x);
     for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
         tiles[t] = make_int4(0, (t / words) * cd::DOM_ROWS, t % words, 0);
     uint32_t *st = (uint32_t *)h.st;
@@ -100,12 +103,14 @@ extern "C" int cd_nds_noalloc(const float *obj, int32_t P, int32_t N, int32_t *f
     h.np = (int32_t *)(base + OFF_NP);
     h.dom = (uint64_t *)(base + OFF_DOM);
     h.st = (cd::SelState *)(base + OFF_ST);
+    h.acc = (cd::SelAcc *)(base + OFF_ACC);
+    h.ranked = (uint64_t *)(base + OFF_RANKED);
     cd::DevJob *dj = (cd::DevJob *)(base + OFF_JOB);
     int4 *tiles = (int4 *)(base + OFF_TILES);
     hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(1024), 0, s, h, dj, tiles, n_tiles, W);
     HIPCHK(hipGetLastError());
     HIPCHK(cd::launch_dominance(dj, tiles, n_tiles, false, nullptr, cd::DOM_ROWS, s));
-    HIPCHK(cd::launch_fronts(dj, 1, cd::SHAPES_JOB.fronts_threads, s));
+    HIPCHK(cd::launch_fronts(dj, 1, P, cd::SHAPES_JOB.fronts_threads, s));
     hipLaunchKernelGGL(k_noalloc_finish, dim3(1), dim3(64), 0, s, (const cd::SelState *)h.st, n_fronts);
     HIPCHK(hipGetLastError());
     return CD_OK;

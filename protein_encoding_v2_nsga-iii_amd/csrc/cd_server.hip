@@ -247,7 +247,7 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res) {
     if (e != hipSuccess) return e;
     const cdhost::Launch L{S->d_jobs,   (int)wl.descs.size(), S->d_off,      (int)wl.off.size(), S->d_dom,
                            (int)wl.dom.size(), S->d_asc,      (int)wl.asc.size(), wl.lds,         wl.stem_lds,
-                           wl.niche_lds, wl.mg,               wl.stem_parts,      S->d_slist,     S->d_sctl,
+                           wl.max_pool,  wl.max_refs,        wl.mg,               wl.stem_parts,      S->d_slist,     S->d_sctl,
                            wl.n_cds,     wl.n_small,          wl.lds_small,       sh};
     if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
     if (S->graph) (void)hipGraphDestroy(S->graph);
