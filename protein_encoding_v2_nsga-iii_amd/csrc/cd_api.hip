@@ -164,8 +164,8 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
         o.off = s.add((size_t)N * sizeof(int2));
         o.planes = s.add((size_t)P * m * cd::cds_plane_bytes(L));
         o.stem_max = s.add((size_t)P * 4);
-        o.slist = s.add((size_t)N * m * 4);
-        o.sctl = s.add(16);
+        o.slist = s.add(cd::stem_list_entries(N, m) * 4);
+        o.sctl = s.add((size_t)cd::STEM_MAX_GROUPS * 4);
     }
     for (int b = 0; b < (sel_only ? 1 : 2); ++b) o.obj[b] = s.add((size_t)P * 6 * 4);
     if (sel_only) o.obj[1] = o.obj[0];
@@ -221,7 +221,7 @@ int validate(const cd_job_desc *d) {
         if ((d->aa[k] == 20) != (k == d->L - 1)) return fail(CD_E_INVALID, "the stop (20) must be exactly the last codon");
     }
     if (3LL * d->L >= (1 << 20)) return fail(CD_E_INVALID, "sequence too long for stem positions");
-    if (cd::design_lds_bytes(d->L, d->m) > cd::LDS_BYTES || cd::stem_lds_bytes(d->L) > cd::LDS_BYTES)
+    if (cd::design_lds_bytes(d->L, d->m) > cd::LDS_BYTES || cd::stem_list_lds(cd::stem_lds_bytes(d->L)) > cd::LDS_BYTES)
         return fail(CD_E_INVALID, "m*L too large for one workgroup's LDS (160 KiB)");
     return CD_OK;
 }
@@ -353,7 +353,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
         for (int c = 0; c < J->N; ++c) off[c] = make_int2(0, c);
         up.add(o.off, off.data(), off.size());
         J->d_off = (int2 *)(base + o.off);
-        J->d_slist = (uint32_t *)(base + o.slist);  // the count in d_sctl starts at zero with the cleared allocation
+        J->d_slist = (uint32_t *)(base + o.slist);  // the counts in d_sctl start at zero with the cleared allocation
         J->d_sctl = (int32_t *)(base + o.sctl);
         J->lds = cd::design_lds_bytes(J->L, J->m);
         J->stem_lds = cd::stem_lds_bytes(J->L);
@@ -406,7 +406,8 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
                                hipMemcpyHostToDevice, J->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(J->stream);  // the staged bytes are released below
     if (e == hipSuccess && J->lds > 48 * 1024) e = cd::set_design_lds_limit(J->lds);
-    if (e == hipSuccess && J->stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(J->stem_lds);
+    if (e == hipSuccess && cd::stem_list_lds(J->stem_lds) > 48 * 1024)
+        e = cd::set_stem_lds_limit(cd::stem_list_lds(J->stem_lds));
     if (e == hipSuccess && cd::niche_lds_bytes(J->P) > 48 * 1024) e = cd::set_niche_lds_limit(cd::niche_lds_bytes(J->P));
     if (e != hipSuccess) {
         (void)hipFreeAsync(J->mem, J->stream);
@@ -440,20 +441,21 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s, hipEvent_t *ev) {
     // dominance, fronts, association, niching
     auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s) : hipSuccess; };
     hipError_t e = mark(0);
+    const cd::StemList sl = cd::stem_list_geom(w.n_off, w.mg);
     if (e == hipSuccess) {
         const int ns = w.n_small > 0 && w.n_small < w.n_off ? w.n_small : 0;
         if (ns)
-            e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off, ns, w.lds_small, s, w.stem_list, w.stem_ctl,
-                                  w.n_cds, 0);
+            e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off, ns, w.lds_small, s, w.stem_list, w.stem_ctl, sl, 0);
         if (e == hipSuccess)
             e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off + ns, w.n_off - ns, w.lds, s, w.stem_list,
-                                  w.stem_ctl, w.n_cds, ns);
+                                  w.stem_ctl, sl, ns);
     }
     if (e == hipSuccess) e = mark(1);
     if (e == hipSuccess)
-        e = cd::launch_stem_list(w.jobs, w.off, w.stem_list, w.stem_ctl, w.n_cds, w.sh.stem_waves, w.stem_lds, s);
+        e = cd::launch_stem_list(w.jobs, w.off, w.stem_list, w.stem_ctl, sl, w.sh.stem_waves, w.stem_lds, s);
     if (e == hipSuccess) e = mark(2);
-    if (e == hipSuccess) e = cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, w.stem_ctl, w.sh.dom_rows, s);
+    if (e == hipSuccess)
+        e = cd::launch_dominance(w.jobs, w.dom, w.n_dom, true, w.stem_ctl, sl.groups, w.sh.dom_rows, s);
     if (e == hipSuccess) e = mark(3);
     if (e == hipSuccess) e = cd::launch_fronts(w.jobs, w.njobs, w.max_pool, w.sh.fronts_threads, s);
     if (e == hipSuccess) e = mark(4);
@@ -503,7 +505,8 @@ void build_lists(cd_job *const *jobs, int count, const cd::Shapes &sh, WorkLists
 hipError_t set_lds_limits(const WorkLists &wl) {
     hipError_t e = hipSuccess;
     if (wl.lds > 48 * 1024) e = cd::set_design_lds_limit(wl.lds);
-    if (e == hipSuccess && wl.stem_lds > 48 * 1024) e = cd::set_stem_lds_limit(wl.stem_lds);
+    if (e == hipSuccess && cd::stem_list_lds(wl.stem_lds) > 48 * 1024)
+        e = cd::set_stem_lds_limit(cd::stem_list_lds(wl.stem_lds));
     if (e == hipSuccess && cd::niche_lds_bytes(wl.max_pool) > 48 * 1024)
         e = cd::set_niche_lds_limit(cd::niche_lds_bytes(wl.max_pool));
     return e;
@@ -890,7 +893,7 @@ static int run_selection(const float *obj, int32_t P, int32_t N, int32_t H, uint
     if (e == hipSuccess) e = hipMemcpy(J->h.st, &st, sizeof(st), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = cd::launch_zero_np(J->d_self, 1, J->stream);
     const cd::Shapes &sh = J->shapes;
-    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, sh.dom_rows, J->stream);
+    if (e == hipSuccess) e = cd::launch_dominance(J->d_self, J->d_dom, J->n_dom, false, nullptr, 0, sh.dom_rows, J->stream);
     if (e == hipSuccess) e = cd::launch_fronts(J->d_self, 1, J->P, sh.fronts_threads, J->stream);
     if (e == hipSuccess && full) e = cd::launch_associate(J->d_self, J->d_asc, J->n_asc, sh.assoc_designs, J->stream);
     if (e == hipSuccess && full) e = cd::launch_niche(J->d_self, 1, J->P, J->R, sh.niche_threads, J->stream);
@@ -988,9 +991,9 @@ int cd_batch_create(cd_job *const *jobs, int32_t count, cd_batch **out) {
     if (e == hipSuccess) e = hipMalloc(&B->d_off, sizeof(int2) * wl.off.size());
     if (e == hipSuccess) e = hipMalloc(&B->d_dom, sizeof(int4) * cap_dom);
     if (e == hipSuccess) e = hipMalloc(&B->d_asc, sizeof(int2) * cap_asc);
-    if (e == hipSuccess) e = hipMalloc(&B->d_slist, sizeof(uint32_t) * wl.n_cds);
-    if (e == hipSuccess) e = hipMalloc(&B->d_sctl, 16);
-    if (e == hipSuccess) e = hipMemset(B->d_sctl, 0, 16);
+    if (e == hipSuccess) e = hipMalloc(&B->d_slist, sizeof(uint32_t) * cd::stem_list_entries((int)wl.off.size(), wl.mg));
+    if (e == hipSuccess) e = hipMalloc(&B->d_sctl, sizeof(int32_t) * cd::STEM_MAX_GROUPS);
+    if (e == hipSuccess) e = hipMemset(B->d_sctl, 0, sizeof(int32_t) * cd::STEM_MAX_GROUPS);
     if (e == hipSuccess) e = hipMemcpy(B->d_jobs, wl.descs.data(), sizeof(cd::DevJob) * count, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_off, wl.off.data(), sizeof(int2) * wl.off.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_dom, wl.dom.data(), sizeof(int4) * wl.dom.size(), hipMemcpyHostToDevice);

@@ -28,8 +28,8 @@ struct cd_job {
     int2 *d_off = nullptr;
     int4 *d_dom = nullptr;        // domination tiles, room for the smallest tile (64 rows)
     int2 *d_asc = nullptr;        // association groups, room for the smallest group (4 designs)
-    uint32_t *d_slist = nullptr;  // changed CDS of a generation (N * m entries) for k_stem_list
-    int32_t *d_sctl = nullptr;    // its count, zero between generations (k_dominance clears it)
+    uint32_t *d_slist = nullptr;  // changed CDS of a generation (stem_list_entries) for k_stem_list
+    int32_t *d_sctl = nullptr;    // its counts (STEM_MAX_GROUPS), zero between generations (k_dominance clears them)
     int n_dom = 0, n_asc = 0;
     cd::Shapes shapes = cd::SHAPES_JOB;  // launch shapes of its generations (the lists above follow them)
     size_t lds = 0, stem_lds = 0;
@@ -47,7 +47,7 @@ struct cd_batch {
     int2 *d_off = nullptr;
     int4 *d_dom = nullptr;  // room for 64-row tiles
     int2 *d_asc = nullptr;  // room for 4-design groups
-    uint32_t *d_slist = nullptr;  // changed CDS of a generation (n_cds entries)
+    uint32_t *d_slist = nullptr;  // changed CDS of a generation (stem_list_entries)
     int32_t *d_sctl = nullptr;
     int n_off = 0, n_dom = 0, n_asc = 0, n_cds = 0;
     int n_small = 0;        // children first in d_off whose k_design fits six per CU
@@ -92,8 +92,8 @@ struct Launch {
     int max_refs;    // most reference points
     int mg;          // largest m
     int stem_parts;  // largest k_stem workgroups per CDS (initialisation and evaluation)
-    uint32_t *stem_list;  // changed CDS of a generation, n_cds entries at most
-    int32_t *stem_ctl;    // its count, zero between generations (k_dominance clears it)
+    uint32_t *stem_list;  // changed CDS of a generation, in sublists of consecutive children (stem_list_entries)
+    int32_t *stem_ctl;    // their counts, zero between generations (k_dominance clears them)
     int n_cds;            // CDS of all children: sum of N * m
     // the first n_small children (0: none apart) are of jobs whose k_design
     // LDS, at most lds_small (at most sh.design_split), they are launched

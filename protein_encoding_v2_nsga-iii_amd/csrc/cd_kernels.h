@@ -39,33 +39,58 @@ int stem_parts_of(int L);  // k_stem workgroups per CDS of L codons (1 .. STEM_P
 hipError_t set_design_lds_limit(size_t bytes);
 hipError_t set_stem_lds_limit(size_t bytes);
 
+// The changed CDS of an offspring generation, in sublists of consecutive
+// children (`group` of them, at least 64, at most STEM_MAX_GROUPS sublists),
+// each with its own count in stem_ctl, so that the children's appends spread
+// over many counters and the sublists in order keep the children list's
+// order (longest jobs first); a sublist holds `cap` entries.
+constexpr int STEM_MAX_GROUPS = 2048;
+struct StemList {
+    int group, groups, cap;
+};
+inline StemList stem_list_geom(int n_items, int mg) {
+    StemList g;
+    g.group = (n_items + STEM_MAX_GROUPS - 1) / STEM_MAX_GROUPS;
+    if (g.group < 64) g.group = 64;
+    g.groups = (n_items + g.group - 1) / g.group;
+    g.cap = g.group * mg;
+    return g;
+}
+// k_stem_list's dynamic LDS holds the sublists' prefix sums (at most this
+// many bytes) before it holds a CDS's planes and tables
+constexpr size_t STEM_PRE_BYTES = 4 * (STEM_MAX_GROUPS + 1);
+inline size_t stem_list_lds(size_t stem_lds) { return stem_lds > STEM_PRE_BYTES ? stem_lds : STEM_PRE_BYTES; }
+inline size_t stem_list_entries(int n_items, int mg) {
+    const StemList g = stem_list_geom(n_items, mg);
+    return (size_t)g.groups * g.cap;
+}
 // items: (job, design or child).  The offspring mode lists every changed CDS
-// in stem_list (entry ((item_base + item) << 4) | c, stem_cap entries at
-// most, the sum of m over the children list) and counts them in
-// stem_ctl[0]; the three are required there.  item_base: the position of
-// items[0] in the children list k_stem_list reads.
+// in the sublists (entry ((item_base + item) << 4) | c; sl =
+// stem_list_geom of the whole children list) and counts them in stem_ctl;
+// the three are required there.  item_base: the position of items[0] in the
+// children list k_stem_list reads.
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
-                         uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, int stem_cap = 0,
+                         uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, StemList sl = StemList{0, 0, 0},
                          int item_base = 0);
 // initialisation and evaluation: parts workgroups per (item, CDS c < mg),
 // parts the largest stem_parts_of of the jobs: stem of each CDS k_design marked
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,
                        hipStream_t s);
 // offspring generations: stem of each CDS the offspring k_design listed
-// (items as given to it), capacity the list's size, on as many workgroups
-// as are resident at once (STEM_CUS x the workgroups per CU that `waves`,
-// 5 to 8, and the LDS allow), at most capacity; launch_dominance with the
-// same stem_ctl clears the count
+// (items and sl as given to it), on as many workgroups as are resident at
+// once (STEM_CUS x the workgroups per CU that `waves`, 5 to 8, and the LDS
+// allow), at most the entries there can be; launch_dominance with the same
+// stem_ctl and sl.groups clears the counts
 hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
-                            const int32_t *stem_ctl, int capacity, int waves, size_t lds, hipStream_t s);
+                            const int32_t *stem_ctl, StemList sl, int waves, size_t lds, hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
 // items: (job, first row, word, 0), rows (64, 128 or 256) rows per item.  finish: the offspring generation, whose children's stem
 // objective (and stem records) k_dominance completes in place of
-// launch_stem_finish; stem_ctl (or null): the changed-CDS count of that
-// generation's k_design, cleared for the next one
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
-                            hipStream_t s);
+// launch_stem_finish; stem_ctl (or null): the ctl_n changed-CDS counts of
+// that generation's k_design, cleared for the next one
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int ctl_n,
+                            int rows, hipStream_t s);
 // k_front0 (grid: jobs x chunks of FRONT0_THREADS designs of the largest
 // pool, max_pool), then k_fronts (one workgroup of `threads` (256, 512 or
 // 1024) per job)

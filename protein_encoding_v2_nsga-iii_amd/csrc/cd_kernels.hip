@@ -981,12 +981,7 @@ __device__ int survivor_at(const uint64_t *selm, int W, int c) {
     for (int w0 = 0; w0 < W; w0 += 64) {
         const int w = w0 + lane;
         const uint64_t x = w < W ? gload(selm + w) : 0ull;
-        int incl = (int)__popcll(x);  // inclusive prefix of the words' counts
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
+        const int incl = wave_scan_i32((int)__popcll(x));  // inclusive prefix of the words' counts
         const uint64_t hit = __ballot(base + incl > c);
         if (hit) {
             const int q = (int)__builtin_ctzll(hit);
@@ -1007,7 +1002,7 @@ __device__ int survivor_at(const uint64_t *selm, int W, int c) {
             }
             return (w0 + q) * 64 + pos;
         }
-        base += __shfl(incl, 63, 64);
+        base += __builtin_amdgcn_readlane(incl, 63);
     }
     return 0;  // fewer than c + 1 survivors: not reached
 }
@@ -1023,8 +1018,8 @@ __device__ int survivor_at(const uint64_t *selm, int W, int c) {
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
-                                                uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_cap,
-                                                int item_base) {
+                                                uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_group,
+                                                int stem_cap, int item_base) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ DesignShared sh;
     const int2 it = items[blockIdx.x];
@@ -1192,13 +1187,13 @@ __global__ __launch_bounds__(256) void k_design(const DevJob *__restrict__ jobs,
             nd += sh.dirty[c] != 0;
             if (!sh.dirty[c]) lmax = max(lmax, cst[c].l);
         }
-        gstore(J.stem_max + out_slot, lmax);
-        atomicAdd(&J.st->cds_scanned, nd);
-        if (MODE == MODE_OFFSPRING && nd) {  // the changed CDS for k_stem_list
-            int e = atomicAdd(stem_ctl, (int)nd);
+        gstore(J.stem_max + out_slot, lmax);  // (k_dominance counts the offspring's changed CDS)
+        if (MODE == MODE_OFFSPRING && nd) {  // the changed CDS for k_stem_list, in the child's sublist
+            const int item = item_base + (int)blockIdx.x, k = item / stem_group;
+            int e = atomicAdd(stem_ctl + k, (int)nd);
             for (int c = 0; c < m; ++c)
                 if (sh.dirty[c] && e < stem_cap)
-                    gstore(stem_list + e++, ((uint32_t)(item_base + (int)blockIdx.x) << 4) | (uint32_t)c);
+                    gstore(stem_list + (size_t)k * stem_cap + e++, ((uint32_t)item << 4) | (uint32_t)c);
         }
     }
 }
@@ -1254,8 +1249,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
 // ---------------------------------------------------------------------------
 // k_stem_list (offspring generations): the changed CDS k_design listed,
-// entry (item << 4) | c, ctl[0] of them; gridDim.x workgroups take the
-// entries round-robin, one workgroup per CDS.  The parent's stem seeds the
+// entry (item << 4) | c, in `groups` sublists of cap entries with their
+// counts in ctl; the entries are numbered sublist by sublist (a prefix sum
+// of the counts), which keeps the children list's order, and gridDim.x
+// workgroups take them round-robin, one workgroup per CDS.  The parent's stem seeds the
 // q-gram search (stem_child).  Each entry's key goes into the CDS record and
 // its length into the design's longest stem (stem_max) with atomic maxima.
 // k_dominance clears ctl[0] for the next generation.
@@ -1263,30 +1260,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 template <int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_stem_list(
     const DevJob *__restrict__ jobs, const int2 *__restrict__ items, const uint32_t *__restrict__ list,
-    const int32_t *__restrict__ ctl, int cap) {
+    const int32_t *__restrict__ ctl, int groups, int cap) {
     extern __shared__ __align__(16) uint8_t dyn[];
     __shared__ uint64_t redu[16];
     __shared__ int32_t bcast[16];
-    const int count = min(gload(ctl), cap);
-    for (int e = (int)blockIdx.x; e < count; e += (int)gridDim.x) {
-        const uint32_t ent = gload(list + e);
-        const int2 it = items[ent >> 4];
-        const int c = (int)(ent & 15u);
-        const DevJob &J = jobs[it.x];
-        const int L = J.L, m = J.m, n = 3 * L;
-        int buf, slot;
-        item_slot(J, MODE_OFFSPRING, it.y, buf, slot);
-        CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
-        uint64_t *planes = (uint64_t *)dyn;  // the CDS's planes as k_design left them
-        const int nwords = 2 * scan_words(n);
-        const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
-        __syncthreads();  // the previous entry's LDS is no longer read
-        for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
+    // the sublists' prefix sums sit where the planes go: built before each
+    // chunk of 256 entries is fetched, overwritten by the planes after
+    int *pre = (int *)dyn;  // [groups + 1]
+    __shared__ int red[16], s_carry;
+    __shared__ uint32_t s_ent[256];  // the workgroup's next entries, fetched together
+    const int G = (int)gridDim.x;
+    for (int e0 = (int)blockIdx.x;; e0 += 256 * G) {
+        __syncthreads();  // the previous entries' planes are no longer read
+        if (threadIdx.x == 0) s_carry = 0;
+        for (int g0 = 0; g0 < groups; g0 += blockDim.x) {  // exclusive prefix of the counts
+            const int g = g0 + (int)threadIdx.x;
+            const int n = g < groups ? min(gload(ctl + g), cap) : 0;
+            int total;
+            const int ex = block_scan_excl(n, red, &total);
+            if (g < groups) pre[g] = s_carry + ex;
+            __syncthreads();
+            if (threadIdx.x == 0) s_carry += total;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) pre[groups] = s_carry;
         __syncthreads();
-        const uint64_t key = stem_child(n, (const uint2 *)planes, cs, bcast, redu, (uint16_t *)(planes + nwords));
-        if (threadIdx.x == 0 && key) {
-            atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
-            atomicMax(&J.stem_max[slot], (int)(key >> 40));
+        const int count = pre[groups];
+        if (e0 >= count) break;
+        {
+            const int e = e0 + (int)threadIdx.x * G;
+            if (e < count) {
+                int k = 0;  // the last sublist whose entries start at or before e
+                for (int h = STEM_MAX_GROUPS / 2; h > 0; h >>= 1)
+                    if (k + h < groups && pre[k + h] <= e) k += h;
+                s_ent[threadIdx.x] = gload(list + (size_t)k * cap + (e - pre[k]));
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < 256 && e0 + j * G < count; ++j) {
+            const uint32_t ent = s_ent[j];
+            const int2 it = items[ent >> 4];
+            const int c = (int)(ent & 15u);
+            const DevJob &J = jobs[it.x];
+            const int L = J.L, m = J.m, n = 3 * L;
+            int buf, slot;
+            item_slot(J, MODE_OFFSPRING, it.y, buf, slot);
+            CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
+            uint64_t *planes = (uint64_t *)dyn;  // the CDS's planes as k_design left them
+            const int nwords = 2 * scan_words(n);
+            const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
+            __syncthreads();  // the previous entry's LDS is no longer read
+            for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
+            __syncthreads();
+            const uint64_t key = stem_child(n, (const uint2 *)planes, cs, bcast, redu, (uint16_t *)(planes + nwords));
+            if (threadIdx.x == 0 && key) {
+                atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
+                atomicMax(&J.stem_max[slot], (int)(key >> 40));
+            }
         }
     }
 }
@@ -1294,16 +1324,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 // Objective 5 of a design: the longest stem over its CDS, ties to the last
 // CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880).  Stores the
 // stems k_stem found and clears the marks k_design left.
-__device__ void stem_finish(const DevJob &J, int buf, int slot) {
+__device__ int stem_finish(const DevJob &J, int buf, int slot) {
     const int m = J.m;
     CdsStat *cs = J.cst[buf] + (size_t)slot * m;
-    int bl = 0, bp = 0, bq = 0, bc = m - 1;
+    int bl = 0, bp = 0, bq = 0, bc = m - 1, changed = 0;
     for (int c = 0; c < m; ++c) {
         CdsStat s = cs[c];
         if (s.pad) {
             stem_unkey(s.key, s.l, s.p, s.q);
             s.pad = 0;
             cs[c] = s;
+            ++changed;
         }
         if (s.l >= bl) {
             bl = s.l;
@@ -1319,6 +1350,7 @@ __device__ void stem_finish(const DevJob &J, int buf, int slot) {
     pq[1] = bq;
     pq[2] = bl;
     pq[3] = 0;
+    return changed;
 }
 
 // stem_finish for every work item's design (initialisation and evaluation;
@@ -1392,11 +1424,12 @@ __device__ void pool_extrema(const DevJob &J, int row0, int rows, const float *v
 // broadcasts), and the wave stores its words and adds their counts at once.
 template <int ROWS>
 __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items,
-                                                   bool finish, int32_t *stem_ctl) {
+                                                   bool finish, int32_t *stem_ctl, int ctl_n) {
     static_assert(ROWS % 4 == 0 && ROWS / 4 <= 64, "a wave's rows fit one word per lane");
     constexpr int WR = ROWS / 4;  // rows per wave
     __shared__ float rows[ROWS * NOBJ];
-    if (stem_ctl && blockIdx.x == 0 && threadIdx.x == 0) gstore(stem_ctl, 0);  // k_stem_list has read the count
+    if (stem_ctl && blockIdx.x == 0)  // k_stem_list has read the counts
+        for (int g = threadIdx.x; g < ctl_n; g += blockDim.x) gstore(stem_ctl + g, 0);
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
     const int P = J.P, N = J.N, word = it.z, buf = pool_buffer(J);
@@ -1417,9 +1450,15 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
 #pragma unroll
     for (int k = 0; k < NOBJ; ++k) cand[k] = i < P ? obj_at(i, k) : 0.f;
     // the workgroups of word 0 complete the stem records of their children
-    if (finish && word == 0 && threadIdx.x < ROWS) {
-        const int r = it.y + (int)threadIdx.x;
-        if (r >= N && r < P) stem_finish(J, buf, r);
+    // and count the CDS the children recomputed
+    if (finish && word == 0) {
+        int changed = 0;
+        if (threadIdx.x < ROWS) {
+            const int r = it.y + (int)threadIdx.x;
+            if (r >= N && r < P) changed = stem_finish(J, buf, r);
+        }
+        changed = wave_sum_i32(changed);
+        if ((threadIdx.x & 63) == 0 && changed) atomicAdd(&J.st->cds_scanned, (unsigned long long)changed);
     }
     __syncthreads();
     // and fold their rows into the pool's extrema (one workgroup per row block)
@@ -1813,7 +1852,6 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
             g.cr[r] = make_int2(0, 0);
             gstore(g.fill + r, 0);
             gstore(g.nearest + r, ~0ull);
-            gstore(g.lowest + r, ~0ull);
         }
     }
     // candidate rows of the extremes and their ASF values for every axis
@@ -1894,11 +1932,11 @@ __device__ __forceinline__ uint64_t niche_prio(const DevJob &J, uint32_t gen, in
 // wave per design, D designs a workgroup (item = (job, first design)); lanes
 // take reference lines r = lane, lane + 64, ... and the wave keeps the
 // smallest distance, ties to the lowest index (the first minimum of a sweep
-// in index order).  The niche arrays are folded in: designs of the fronts
-// before the last count toward their niche, those of the last front toward
-// its bucket, with the bucket's nearest and lowest designs as 64-bit minima;
-// the workgroup sums them in LDS first (when R fits) and adds each nonzero
-// entry to the job's arrays once.
+// in index order).  The niche arrays are folded in with atomics: designs of
+// the fronts before the last count toward their niche, those of the last
+// front toward its bucket, with the bucket's nearest design as a 64-bit
+// (distance, design) minimum.  A distance is never NaN or above FLT_MAX: the
+// sweep starts at FLT_MAX and takes only smaller values.
 template <int D>
 __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__ jobs,
                                                       const int2 *__restrict__ items) {
@@ -1906,64 +1944,39 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
     const DevJob &J = jobs[it.x];
     const SelState *st = J.st;
     if (st->n_cut) return;  // the whole job: no niching
-    constexpr int NT = 64 * D;
-    const int tid = threadIdx.x, lane = tid & 63, R = J.R, P = J.P, last = st->last;
-    const int d = it.y + (tid >> 6);
-    const NicheView g = niche_view(J.niche, R);
-    const bool agg = R <= NICHE_STAGE;
-    __shared__ int2 l_cr[NICHE_STAGE];
-    __shared__ unsigned long long l_near[NICHE_STAGE], l_low[NICHE_STAGE];
-    if (agg)
-        for (int r = tid; r < R; r += NT) {
-            l_cr[r] = make_int2(0, 0);
-            l_near[r] = ~0ull;
-            l_low[r] = ~0ull;
-        }
-    __syncthreads();
-    const int fr = d < P ? gload(J.front_of + d) : -1;
-    if (fr >= 0) {  // whole wave
-        const float *o = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
-        float f[NOBJ];
+    const int lane = threadIdx.x & 63, R = J.R;
+    const int d = it.y + (int)(threadIdx.x >> 6);
+    if (d >= J.P) return;  // whole wave
+    const int fr = gload(J.front_of + d);
+    if (fr < 0) return;  // whole wave
+    const float *o = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
+    float f[NOBJ];
 #pragma unroll
-        for (int k = 0; k < NOBJ; ++k) {
-            const float den = st->nadir[k] - st->ideal[k];
-            f[k] = (den == 0.f) ? 0.f : (o[k] - st->ideal[k]) / den;
-        }
-        float best = FLT_MAX;
-        int br = 0;
-        for (int r = lane; r < R; r += 64) {
-            const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
-            if (dd < best) {
-                best = dd;
-                br = r;
-            }
-        }
-#pragma unroll
-        for (int o2 = 32; o2 > 0; o2 >>= 1) argmin_pair(best, br, __shfl_xor(best, o2, 64), __shfl_xor(br, o2, 64));
-        if (lane == 0) {
-            J.ref_of[d] = br;
-            J.dist[d] = best;
-            int2 *cr = agg ? &l_cr[br] : g.cr + br;
-            if (fr < last) {
-                atomicAdd(&cr->x, 1);
-            } else {
-                atomicAdd(&cr->y, 1);
-                if (best <= FLT_MAX) atomicMin(agg ? &l_near[br] : g.nearest + br, asf_key(best, d));
-                atomicMin(agg ? &l_low[br] : g.lowest + br,
-                          ((unsigned long long)d << 32) | __float_as_uint(best));
-                J.prio[d] = (uint32_t)(niche_prio(J, st->gen, d) >> 32);
-            }
+    for (int k = 0; k < NOBJ; ++k) {
+        const float den = st->nadir[k] - st->ideal[k];
+        f[k] = (den == 0.f) ? 0.f : (o[k] - st->ideal[k]) / den;
+    }
+    float best = FLT_MAX;
+    int br = 0;
+    for (int r = lane; r < R; r += 64) {
+        const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
+        if (dd < best) {
+            best = dd;
+            br = r;
         }
     }
-    if (!agg) return;
-    __syncthreads();
-    for (int r = tid; r < R; r += NT) {
-        const int2 c = l_cr[r];
-        if (c.x) atomicAdd(&g.cr[r].x, c.x);
-        if (c.y) {
-            atomicAdd(&g.cr[r].y, c.y);
-            if (l_near[r] != ~0ull) atomicMin(g.nearest + r, l_near[r]);
-            atomicMin(g.lowest + r, l_low[r]);
+#pragma unroll
+    for (int o2 = 32; o2 > 0; o2 >>= 1) argmin_pair(best, br, __shfl_xor(best, o2, 64), __shfl_xor(br, o2, 64));
+    if (lane == 0) {
+        J.ref_of[d] = br;
+        J.dist[d] = best;
+        const NicheView g = niche_view(J.niche, R);
+        if (fr < st->last) {
+            atomicAdd(&g.cr[br].x, 1);
+        } else {
+            atomicAdd(&g.cr[br].y, 1);
+            atomicMin(g.nearest + br, asf_key(best, d));
+            J.prio[d] = (uint32_t)(niche_prio(J, st->gen, d) >> 32);
         }
     }
 }
@@ -1975,12 +1988,11 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
 // the smallest i with C(i) >= need (or P), found by bisection, where the
 // first eligible reference points in index order take one more.  Per
 // reference point: its picks, its level-0 pick (an empty niche takes its
-// nearest design, ties to the lowest index; a sweep in ascending index keeps
-// the lowest index when that design's distance is NaN or nothing compares
-// below FLT_MAX), and, for a bucket only part of whose rest is taken (a cut
-// bucket), its offset in the cut list.  Returns the cut list's size.
+// nearest design, ties to the lowest index), and, for a bucket only part of
+// whose rest is taken (a cut bucket), its offset in the cut list.  Returns
+// the cut list's size.
 __device__ __forceinline__ int niche_levels(const DevJob &J, const int2 *cr, const unsigned long long *nearest,
-                                            const unsigned long long *lowest, int *kp, int *lvl0, int *coff) {
+                                            int *kp, int *lvl0, int *coff) {
     const int P = J.P, N = J.N, R = J.R, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     __shared__ int s_ncut;
     if (wid == 0) {
@@ -2023,13 +2035,11 @@ __device__ __forceinline__ int niche_levels(const DevJob &J, const int2 *cr, con
             const uint64_t eb = __ballot(e);
             const int ex = carry + (int)__popcll(eb & ((1ull << lane) - 1ull));
             const int k = min(max(istar - c.x, 0), c.y) + ((e && ex < need_final) ? 1 : 0);
-            int l0 = -1;
-            if (r < R && k > 0 && c.x == 0) {
-                const unsigned long long nk = nearest[r], lk = lowest[r];
-                const int low = (int)(lk >> 32);
-                const int bi = nk == ~0ull ? 0x7FFFFFFF : (int)(uint32_t)nk;
-                l0 = (bi == 0x7FFFFFFF || isnan(__uint_as_float((uint32_t)lk))) ? low : bi;
-            }
+            // level 0 of an empty niche: its nearest design, ties to the
+            // lowest index (the reference's fallback to the lowest index, for a
+            // NaN distance or none below FLT_MAX, never applies: every distance
+            // is at most FLT_MAX and none is NaN)
+            const int l0 = (r < R && k > 0 && c.x == 0) ? (int)(uint32_t)nearest[r] : -1;
             const int h = l0 >= 0 ? 1 : 0;
             const int cut = (r < R && k - h > 0 && k - h < c.y - h) ? c.y : 0;
             const int incl = wave_scan_i32(cut);  // inclusive prefix of the cut sizes over the wave
@@ -2053,7 +2063,7 @@ __global__ __launch_bounds__(256) void k_levels(const DevJob *__restrict__ jobs)
     const DevJob &J = jobs[blockIdx.x];
     if (J.st->n_cut || J.R <= NICHE_STAGE) return;
     const NicheView g = niche_view(J.niche, J.R);
-    const int ncut = niche_levels(J, g.cr, g.nearest, g.lowest, g.kp, g.lvl0, g.coff);
+    const int ncut = niche_levels(J, g.cr, g.nearest, g.kp, g.lvl0, g.coff);
     if (threadIdx.x == 0) J.acc->ncut = ncut;
 }
 
@@ -2076,18 +2086,17 @@ __global__ __launch_bounds__(1024) void k_place(const DevJob *__restrict__ jobs)
     const NicheView g = niche_view(J.niche, R);
     const bool staged = R <= NICHE_STAGE;
     __shared__ int2 s_cr[NICHE_STAGE];
-    __shared__ unsigned long long s_near[NICHE_STAGE], s_low[NICHE_STAGE];
+    __shared__ unsigned long long s_near[NICHE_STAGE];
     __shared__ int s_kp[NICHE_STAGE], s_l0[NICHE_STAGE], s_co[NICHE_STAGE], l_cnt[NICHE_STAGE], l_base[NICHE_STAGE];
     const int *kp = g.kp, *lvl0 = g.lvl0, *coff = g.coff;
     if (staged) {
         for (int r = tid; r < R; r += NT) {
             s_cr[r] = g.cr[r];
             s_near[r] = gload(g.nearest + r);
-            s_low[r] = gload(g.lowest + r);
             l_cnt[r] = 0;
         }
         __syncthreads();
-        const int ncut = niche_levels(J, s_cr, s_near, s_low, s_kp, s_l0, s_co);
+        const int ncut = niche_levels(J, s_cr, s_near, s_kp, s_l0, s_co);
         if (blockIdx.y == 0) {
             for (int r = tid; r < R; r += NT) {
                 gstore(g.kp + r, s_kp[r]);
@@ -2210,14 +2219,15 @@ __global__ void k_zero_np(const DevJob *__restrict__ jobs) {
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
-                         uint32_t *stem_list, int32_t *stem_ctl, int stem_cap, int item_base) {
+                         uint32_t *stem_list, int32_t *stem_ctl, StemList sl, int item_base) {
     if (n <= 0) return hipSuccess;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0, 0); break;
-    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 0, 0); break;
+    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 1, 0, 0); break;
+    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 1, 0, 0); break;
     default:
-        if (!stem_list || !stem_ctl || stem_cap <= 0 || item_base < 0) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, stem_cap,
+        if (!stem_list || !stem_ctl || sl.cap <= 0 || sl.group <= 0 || sl.groups > STEM_MAX_GROUPS || item_base < 0)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.group, sl.cap,
                            item_base);
         break;
     }
@@ -2233,8 +2243,11 @@ hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, i
     return hipGetLastError();
 }
 hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_t *stem_list,
-                            const int32_t *stem_ctl, int capacity, int waves, size_t lds, hipStream_t s) {
-    if (capacity <= 0) return hipSuccess;
+                            const int32_t *stem_ctl, StemList sl, int waves, size_t lds, hipStream_t s) {
+    if (sl.cap <= 0 || sl.groups <= 0) return hipSuccess;
+    if (sl.groups > STEM_MAX_GROUPS) return hipErrorInvalidValue;
+    const int capacity = sl.groups * sl.cap;
+    if (lds < 4 * ((size_t)sl.groups + 1)) lds = 4 * ((size_t)sl.groups + 1);  // the prefix sums share the planes' LDS
     if (!stem_list || !stem_ctl || waves < 5 || waves > 8) return hipErrorInvalidValue;
     // workgroups take the entries round-robin, so every one must be resident
     // from the start: per CU, the waves per SIMD or what the LDS (160 KiB,
@@ -2243,10 +2256,10 @@ hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_
     const int per_cu = by_lds < waves ? (by_lds > 1 ? by_lds : 1) : waves;
     const int grid = capacity < STEM_CUS * per_cu ? capacity : STEM_CUS * per_cu;
     switch (waves) {
-    case 5: hipLaunchKernelGGL(k_stem_list<5>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
-    case 6: hipLaunchKernelGGL(k_stem_list<6>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
-    case 7: hipLaunchKernelGGL(k_stem_list<7>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
-    default: hipLaunchKernelGGL(k_stem_list<8>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, capacity); break;
+    case 5: hipLaunchKernelGGL(k_stem_list<5>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.groups, sl.cap); break;
+    case 6: hipLaunchKernelGGL(k_stem_list<6>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.groups, sl.cap); break;
+    case 7: hipLaunchKernelGGL(k_stem_list<7>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.groups, sl.cap); break;
+    default: hipLaunchKernelGGL(k_stem_list<8>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.groups, sl.cap); break;
     }
     return hipGetLastError();
 }
@@ -2255,13 +2268,13 @@ hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, i
     hipLaunchKernelGGL(k_stem_finish, dim3((n + 255) / 256), dim3(256), 0, s, jobs, items, n, mode);
     return hipGetLastError();
 }
-hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int rows,
+hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int ctl_n, int rows,
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
     switch (rows) {
-    case 64: hipLaunchKernelGGL(k_dominance<64>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl); break;
-    case 128: hipLaunchKernelGGL(k_dominance<128>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl); break;
-    case 256: hipLaunchKernelGGL(k_dominance<256>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl); break;
+    case 64: hipLaunchKernelGGL(k_dominance<64>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl, ctl_n); break;
+    case 128: hipLaunchKernelGGL(k_dominance<128>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl, ctl_n); break;
+    case 256: hipLaunchKernelGGL(k_dominance<256>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl, ctl_n); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

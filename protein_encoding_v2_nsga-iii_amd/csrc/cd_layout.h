@@ -86,14 +86,14 @@ __host__ __device__ inline uint32_t acc_clear_word(int i) {
 // Per-reference-point arrays of the niching (DevJob::niche, R entries each):
 // int2 (designs of the earlier fronts, designs of the last front); int picks,
 // level-0 pick, offset of a cut bucket in the cut list (-1 when not cut), its
-// fill count; u64 (distance bits << 32 | design) minimum and (design << 32 |
-// distance bits) minimum over the bucket (its nearest and its lowest design).
+// fill count; u64 (distance bits << 32 | design) minimum over the bucket
+// (its nearest design).
 struct NicheView {
     int2 *cr;
     int32_t *kp, *lvl0, *coff, *fill;
-    unsigned long long *nearest, *lowest;
+    unsigned long long *nearest;
 };
-constexpr int NICHE_INTS = 10;  // 32-bit words per reference point
+constexpr int NICHE_INTS = 8;  // 32-bit words per reference point
 __host__ __device__ inline NicheView niche_view(int32_t *base, int R) {
     NicheView v;
     v.cr = (int2 *)base;
@@ -102,7 +102,6 @@ __host__ __device__ inline NicheView niche_view(int32_t *base, int R) {
     v.coff = base + 4 * R;
     v.fill = base + 5 * R;
     v.nearest = (unsigned long long *)(base + 6 * R);
-    v.lowest = (unsigned long long *)(base + 8 * R);
     return v;
 }
 constexpr int NICHE_STAGE = 1024;  // reference points whose niche arrays fit in a workgroup's LDS

@@ -109,7 +109,7 @@ extern "C" int cd_nds_noalloc(const float *obj, int32_t P, int32_t N, int32_t *f
     int4 *tiles = (int4 *)(base + OFF_TILES);
     hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(1024), 0, s, h, dj, tiles, n_tiles, W);
     HIPCHK(hipGetLastError());
-    HIPCHK(cd::launch_dominance(dj, tiles, n_tiles, false, nullptr, cd::DOM_ROWS, s));
+    HIPCHK(cd::launch_dominance(dj, tiles, n_tiles, false, nullptr, 0, cd::DOM_ROWS, s));
     HIPCHK(cd::launch_fronts(dj, 1, P, cd::SHAPES_JOB.fronts_threads, s));
     hipLaunchKernelGGL(k_noalloc_finish, dim3(1), dim3(64), 0, s, (const cd::SelState *)h.st, n_fronts);
     HIPCHK(hipGetLastError());

@@ -98,8 +98,8 @@ struct cd_server {
     cd::DevJob *d_jobs = nullptr;
     int2 *d_off = nullptr, *d_asc = nullptr;
     int4 *d_dom = nullptr;
-    uint32_t *d_slist = nullptr;  // changed CDS of a tick
-    int32_t *d_sctl = nullptr;    // its count, zero between ticks
+    uint32_t *d_slist = nullptr;  // changed CDS of a tick (stem_list_entries)
+    int32_t *d_sctl = nullptr;    // their counts (STEM_MAX_GROUPS), zero between ticks
     size_t cap_jobs = 0, cap_off = 0, cap_dom = 0, cap_asc = 0, cap_slist = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -237,7 +237,7 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res) {
     if (e == hipSuccess) e = ensure(S->d_off, S->cap_off, wl.off.size());
     if (e == hipSuccess) e = ensure(S->d_dom, S->cap_dom, wl.dom.size());
     if (e == hipSuccess) e = ensure(S->d_asc, S->cap_asc, wl.asc.size());
-    if (e == hipSuccess) e = ensure(S->d_slist, S->cap_slist, (size_t)wl.n_cds);
+    if (e == hipSuccess) e = ensure(S->d_slist, S->cap_slist, cd::stem_list_entries((int)wl.off.size(), wl.mg));
     if (e == hipSuccess)
         e = hipMemcpy(S->d_jobs, wl.descs.data(), sizeof(cd::DevJob) * wl.descs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(S->d_off, wl.off.data(), sizeof(int2) * wl.off.size(), hipMemcpyHostToDevice);
@@ -398,8 +398,8 @@ int cd_server_create(int64_t hbm_budget_bytes, int32_t max_jobs, cd_server **out
     S->budget = hbm_budget_bytes;
     S->max_jobs = max_jobs;
     hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&S->d_sctl, 16);
-    if (e == hipSuccess) e = hipMemset(S->d_sctl, 0, 16);
+    if (e == hipSuccess) e = hipMalloc(&S->d_sctl, sizeof(int32_t) * cd::STEM_MAX_GROUPS);
+    if (e == hipSuccess) e = hipMemset(S->d_sctl, 0, sizeof(int32_t) * cd::STEM_MAX_GROUPS);
     if (e != hipSuccess) {
         if (S->stream) (void)hipStreamDestroy(S->stream);
         if (S->d_sctl) (void)hipFree(S->d_sctl);

@@ -86,3 +86,38 @@ def test_batch_rejects_uninitialised(cdp):
         cdp.Batch([j0, j1])
     with pytest.raises(cdp.CdError):
         cdp.Batch([j0, j0])
+
+
+def test_cds_scanned_counts(cdp):
+    """The count of CDS whose objectives were recomputed since initialisation:
+    per generation at most every CDS of the children (one to m per child,
+    fewer when an operator finds nothing to change); the same whether a job
+    advances alone or in a batch."""
+    print("cds scanned", flush=True)
+    specs = make_jobs(cdp, SPECS)
+    G = 5
+    alone = []
+    for s in specs:
+        m, N = s[2], s[3]
+        j = cdp.Job(*s).init()
+        j.sync()
+        n0 = j.cds_scanned()
+        assert n0 == 0
+        prev = n0
+        for _ in range(G):
+            j.step(1)
+            j.sync()
+            n = j.cds_scanned()
+            assert 0 <= n - prev <= N * m
+            prev = n
+        assert prev > n0  # the children did change CDS
+        alone.append(prev)
+        j.close()
+    jobs = [cdp.Job(*s).init() for s in specs]
+    b = cdp.Batch(jobs)
+    b.step(G)
+    b.sync()
+    assert [j.cds_scanned() for j in jobs] == alone
+    b.close()
+    for j in jobs:
+        j.close()
