@@ -361,35 +361,44 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
             cs[u] = dmask ? __builtin_ctz(dmask) : -1;
             dmask &= dmask - 1u;
         }
-        int64_t cai[4] = {0, 0, 0, 0}, cpb[4] = {0, 0, 0, 0};
-        int hsc[4] = {0, 0, 0, 0}, gcn[4] = {0, 0, 0, 0};
-        for (int k = tid; k < L; k += NT) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (cs[u] < 0) continue;
-                const uint8_t *g = cd6 + cs[u] * Ls;
-                const int a = g[k];
-                cai[u] += D.lw6[a];
-                gcn[u] += code_gc3(a);
-                if (k < L - 1) {
-                    const int b = g[k + 1];
-                    hsc[u] += code_hidden(D.stop6, a, b);
-                    if (k < L - 2) cpb[u] += gload(J.cpfx6 + a * 64 + b);
-                }
-            }
-        }
+        // one CDS at a time (its sums reduced over the wave into LDS before
+        // the next), four codons a thread per round: the pair scores of all
+        // four (global table, one dependent load each) are fetched together
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (cs[u] < 0) continue;
-            cai[u] = wave_sum_i64(cai[u]);
-            cpb[u] = wave_sum_i64(cpb[u]);
-            hsc[u] = wave_sum_i32(hsc[u]);
-            gcn[u] = wave_sum_i32(gcn[u]);
+            const uint8_t *g = cd6 + cs[u] * Ls;
+            int64_t cai = 0, cpb = 0;
+            int hsc = 0, gcn = 0;
+            for (int k0 = tid; k0 < L; k0 += 4 * NT) {
+                int64_t ps[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = k0 + v * NT;
+                    const bool pair = k < L - 2;
+                    const int kk = pair ? k : 0;  // an index inside the row either way
+                    ps[v] = pair ? gload(J.cpfx6 + g[kk] * 64 + g[kk + 1]) : 0;
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = k0 + v * NT;
+                    if (k >= L) break;
+                    const int a = g[k];
+                    cai += D.lw6[a];
+                    gcn += code_gc3(a);
+                    if (k < L - 1) hsc += code_hidden(D.stop6, a, g[k + 1]);
+                    cpb += ps[v];
+                }
+            }
+            cai = wave_sum_i64(cai);
+            cpb = wave_sum_i64(cpb);
+            hsc = wave_sum_i32(hsc);
+            gcn = wave_sum_i32(gcn);
             if (lane == 0) {
-                sh.red[wid * 8 + 2 * u] = cai[u];
-                sh.red[wid * 8 + 2 * u + 1] = cpb[u];
-                sh.red32[wid * 8 + 2 * u] = hsc[u];
-                sh.red32[wid * 8 + 2 * u + 1] = gcn[u];
+                sh.red[wid * 8 + 2 * u] = cai;
+                sh.red[wid * 8 + 2 * u + 1] = cpb;
+                sh.red32[wid * 8 + 2 * u] = hsc;
+                sh.red32[wid * 8 + 2 * u + 1] = gcn;
             }
         }
         lds_barrier();
@@ -417,47 +426,31 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
         }
         lds_barrier();
     }
-    // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order;
-    // the rows' padding codes are 0 in every CDS and add nothing
-    const int npair = m * (m - 1) / 2, nwd = Ls >> 2;
-    const uint32_t *w6 = (const uint32_t *)cd6;
-    int pi = 0, pj = 1;  // first pair of the pass
-    for (int p0 = 0; p0 < npair; p0 += 8) {
-        int ai[8], aj[8];
-        {
-            int i = pi, j = pj;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const bool ok = p0 + u < npair;
-                ai[u] = ok ? i * nwd : -1;
-                aj[u] = j * nwd;
-                if (ok && ++j == m) {
+    // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order,
+    // a wave each (pair p on wave p mod nw); the rows' padding codes are 0 in
+    // every CDS and add nothing
+    {
+        const int npair = m * (m - 1) / 2, nwd = Ls >> 2;
+        const uint32_t *w6 = (const uint32_t *)cd6;
+        int i = 0, j = 1;  // pair wid
+        for (int t = 0; t < wid && t < npair; ++t)
+            if (++j == m) {
+                ++i;
+                j = i + 1;
+            }
+        for (int p = wid; p < npair; p += nw) {
+            int acc = 0;
+            for (int q = lane; q < nwd; q += 64) acc += code_mismatch4(w6[i * nwd + q] ^ w6[j * nwd + q]);
+            acc = wave_sum_i32(acc);
+            if (lane == 0) sh.hd[p] = acc;
+            for (int t = 0; t < nw; ++t)  // the pair nw further on
+                if (++j == m) {
                     ++i;
                     j = i + 1;
                 }
-            }
-            pi = i;
-            pj = j;
         }
-        int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int q = tid; q < nwd; q += NT)
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (ai[u] >= 0) acc[u] += code_mismatch4(w6[ai[u] + q] ^ w6[aj[u] + q]);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (ai[u] < 0) continue;
-            acc[u] = wave_sum_i32(acc[u]);
-            if (lane == 0) sh.red32[wid * 8 + u] = acc[u];
-        }
-        lds_barrier();
-        if (tid < 8 && p0 + tid < npair) {
-            int v = 0;
-            for (int w = 0; w < nw; ++w) v += sh.red32[w * 8 + tid];
-            sh.hd[p0 + tid] = v;
-        }
-        lds_barrier();
     }
+    lds_barrier();
     if (tid < m) {  // per-CDS values, one lane each
         const CdsStat s = cst[tid];
         sh.ov[0][tid] = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
@@ -466,48 +459,52 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
         sh.ov[3][tid] = (float)s.gc3 / (float)L;
     }
     lds_barrier();
-    if (tid == 0) {
-        float cai_v = 0.f, cpb_v = 0.f, hsc_v = 0.f, gc_v = 0.f, hd_v = 0.f;
-        for (int i = 0; i < 2 * NOBJ; ++i) oidx[i] = 0;
-        for (int c = 0; c < m; ++c) {
-            const float cai = sh.ov[0][c];
-            if (c == 0 || -cai >= cai_v) {  // ties -> last CDS, utils.cuh:327
-                cai_v = -cai;
-                oidx[0] = c;
-            }
-            const float cpb = sh.ov[1][c];
-            if (c == 0 || -cpb >= cpb_v) {
-                cpb_v = -cpb;
-                oidx[2] = c;
-            }
-            const float hq = sh.ov[2][c];
-            if (c == 0 || -hq >= hsc_v) {
-                hsc_v = -hq;
-                oidx[4] = c;
-            }
-            const float frac = sh.ov[3][c];
-            const float dev = fabsf(frac - J.gc3_ref);
-            if (c == 0 || dev >= gc_v) {
-                gc_v = dev;
-                oidx[8] = c;
-                oidx[9] = (frac - J.gc3_ref) < 0 ? GC_UP : GC_DOWN;
-            }
+    // The objectives (calMinimumCAI .. calMaximumGC3, utils.cuh:271-626): each
+    // is the largest of the CDS's (pairs') values, ties to the last CDS (pair)
+    // -- a serial sweep that takes every value at least as large as the one it
+    // holds.  One lane per CDS (pair) and the largest (value bits, index) key
+    // over the wave give the same choice (the values are finite).
+    if (wid == 0) {
+        const float gc3_ref = J.gc3_ref;
+        const bool cl = lane < m;
+        const int c = cl ? lane : 0;
+        auto key = [](bool ok, float v, int i) -> uint64_t {
+            return ok ? ((uint64_t)ord_bits(v) << 32) | (uint32_t)i : 0ull;
+        };
+        const uint64_t k_cai = wave_max_u64(key(cl, -sh.ov[0][c], lane));
+        const uint64_t k_cpb = wave_max_u64(key(cl, -sh.ov[1][c], lane));
+        const uint64_t k_hsc = wave_max_u64(key(cl, -sh.ov[2][c], lane));
+        const uint64_t k_gc = wave_max_u64(key(cl, fabsf(sh.ov[3][c] - gc3_ref), lane));
+        const int npair = m * (m - 1) / 2;
+        uint64_t kh = 0;
+        for (int q = lane; q < npair; q += 64) {
+            const uint64_t k = key(true, -((float)sh.hd[q] / (float)n), q);
+            kh = k > kh ? k : kh;
         }
-        int q = 0;
-        for (int i = 0; i < m - 1; ++i)
-            for (int j = i + 1; j < m; ++j, ++q) {
-                const float dq = (float)sh.hd[q] / (float)n;
-                if ((i == 0 && j == 1) || -dq >= hd_v) {
-                    hd_v = -dq;
-                    oidx[6] = i;
-                    oidx[7] = j;
+        kh = wave_max_u64(kh);
+        if (lane < 2 * NOBJ) oidx[lane] = 0;
+        if (lane == 0) {
+            const int ca = (int)(uint32_t)k_cai, cb = (int)(uint32_t)k_cpb, ch = (int)(uint32_t)k_hsc;
+            const int cg = (int)(uint32_t)k_gc, qh = (int)(uint32_t)kh;
+            int i = 0, j = 1;  // pair qh
+            for (int t = 0; t < qh; ++t)
+                if (++j == m) {
+                    ++i;
+                    j = i + 1;
                 }
-            }
-        obj[0] = cai_v;
-        obj[1] = cpb_v;
-        obj[2] = hsc_v;
-        obj[3] = hd_v;
-        obj[4] = gc_v;
+            oidx[0] = ca;
+            oidx[2] = cb;
+            oidx[4] = ch;
+            oidx[6] = i;
+            oidx[7] = j;
+            oidx[8] = cg;
+            oidx[9] = (sh.ov[3][cg] - gc3_ref) < 0 ? GC_UP : GC_DOWN;
+            obj[0] = -sh.ov[0][ca];
+            obj[1] = -sh.ov[1][cb];
+            obj[2] = -sh.ov[2][ch];
+            obj[3] = -((float)sh.hd[qh] / (float)n);
+            obj[4] = fabsf(sh.ov[3][cg] - gc3_ref);
+        }
     }
     lds_barrier();
 }
