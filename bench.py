@@ -59,8 +59,8 @@ DEFAULT_BATCH_SHAPES = dict(design_split=24 * 1024, stem_waves=7, dom_rows=128, 
 # summarised in profiles/pmc_batched_mix_64jobs.md: HBM bytes (FETCH_SIZE x
 # its copy-kernel factor + WRITE_SIZE x its factor), VALU wave-instructions,
 # LDS instructions and LDS bank-conflict cycles.
-TICK_COUNTERS = {"hbm_bytes": 725377163, "valu_wave_instr": 440400063, "lds_instr": 65130170,
-                 "lds_conflict_cycles": 32092683}
+TICK_COUNTERS = {"hbm_bytes": 717912311, "valu_wave_instr": 312198705, "lds_instr": 33711866,
+                 "lds_conflict_cycles": 4688963}
 
 
 def log(msg):
@@ -212,6 +212,7 @@ def bench_single(cdp, residues, m, N, H, seed, G, label):
     t = cdp.Job(aa, tab, m, N, H, 0.1, seed).init()  # a copy to tune on; the shapes are cached
     shapes, _ = t.tune(2)
     t.close()
+    log(f"{label}: shapes tuned: {shapes}")
     j = cdp.Job(aa, tab, m, N, H, 0.1, seed).init()
     assert j.shapes() == shapes
     j.sync()
@@ -222,7 +223,7 @@ def bench_single(cdp, residues, m, N, H, seed, G, label):
     j.close()
     log(f"{label}: {G} generations in {ms:.1f} ms")
     return {"workload": f"single job: {residues} residues (L={residues + 1} with the stop), m={m}, N={N} "
-                        f"(pool {2 * N}), H={H}, Pm=0.1, seed {seed}; all {G} generations after initialisation",
+                        f"(pool {2 * N}), H={H}, Pm=0.1, seed {seed}; {G} generations after initialisation",
             "designs_s": round(N * G / (ms / 1e3), 1), "ms_per_generation": round(ms / G, 4),
             "latency_ms_per_generation": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
             "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in prof])), "shapes": shapes,
@@ -281,12 +282,18 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="timed batched ticks")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sections", default=",".join(SECTIONS),
-                    help="comma-separated parts to run: " + ",".join(SECTIONS) + " (the headline always runs)")
+                    help="comma-separated parts to run: " + ",".join(SECTIONS) + " (the headline runs unless --no-headline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--served", type=float, default=0.0, metavar="SECONDS",
                     help="instead of the batched line, run the served form: open-loop Poisson arrivals for "
                          "this many seconds and print its own JSON line")
     ap.add_argument("--rate", type=float, default=20.0, help="arrival rate of --served, jobs/s")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="skip the batched headline and print only the sections asked for (to profile one of them "
+                         "alone); the line then carries no value")
+    ap.add_argument("--single-gens", type=int, default=0,
+                    help="generations of the c1 and c2 sections (default: their full runs of 1000 and 2000); "
+                         "for shorter profiling runs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB three times on the GPU (a plain copy "
                          "kernel of known size, to calibrate hardware byte counters); not part of the measurement")
@@ -300,8 +307,26 @@ def main():
         print(json.dumps(bench_served(cdp, args)), flush=True)
         return
     sections = set(x.strip() for x in args.sections.split(",") if x.strip())
-    head = bench_headline(cdp, args)
-    out = {
+    if args.no_headline:
+        out = {"metric": METRIC, "value": None, "unit": "designs/s", "n_gpus": 1, "note": "headline skipped"}
+    else:
+        out = headline_line(bench_headline(cdp, args), args)
+    if "c3full" in sections:
+        out["c3_full"] = bench_c3_full(cdp, args)
+    if "c1" in sections:
+        out["c1"] = bench_single(cdp, 500, 4, 1024, 5, 2, args.single_gens or 1000, "c1")
+    if "c2" in sections:
+        out["c2"] = bench_single(cdp, 1500, 8, 4096, 5, 3, args.single_gens or 2000, "c2")
+    if "cpu" in sections and not args.no_cpu:
+        out.update(cpu_runs(cdp, args))
+    print(json.dumps(out), flush=True)
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
+
+
+def headline_line(head, args):
+    """The JSON line of the batched headline."""
+    return {
         "metric": METRIC,
         "value": head["value"],
         "unit": "designs/s",
@@ -322,17 +347,6 @@ def main():
         "roofline": head["roofline"],
         "counters": head["counters"],
     }
-    if "c3full" in sections:
-        out["c3_full"] = bench_c3_full(cdp, args)
-    if "c1" in sections:
-        out["c1"] = bench_single(cdp, 500, 4, 1024, 5, 2, 1000, "c1")
-    if "c2" in sections:
-        out["c2"] = bench_single(cdp, 1500, 8, 4096, 5, 3, 2000, "c2")
-    if "cpu" in sections and not args.no_cpu:
-        out.update(cpu_runs(cdp, args))
-    print(json.dumps(out), flush=True)
-    if args.copy_mib > 0:
-        copy_calibration(args.copy_mib)
 
 
 def bench_served(cdp, args):

@@ -136,7 +136,7 @@ struct DevJob {
     int32_t *front_of;     // [P]
     int32_t *np;           // [P] dominator counts
     int32_t *stem_max;     // [P] longest stem of the design (k_design: unchanged CDS, k_stem: atomic maximum)
-    uint64_t *dom;         // [P][W]: bit i of row j set when i dominates j
+    uint64_t *dom;         // [W][P], word-major: bit b of dom[w * P + j] set when design 64w + b dominates j
     int32_t *ref_of;       // [P]
     float *dist;           // [P]
     int32_t *niche;        // NICHE_INTS x R words per reference point (NicheView); null for a sort alone

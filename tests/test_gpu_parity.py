@@ -50,9 +50,10 @@ def test_nds_bit_exact(cdp, orc, P, N):
     assert nf_g == nf_o and np.array_equal(fo_g, fo_o)
 
 
-# k_niche keeps its per-reference-point arrays and the last front in LDS when
-# they fit (P <= 2048 here), in LDS without the copies of the pool arrays at
-# P = 8192 with 252 points, and in global memory at P = 8192 with 2002 points
+# Each k_place workgroup settles the niche levels from the niche arrays staged
+# in LDS when the reference points fit there (at most 1024: every case here
+# but H = 9, whose 2002 points take k_levels on the arrays in global memory);
+# at P = 8192 the last front and the cut list span many workgroups
 @pytest.mark.parametrize("N,H,seed", [(8, 1, 1), (50, 2, 2), (128, 4, 3), (1024, 5, 4), (300, 3, 5), (4096, 5, 6),
                                       (4096, 9, 7)])
 def test_select_bit_exact(cdp, orc, N, H, seed):
