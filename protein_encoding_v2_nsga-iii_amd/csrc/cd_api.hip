@@ -138,7 +138,7 @@ namespace {
 
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
-    size_t aa, code, lw6, cpfx6, cps6, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
+    size_t pos, code, lw6, cpfx6, cps6, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
         sctl,
         front,
         np, dom,
@@ -150,7 +150,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
     Sizes s;
     Layout o{};
     if (!sel_only) {
-        o.aa = s.add(L);
+        o.pos = s.add(2 * (size_t)cd::row_stride(L));
         o.code = s.add(64);
         o.lw6 = s.add(64 * 8);
         o.cpfx6 = s.add(64 * 64 * 8);
@@ -332,12 +332,21 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
     h.thr_2pm = bern_threshold_host(d->Pm * 2.0f);
     h.gc3_ref = d->tables.gc3_ref;
     if (!sel_only) {
-        up.add(o.aa, d->aa, J->L);
+        {  // per position the first codon of its amino acid and its synonym count
+            const int Ls = cd::row_stride(J->L);
+            std::vector<uint8_t> pos(2 * (size_t)Ls, 0);
+            for (int k = 0; k < Ls; ++k) {
+                const int a = k < J->L ? d->aa[k] : -1;
+                pos[k] = a >= 0 ? (uint8_t)SYN_START[a] : 0;
+                pos[Ls + k] = a >= 0 ? (uint8_t)SYN_NUM[a] : 1;
+            }
+            up.add(o.pos, pos.data(), pos.size());
+        }
         up.add(o.code, t.code, 64);
         up.add(o.lw6, t.lw6, 64);
         up.add(o.cpfx6, t.cpfx6, 64 * 64);
         up.add(o.cps6, t.cps6, 64 * 64);
-        h.aa = (const int8_t *)(base + o.aa);
+        h.pos = (const uint8_t *)(base + o.pos);
         h.code = (const uint8_t *)(base + o.code);
         h.lw6 = (const int64_t *)(base + o.lw6);
         h.cpfx6 = (const int64_t *)(base + o.cpfx6);

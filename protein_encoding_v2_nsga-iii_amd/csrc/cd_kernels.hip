@@ -1032,22 +1032,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     CdsStat *cst = (CdsStat *)(cd6 + m * Ls);
     const DesignLds D{(const int64_t *)(dyn + DL_LW6), dyn + DL_CODE, pbase, pnum, J.stop6, Ls};
 
-    // stage the tables and the synonym ranges (padding positions: one
-    // synonym at codon 0)
+    // The descriptor fields of the staging phase, read before its stores (a
+    // store through a pool pointer could alias the descriptor as far as the
+    // compiler knows, and would make every later field a reload).
+    const uint8_t *const pos = J.pos;
+    const int64_t *const lw6 = J.lw6;
+    const uint8_t *const ctab = J.code;
+    int8_t *const genome0 = J.genome[0], *const genome1 = J.genome[1];
+    float *const obj0 = J.obj[0], *const obj1 = J.obj[1];
+    int8_t *const oidx0 = J.oidx[0], *const oidx1 = J.oidx[1];
+    int32_t *const pql0 = J.pql[0], *const pql1 = J.pql[1];
+    CdsStat *const cst0 = J.cst[0], *const cst1 = J.cst[1];
+    int32_t *const np = J.np;
+    SelAcc *const acc = J.acc;
+    // stage the tables and the synonym ranges (rows pbase and pnum, as the
+    // job keeps them: padding positions have one synonym at codon 0)
     if (tid < 64)
-        ((int64_t *)(dyn + DL_LW6))[tid] = gload(J.lw6 + tid);
+        ((int64_t *)(dyn + DL_LW6))[tid] = gload(lw6 + tid);
     else if (tid < 128)
-        dyn[DL_CODE + tid - 64] = gload(J.code + tid - 64);
-    for (int k = tid; k < Ls; k += NT) {
-        const int a = k < L ? (int)gload(J.aa + k) : -1;
-        pbase[k] = a >= 0 ? (uint8_t)c_syn_start[a] : 0;
-        pnum[k] = a >= 0 ? (uint8_t)c_syn_num[a] : 1;
-    }
+        dyn[DL_CODE + tid - 64] = gload(ctab + tid - 64);
+    for (int i = tid; i < (2 * Ls) >> 4; i += NT) ((u32x4 *)pbase)[i] = gload((const u32x4 *)pos + i);
     if (tid < MAX_M) sh.dirty[tid] = (MODE != 2);
     uint32_t gen = 0;
     int dst = 0, out_slot = it.y, src = 0, parent = 0;
     if (MODE == 2) {
-        gen = J.st->gen;
+        gen = __builtin_amdgcn_readfirstlane(gload(&J.st->gen));
         src = (int)(gen & 1u);
         dst = src ^ 1;
         out_slot = N + it.y;
@@ -1055,86 +1064,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
     u32x4 *syn4 = (u32x4 *)syn;
     if (MODE == 0) {
-        // position k's synonym count was written above by this same thread
-        // (k = tid + j*NT in both loops), so no barrier is needed before it
+        // positions take their synonym counts from the job's rows (the LDS
+        // copy above is not yet visible to every thread)
         const int d = it.y;
         for (int c = 0; c < m; ++c)
             for (int k = tid; k < Ls; k += NT) {
-                const int ns = pnum[k];
                 int s = 0;
                 if (k >= L) {
                     s = 0;
                 } else if (d == 0) {
-                    s = ns - 1;
+                    s = (int)gload(pos + Ls + k) - 1;
                 } else {
                     uint32_t r[4];
                     philox(0u, (uint32_t)d, (uint32_t)(c * L + k), TAG_INIT, J.seed, r);
-                    s = (int)uniform_int(r[0], (uint32_t)ns);
+                    s = (int)uniform_int(r[0], (uint32_t)gload(pos + Ls + k));
                 }
                 syn[c * Ls + k] = (uint8_t)s;
             }
     } else if (MODE == 1) {
-        const u32x4 *sg = (const u32x4 *)(J.genome[0] + (size_t)it.y * m * Ls);
+        const u32x4 *sg = (const u32x4 *)(genome0 + (size_t)it.y * m * Ls);
         for (int i = tid; i < G16; i += NT) syn4[i] = gload(sg + i);
     } else {
         const int c = it.y;
-        const u32x4 *pg = (const u32x4 *)(J.genome[src] + (size_t)parent * m * Ls);
-        u32x4 *sg = (u32x4 *)(J.genome[dst] + (size_t)c * m * Ls);
-        for (int i = tid; i < G16; i += NT) {
-            const u32x4 v = gload(pg + i);
-            gstore(sg + i, v);
-            syn4[i] = v;
+        // the survivor copy (copySolution, utils.cuh:1106-1147): every load
+        // is issued before the stores, at clamped indices
+        const float *const obj_s = src ? obj1 : obj0;
+        float *const obj_d = src ? obj0 : obj1;
+        const int8_t *const oidx_s = src ? oidx1 : oidx0;
+        int8_t *const oidx_d = src ? oidx0 : oidx1;
+        const int32_t *const pql_s = src ? pql1 : pql0;
+        int32_t *const pql_d = src ? pql0 : pql1;
+        const CdsStat *const cst_s = src ? cst1 : cst0;
+        CdsStat *const cst_d = src ? cst0 : cst1;
+        const u32x4 *pg = (const u32x4 *)((src ? genome1 : genome0) + (size_t)parent * m * Ls);
+        u32x4 *sg = (u32x4 *)((dst ? genome1 : genome0) + (size_t)c * m * Ls);
+        // the records' loads first (lanes of wave 0), then the genome row's
+        float vo = 0.f;
+        int8_t vi = 0;
+        int32_t vp = 0;
+        CdsStat cs{};
+        if (tid < 16) {
+            vo = gload(obj_s + (size_t)parent * NOBJ + min(tid, NOBJ - 1));
+            vi = gload(oidx_s + (size_t)parent * 2 * NOBJ + min(tid, 2 * NOBJ - 1));
+            vp = gload(pql_s + (size_t)parent * 4 + min(tid, 3));
+            cs = gload_stat(cst_s + (size_t)parent * m + min(tid, m - 1));
         }
-        // survivor records (copySolution, utils.cuh:1106-1147)
-        if (tid < NOBJ) {
-            const float v = gload(J.obj[src] + (size_t)parent * NOBJ + tid);
-            gstore(J.obj[dst] + (size_t)c * NOBJ + tid, v);
-            sh.pobj[tid] = v;
+        for (int i0 = tid; i0 < G16; i0 += 4 * NT) {
+            u32x4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = gload(pg + min(i0 + u * NT, G16 - 1));
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * NT < G16) {
+                    gstore(sg + i0 + u * NT, v[u]);
+                    syn4[i0 + u * NT] = v[u];
+                }
         }
-        if (tid < 2 * NOBJ) {
-            const int8_t v = gload(J.oidx[src] + (size_t)parent * 2 * NOBJ + tid);
-            gstore(J.oidx[dst] + (size_t)c * 2 * NOBJ + tid, v);
-            sh.pidx[tid] = v;
+        if (tid < 16) {
+            if (tid < NOBJ) {
+                gstore(obj_d + (size_t)c * NOBJ + tid, vo);
+                sh.pobj[tid] = vo;
+            }
+            if (tid < 2 * NOBJ) {
+                gstore(oidx_d + (size_t)c * 2 * NOBJ + tid, vi);
+                sh.pidx[tid] = vi;
+            }
+            if (tid < 4) {
+                gstore(pql_d + (size_t)c * 4 + tid, vp);
+                sh.ppql[tid] = vp;
+            }
+            if (tid < m) {
+                cs.pad = 0;
+                gstore_stat(cst_d + (size_t)c * m + tid, cs);
+                cst[tid] = cs;
+            }
+        } else if (tid == 64) {  // the sort of this generation starts from cleared counts
+            gstore(np + c, 0);
+            gstore(np + N + c, 0);
         }
-        if (tid < 4) {
-            const int32_t v = gload(J.pql[src] + (size_t)parent * 4 + tid);
-            gstore(J.pql[dst] + (size_t)c * 4 + tid, v);
-            sh.ppql[tid] = v;
-        }
-        if (tid < m) {
-            CdsStat s = gload_stat(J.cst[src] + (size_t)parent * m + tid);
-            s.pad = 0;
-            gstore_stat(J.cst[dst] + (size_t)c * m + tid, s);
-            cst[tid] = s;
-        }
-        // the sort of this generation starts from cleared counts
-        if (tid == 0) {
-            gstore(J.np + c, 0);
-            gstore(J.np + N + c, 0);
-        }
-        if (c == 0 && tid < ACC_WORDS) gstore((uint32_t *)J.acc + tid, acc_clear_word(tid));
+        if (c == 0 && tid >= 128 && tid < 128 + ACC_WORDS) gstore((uint32_t *)acc + tid - 128, acc_clear_word(tid - 128));
     }
     lds_barrier();  // tables, synonym ranges, genome row and records
     // codon codes of the row, four per thread step; the padding positions
-    // are forced to synonym 0 and code 0
+    // are forced to synonym 0 and code 0 (the table reads stay below 64: a
+    // padding position reads codon 0)
     {
         const int nwd = Ls >> 2;
         const uint32_t *pb4 = (const uint32_t *)pbase;
         uint32_t *s4 = (uint32_t *)syn, *c4 = (uint32_t *)cd6;
         for (int c = 0; c < m; ++c)
             for (int q = tid; q < nwd; q += NT) {
-                const uint32_t g = s4[c * nwd + q] + pb4[q];  // bytewise: synonyms < 6, first codons < 64
-                uint32_t s = s4[c * nwd + q], code = 0;
+                const int left = L - 4 * q;  // positions of the word below L
+                const uint32_t keep = left >= 4 ? ~0u : (left <= 0 ? 0u : (1u << (8 * left)) - 1u);
+                const uint32_t s = s4[c * nwd + q] & keep;
+                const uint32_t g = s + pb4[q];  // bytewise: synonyms < 6, first codons < 64
+                uint32_t code = 0;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    if (4 * q + b < L) {
-                        code |= (uint32_t)D.code[(g >> (8 * b)) & 0xFFu] << (8 * b);
-                    } else {
-                        s &= ~(0xFFu << (8 * b));
-                    }
-                }
+                for (int b = 0; b < 4; ++b) code |= (uint32_t)D.code[(g >> (8 * b)) & 0xFFu] << (8 * b);
                 s4[c * nwd + q] = s;
-                c4[c * nwd + q] = code;
+                c4[c * nwd + q] = code & keep;
             }
     }
     lds_barrier();

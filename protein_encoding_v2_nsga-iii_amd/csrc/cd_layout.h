@@ -120,7 +120,8 @@ struct DevJob {
     uint64_t seed, thr_pm, thr_2pm;
     uint64_t stop6;        // bit c set when code c is a stop codon (the table's last three codons)
     float gc3_ref, pad1;
-    const int8_t *aa;      // [L]
+    const uint8_t *pos;    // [2][Ls]: per position the first codon of its amino acid, then its synonym count
+                           // (padding positions: codon 0, one synonym)
     const uint8_t *code;   // [64] code of each codon of the table
     const int64_t *lw6;    // [64] 2^40-scaled log weights by code
     const int64_t *cpfx6;  // [64*64] 2^40-scaled pair scores by codes (0 where either is a stop)

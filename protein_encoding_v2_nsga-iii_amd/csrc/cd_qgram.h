@@ -106,10 +106,12 @@ __host__ __device__ inline void qg_measure(const uint2 *sp, const uint2 *rp, int
 // chunks ch0, ch0 + stride, ...).  Each step reads one chain entry (or moves
 // to the lane's next chunk) and yields the hit it finds when its q-gram is
 // valid, so that a wave's steps stay cheap and uniform and the hits can be
-// measured apart from the walk.
+// measured apart from the walk.  A chunk's eight heads are read together when
+// the chunk is loaded and kept in registers, so a step reads at most one link.
 struct QgLane {
     uint32_t lo, hi;  // plane bits of rc from the chunk's first position
     uint32_t mask;    // positions of the chunk whose chains are still to start
+    uint32_t h0, h1, h2, h3;  // heads of the chunk's positions, 2j and 2j+1 in the halves of hj
     uint32_t e;       // current chain entry, QG_NIL between chains
     int j;            // position of the current chain within the chunk
     int ch;           // current chunk; the walk is over once it reaches nch
@@ -121,16 +123,24 @@ __host__ __device__ inline uint32_t qg_lane_code(const QgLane &s, int j) {
     return ((s.lo >> j) & M) | (((s.hi >> j) & M) << QG_Q);
 }
 
-// Plane bits of chunk ch (one pair of plane words) and its positions with a
-// chain (the eight heads are read together).
+// Plane bits of chunk ch (one pair of plane words), the heads of its eight
+// positions (read unconditionally, so that the reads issue together: a code
+// is always below QG_CODES) and its positions with a chain.
 __host__ __device__ inline void qg_lane_load(QgLane &s, const uint2 *rp, const uint16_t *head, int xe) {
     const int x0 = s.ch << 3, w = x0 >> 5;
     const uint2 p0 = rp[w], p1 = rp[w + 1];
     s.lo = scan_funnel(p1.x, p0.x, x0 & 31);
     s.hi = scan_funnel(p1.y, p0.y, x0 & 31);
+    uint32_t hv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) hv[j] = head[qg_lane_code(s, j)];
     uint32_t mask = 0u;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) mask |= (uint32_t)(x0 + j < xe && head[qg_lane_code(s, j)] != QG_NIL) << j;
+    for (int j = 0; j < 8; ++j) mask |= (uint32_t)(x0 + j < xe && hv[j] != QG_NIL) << j;
+    s.h0 = hv[0] | (hv[1] << 16);
+    s.h1 = hv[2] | (hv[3] << 16);
+    s.h2 = hv[4] | (hv[5] << 16);
+    s.h3 = hv[6] | (hv[7] << 16);
     s.mask = mask;
 }
 
@@ -141,6 +151,7 @@ __host__ __device__ inline void qg_lane_start(QgLane &s, const uint2 *rp, const 
     s.j = 0;
     s.mask = 0u;
     s.lo = s.hi = 0u;
+    s.h0 = s.h1 = s.h2 = s.h3 = 0xFFFFFFFFu;
     if (ch0 < nch) qg_lane_load(s, rp, head, xe);
 }
 
@@ -157,9 +168,13 @@ __host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const u
             if (s.ch < nch) qg_lane_load(s, rp, head, n - QG_Q + 1);
             return false;
         }
-        s.j = __builtin_ctz(s.mask);
+        const int j = __builtin_ctz(s.mask);
+        s.j = j;
         s.mask &= s.mask - 1u;
-        s.e = head[qg_lane_code(s, s.j)];  // not QG_NIL: the mask has only positions with a chain
+        // the head of position j (not QG_NIL: the mask has only positions with a chain)
+        const uint32_t lo2 = (j & 2) ? s.h1 : s.h0, hi2 = (j & 2) ? s.h3 : s.h2;
+        const uint32_t hw = (j & 4) ? hi2 : lo2;
+        s.e = (hw >> (16 * (j & 1))) & 0xFFFFu;
     }
     --budget;
     const uint32_t e = s.e;
