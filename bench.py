@@ -51,8 +51,8 @@ HBM_BYTES_S = 8.0e12            # MI355X HBM3E peak
 VALU_WAVE_INSTR_S = 256 * 2.4e9  # 256 CUs x 4 SIMDs x one wave64 VALU instruction per 4 cycles, 2.4 GHz
 KERNELS = ("offspring", "stem", "dominance", "fronts", "associate", "niche")
 SECTIONS = ("headline", "c3full", "c1", "c2", "cpu")
-DEFAULT_BATCH_SHAPES = dict(design_split=24 * 1024, stem_waves=7, dom_rows=128, fronts_threads=1024,
-                            assoc_designs=4, niche_threads=1024)
+DEFAULT_BATCH_SHAPES = dict(design_split=24 * 1024, design_threads=512, stem_waves=7, dom_rows=128,
+                            fronts_threads=1024, assoc_designs=4, niche_threads=1024)
 
 # Hardware counters of the offspring phase of one batched tick at the default
 # workload (both k_design<2> launch classes), from the rocprofv3 --pmc runs
@@ -282,7 +282,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="timed batched ticks")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sections", default=",".join(SECTIONS),
-                    help="comma-separated parts to run: " + ",".join(SECTIONS) + " (the headline runs unless --no-headline)")
+                    help="comma-separated parts to run: " + ",".join(SECTIONS)
+                    + " (the headline runs unless --no-headline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--served", type=float, default=0.0, metavar="SECONDS",
                     help="instead of the batched line, run the served form: open-loop Poisson arrivals for "

@@ -50,7 +50,8 @@ class _Tables(C.Structure):
 
 
 # launch shapes (cd_shapes): every shape gives the same results
-SHAPE_FIELDS = ("design_split", "stem_waves", "dom_rows", "fronts_threads", "assoc_designs", "niche_threads")
+SHAPE_FIELDS = ("design_split", "design_threads", "stem_waves", "dom_rows", "fronts_threads", "assoc_designs",
+                "niche_threads")
 
 
 class _Shapes(C.Structure):

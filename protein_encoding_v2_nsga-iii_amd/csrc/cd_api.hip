@@ -457,7 +457,7 @@ hipError_t enqueue_generation(const Launch &w, hipStream_t s, hipEvent_t *ev) {
             e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off, ns, w.lds_small, s, w.stem_list, w.stem_ctl, sl, 0);
         if (e == hipSuccess)
             e = cd::launch_design(cd::MODE_OFFSPRING, w.jobs, w.off + ns, w.n_off - ns, w.lds, s, w.stem_list,
-                                  w.stem_ctl, sl, ns);
+                                  w.stem_ctl, sl, ns, w.sh.design_threads);
     }
     if (e == hipSuccess) e = mark(1);
     if (e == hipSuccess)
@@ -593,6 +593,7 @@ int tune_shapes(const cd::Shapes &start, int njobs, int gens, hipStream_t s, App
     };
     const std::vector<Param> params = {
         {&cd::Shapes::design_split, {0, 20 * 1024, 24 * 1024, 29 * 1024}, 0x3u},
+        {&cd::Shapes::design_threads, {256, 512}, 0x1u},
         {&cd::Shapes::stem_waves, {5, 6, 7, 8}, 0x2u},
         {&cd::Shapes::dom_rows, {64, 128, 256}, 0x4u},
         {&cd::Shapes::fronts_threads, {256, 512, 1024}, 0x8u},

@@ -21,6 +21,7 @@ constexpr int STEM_CUS = 256;              // compute units of an MI355X: k_stem
 // lanes -- so the autotuner may pick any of them while a job runs.
 struct Shapes {
     int design_split;    // k_design LDS bytes up to which children launch apart from the rest (0: one launch)
+    int design_threads;  // k_design workgroup of the other children (a lone job: all): 256 or 512 threads
     int stem_waves;      // k_stem_list waves per SIMD (workgroups per CU): 5, 6, 7 or 8
     int dom_rows;        // rows per k_dominance workgroup: 64, 128 or 256 (a quarter per wave)
     int fronts_threads;  // k_fronts workgroup: 256, 512 or 1024 threads
@@ -28,8 +29,8 @@ struct Shapes {
     int niche_threads;   // k_niche workgroup: 256, 512 or 1024 threads
 };
 // Untuned shapes: a lone job, and a batch or served tick.
-constexpr Shapes SHAPES_JOB = {0, 6, 64, 512, 4, 1024};
-constexpr Shapes SHAPES_BATCH = {24 * 1024, 7, 128, 1024, 4, 1024};
+constexpr Shapes SHAPES_JOB = {0, 512, 6, 64, 512, 4, 1024};
+constexpr Shapes SHAPES_BATCH = {24 * 1024, 512, 7, 128, 1024, 4, 1024};
 bool shapes_valid(const Shapes &s);
 
 size_t design_lds_bytes(int L, int m);
@@ -68,10 +69,11 @@ inline size_t stem_list_entries(int n_items, int mg) {
 // in the sublists (entry ((item_base + item) << 4) | c; sl =
 // stem_list_geom of the whole children list) and counts them in stem_ctl;
 // the three are required there.  item_base: the position of items[0] in the
-// children list k_stem_list reads.
+// children list k_stem_list reads.  threads: the offspring mode's workgroup,
+// 256 or 512 (initialisation and evaluation take 256).
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
                          uint32_t *stem_list = nullptr, int32_t *stem_ctl = nullptr, StemList sl = StemList{0, 0, 0},
-                         int item_base = 0);
+                         int item_base = 0, int threads = 256);
 // initialisation and evaluation: parts workgroups per (item, CDS c < mg),
 // parts the largest stem_parts_of of the jobs: stem of each CDS k_design marked
 hipError_t launch_stem(int mode, const DevJob *jobs, const int2 *items, int n, int mg, int parts, size_t lds,

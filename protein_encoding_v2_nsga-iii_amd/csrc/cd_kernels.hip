@@ -73,19 +73,23 @@ size_t cds_plane_bytes(int L) { return (size_t)2 * scan_words(3 * L) * sizeof(ui
 // index and hit queue
 size_t stem_lds_bytes(int L) { return cds_plane_bytes(L) + qg_lds_bytes(3 * L); }
 
+constexpr int HITQ = 192;  // a wave's queue of hits: a full batch of 64 and one step's 128
+// k_design's static LDS for a workgroup of NW waves.
+template <int NW>
 struct DesignShared {
     float pobj[NOBJ];
     int32_t pidx[2 * NOBJ];
     int32_t ppql[4];
     int32_t dirty[MAX_M];
     int32_t hd[MAX_M * (MAX_M - 1) / 2];
-    int64_t red[64];
-    int32_t red32[64];
+    int64_t red[8 * NW];
+    int32_t red32[8 * NW];
     float ov[4][MAX_M];  // per CDS: CAI, CPB, HSC quotient, GC3 fraction
     uint64_t redu[16];
     int32_t misc[16];
-    int32_t slw[4 * 11];  // mut_sl: per wave, first [4], last [4], full [3]
+    int32_t slw[NW * 11];  // mut_sl: per wave, first [4], last [4], full [3]
     int32_t dlist[MAX_M];  // the changed CDS, in order
+    uint16_t hitq[NW][HITQ];  // per wave: codons that passed an operator's test, awaiting their work
 };
 
 // CdsStat records in global memory, field by field through the global
@@ -348,8 +352,9 @@ __host__ __device__ inline int stem_parts(int n) {
 // CDS pairs per pass; the sums are wave sums (DPP) added over the waves by
 // one thread per value.
 // ---------------------------------------------------------------------------
+template <class SH>
 __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6, CdsStat *cst, bool have_parent,
-                         DesignShared &sh, float *obj, int32_t *oidx) {
+                         SH &sh, float *obj, int32_t *oidx) {
     const int L = J.L, m = J.m, n = 3 * L, Ls = D.Ls;
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     uint32_t dmask = 0;  // the same in every thread
@@ -541,6 +546,37 @@ __device__ __forceinline__ void pair_draws(const MutArgs &A, int c, int q, uint3
     philox(A.gen, A.child, (uint32_t)(c * A.J->L + q), TAG_MUT, A.J->seed, r);
 }
 
+// The codons of an operator whose work per accepted codon is long, in
+// batches: wave w tests the draw pairs q = 64w + lane, 64w + 256 + lane, ...
+// (test(q) returns bit h set when codon h of pair q passes the operator's
+// Bernoulli test), queues the codons that pass (slot 2q + h) in its segment of
+// sh.hitq, and runs work(slot) one codon a lane whenever 64 are queued, then on
+// the rest.  Accepted codons change only themselves and the tests and the work
+// of other codons read only codons no test of this operator accepts, so the
+// order of the work does not matter.  Wave-local: the waves never wait for
+// each other.
+template <class Test, class Work>
+__device__ __forceinline__ void hits_by_wave(int npairs, uint16_t *seg, Test test, Work work) {
+    const int lane = threadIdx.x & 63;
+    int cnt = 0;  // the same in every lane of the wave
+    for (int q0 = (int)(threadIdx.x & ~63u); q0 < npairs; q0 += blockDim.x) {
+        const int q = q0 + lane;
+        const uint32_t hm = q < npairs ? test(q) : 0u;
+        const uint64_t b0 = __ballot(hm & 1u), b1 = __ballot(hm & 2u);
+        const int r0 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b0, 0u));
+        const int r1 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b1, 0u));
+        const int n0 = (int)__popcll(b0);
+        if (hm & 1u) seg[cnt + r0] = (uint16_t)(2 * q);
+        if (hm & 2u) seg[cnt + n0 + r1] = (uint16_t)(2 * q + 1);
+        cnt += n0 + (int)__popcll(b1);
+        while (cnt >= 64) {  // at most 63 + 128 queued: within HITQ
+            work((int)seg[cnt - 64 + lane]);
+            cnt -= 64;
+        }
+    }
+    if (lane < cnt) work((int)seg[lane]);
+}
+
 // mutationRandom, mutation.cuh:16-63: every codon of every CDS; a thread
 // takes codons 2q and 2q+1 of CDS c for its (c, q) pairs
 __device__ void mut_random(const MutArgs &A, int32_t *dirty) {
@@ -595,87 +631,106 @@ __device__ void mut_cai(const MutArgs &A, int cds, int32_t *dirty) {
 // mutationCBP with SELECT_UPPER_RANDOM, mutation.cuh:144-262: the codons
 // k = 2i + parity below L-1; a thread takes i = 2q, 2q+1 (k = 4q + parity,
 // 4q + parity + 2)
-__device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty) {
+__device__ void mut_cbp(const MutArgs &A, int cds, int32_t *dirty, uint16_t *seg) {
     const DevJob &J = *A.J;
     const int L = J.L, Ls = A.D->Ls, parity = (int)(A.gen & 1u);
+    const uint64_t thr = J.thr_2pm;
+    const float *const cps6 = J.cps6;
     const uint8_t *row = A.cd6 + cds * Ls;
-    for (int q = threadIdx.x; 4 * q + parity < L - 1; q += blockDim.x) {
+    const int npairs = L - 1 - parity > 0 ? (L - 1 - parity + 3) >> 2 : 0;  // 4q + parity < L - 1
+    auto test = [&](int q) -> uint32_t {
         uint32_t r[4];
         pair_draws(A, cds, q, r);
+        uint32_t hm = 0;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = 4 * q + parity + 2 * h;
-            if (k >= L - 1) break;
-            const int ns = A.D->pnum[k];
-            if (!bern(r[2 * h], J.thr_2pm) || ns == 1) continue;
-            const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
-            const int left = (k > 0) ? (int)row[k - 1] : 0;
-            const int right = row[k + 1];
-            // every synonym's score, the pair-score loads issued together (a
-            // missing neighbour adds nothing; stop codes score 0 in the table)
-            const bool hl = k > 0, hr = !((A.D->stop6 >> right) & 1u);
-            const float *rowl = J.cps6 + left * 64, *colr = J.cps6 + right;
-            float sc[MAX_SYN];
-#pragma unroll
-            for (int j = 0; j < MAX_SYN; ++j) {
-                const int g = A.D->code[base + (j < ns ? j : 0)];
-                const float a = gload(rowl + g), b = gload(colr + g * 64);
-                float v = 0.f;
-                v += hl ? a : 0.f;
-                v += hr ? b : 0.f;
-                sc[j] = v;
-            }
-            float curv = sc[0];
-#pragma unroll
-            for (int j = 1; j < MAX_SYN; ++j) curv = j == cur ? sc[j] : curv;
-            uint32_t cand = 0;  // synonyms scoring above the current codon, in order
-#pragma unroll
-            for (int j = 0; j < MAX_SYN; ++j)
-                if (j < ns && j != cur && sc[j] > curv) cand |= 1u << j;
-            if (cand) {
-                set_syn(A, cds, k, nth_bit(cand, uniform_int(r[2 * h + 1], (uint32_t)__popc(cand))));
-                dirty[cds] = 1;
-            }
+            if (k < L - 1 && bern(r[2 * h], thr) && A.D->pnum[k] > 1) hm |= 1u << h;
         }
-    }
+        return hm;
+    };
+    auto work = [&](int slot) {
+        const int q = slot >> 1, h = slot & 1, k = 4 * q + parity + 2 * h;
+        uint32_t r[4];
+        pair_draws(A, cds, q, r);
+        const int ns = A.D->pnum[k];
+        const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
+        const int left = (k > 0) ? (int)row[k - 1] : 0;
+        const int right = row[k + 1];
+        // every synonym's score, the pair-score loads issued together (a
+        // missing neighbour adds nothing; stop codes score 0 in the table)
+        const bool hl = k > 0, hr = !((A.D->stop6 >> right) & 1u);
+        const float *rowl = cps6 + left * 64, *colr = cps6 + right;
+        float sc[MAX_SYN];
+#pragma unroll
+        for (int j = 0; j < MAX_SYN; ++j) {
+            const int g = A.D->code[base + (j < ns ? j : 0)];
+            const float a = gload(rowl + g), b = gload(colr + g * 64);
+            float v = 0.f;
+            v += hl ? a : 0.f;
+            v += hr ? b : 0.f;
+            sc[j] = v;
+        }
+        float curv = sc[0];
+#pragma unroll
+        for (int j = 1; j < MAX_SYN; ++j) curv = j == cur ? sc[j] : curv;
+        uint32_t cand = 0;  // synonyms scoring above the current codon, in order
+#pragma unroll
+        for (int j = 0; j < MAX_SYN; ++j)
+            if (j < ns && j != cur && sc[j] > curv) cand |= 1u << j;
+        if (cand) {
+            set_syn(A, cds, k, nth_bit(cand, uniform_int(r[2 * h + 1], (uint32_t)__popc(cand))));
+            dirty[cds] = 1;
+        }
+    };
+    hits_by_wave(npairs, seg, test, work);
 }
 
 // mutationHSC with SELECT_UPPER_RANDOM, mutation.cuh:264-388: the codons
 // k = 2i + parity below L, two per thread as in mut_cbp
-__device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty) {
+__device__ void mut_hsc(const MutArgs &A, int cds, int32_t *dirty, uint16_t *seg) {
     const DevJob &J = *A.J;
     const int L = J.L, Ls = A.D->Ls, parity = (int)(A.gen & 1u);
+    const uint64_t thr = J.thr_2pm;
     const uint8_t *row = A.cd6 + cds * Ls;
     const uint64_t stop6 = A.D->stop6;
-    for (int q = threadIdx.x; 4 * q + parity < L; q += blockDim.x) {
+    const int npairs = L - parity > 0 ? (L - parity + 3) >> 2 : 0;  // 4q + parity < L
+    auto test = [&](int q) -> uint32_t {
         uint32_t r[4];
         pair_draws(A, cds, q, r);
+        uint32_t hm = 0;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = 4 * q + parity + 2 * h;
-            if (k >= L) break;
-            const int ns = A.D->pnum[k];
-            if (!bern(r[2 * h], J.thr_2pm) || ns == 1) continue;
-            const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
-            const int left = (k > 0) ? (int)row[k - 1] : -1;
-            const int right = (k < L - 1) ? (int)row[k + 1] : -1;
-            auto count = [&](int j) -> int {
-                const int g = A.D->code[base + j];
-                int v = 0;
-                if (left >= 0) v += code_hidden(stop6, left, g);
-                if (right >= 0) v += code_hidden(stop6, g, right);
-                return v;
-            };
-            const int curv = count(cur);
-            uint32_t cand = 0;
-            for (int j = 0; j < ns; ++j)
-                if (j != cur && count(j) > curv) cand |= 1u << j;
-            if (cand) {
-                set_syn(A, cds, k, nth_bit(cand, uniform_int(r[2 * h + 1], (uint32_t)__popc(cand))));
-                dirty[cds] = 1;
-            }
+            if (k < L && bern(r[2 * h], thr) && A.D->pnum[k] > 1) hm |= 1u << h;
         }
-    }
+        return hm;
+    };
+    auto work = [&](int slot) {
+        const int q = slot >> 1, h = slot & 1, k = 4 * q + parity + 2 * h;
+        uint32_t r[4];
+        pair_draws(A, cds, q, r);
+        const int ns = A.D->pnum[k];
+        const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
+        const int left = (k > 0) ? (int)row[k - 1] : -1;
+        const int right = (k < L - 1) ? (int)row[k + 1] : -1;
+        auto count = [&](int j) -> int {
+            const int g = A.D->code[base + j];
+            int v = 0;
+            if (left >= 0) v += code_hidden(stop6, left, g);
+            if (right >= 0) v += code_hidden(stop6, g, right);
+            return v;
+        };
+        const int curv = count(cur);
+        uint32_t cand = 0;
+        for (int j = 0; j < ns; ++j)
+            if (j != cur && count(j) > curv) cand |= 1u << j;
+        if (cand) {
+            set_syn(A, cds, k, nth_bit(cand, uniform_int(r[2 * h + 1], (uint32_t)__popc(cand))));
+            dirty[cds] = 1;
+        }
+    };
+    hits_by_wave(npairs, seg, test, work);
 }
 
 // indexArrayShuffling, utils.cuh:203-218: Fisher-Yates over the n <= 6
@@ -701,90 +756,103 @@ __device__ __forceinline__ uint32_t shuffle(uint32_t x, int n) {
 __device__ __forceinline__ int perm_at(uint32_t perm, int j) { return (int)((perm >> (4 * j)) & 0xFu); }
 
 // mutationHD, mutation.cuh:390-534: codons of cds1, scored against all CDS
-__device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty) {
+__device__ void mut_hd(const MutArgs &A, int cds1, int cds2, int32_t *dirty, uint16_t *seg) {
     const DevJob &J = *A.J;
     const int L = J.L, m = J.m, Ls = A.D->Ls;
-    for (int q = threadIdx.x; 2 * q < L; q += blockDim.x) {
+    const uint64_t thr = J.thr_pm;
+    auto test = [&](int q) -> uint32_t {
         uint32_t r[4];
         pair_draws(A, cds1, q, r);
+        uint32_t hm = 0;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = 2 * q + h;
-            if (k >= L) break;
-            const int ns = A.D->pnum[k];
-            if (!bern(r[2 * h], J.thr_pm) || ns == 1) continue;
-            const uint32_t order = shuffle(r[2 * h + 1], ns);
-            const int base = A.D->pbase[k], cur = A.syn[cds1 * Ls + k];
-            const int g2 = A.cd6[cds2 * Ls + k];
-            // every candidate's codes (the synonyms in shuffled order, slot
-            // MAX_SYN the current codon) first, then the smallest distance
-            // to the other CDS's codons at k (only cds1's codons change here,
-            // each by its own thread) and the distance to cds2's codon
-            int gc[MAX_SYN + 1], mn[MAX_SYN + 1], pr[MAX_SYN + 1];
+            if (k < L && bern(r[2 * h], thr) && A.D->pnum[k] > 1) hm |= 1u << h;
+        }
+        return hm;
+    };
+    auto work = [&](int k) {
+        uint32_t r[4];
+        pair_draws(A, cds1, k >> 1, r);
+        const int ns = A.D->pnum[k];
+        const uint32_t order = shuffle(r[2 * (k & 1) + 1], ns);
+        const int base = A.D->pbase[k], cur = A.syn[cds1 * Ls + k];
+        const int g2 = A.cd6[cds2 * Ls + k];
+        // every candidate's codes (the synonyms in shuffled order, slot
+        // MAX_SYN the current codon) first, then the smallest distance
+        // to the other CDS's codons at k (only cds1's codons change here,
+        // each by its own lane) and the distance to cds2's codon
+        int gc[MAX_SYN + 1], mn[MAX_SYN + 1], pr[MAX_SYN + 1];
 #pragma unroll
-            for (int j = 0; j <= MAX_SYN; ++j) {
-                gc[j] = A.D->code[base + (j == MAX_SYN ? cur : perm_at(order, j < ns ? j : ns - 1))];
-                mn[j] = 127;
-                pr[j] = code_mismatch(gc[j], g2);
-            }
-            for (int o = 0; o < m; ++o) {
-                if (o == cds1) continue;
-                const int oc = A.cd6[o * Ls + k];
+        for (int j = 0; j <= MAX_SYN; ++j) {
+            gc[j] = A.D->code[base + (j == MAX_SYN ? cur : perm_at(order, j < ns ? j : ns - 1))];
+            mn[j] = 127;
+            pr[j] = code_mismatch(gc[j], g2);
+        }
+        for (int o = 0; o < m; ++o) {
+            if (o == cds1) continue;
+            const int oc = A.cd6[o * Ls + k];
 #pragma unroll
-                for (int j = 0; j <= MAX_SYN; ++j) mn[j] = min(mn[j], code_mismatch(gc[j], oc));
-            }
-            const int cur_pair = pr[MAX_SYN], cur_mhd = mn[MAX_SYN];
-            int best_pair = -1, best_mhd = -1, best = -1;
+            for (int j = 0; j <= MAX_SYN; ++j) mn[j] = min(mn[j], code_mismatch(gc[j], oc));
+        }
+        const int cur_pair = pr[MAX_SYN], cur_mhd = mn[MAX_SYN];
+        int best_pair = -1, best_mhd = -1, best = -1;
 #pragma unroll
-            for (int j = 0; j < MAX_SYN; ++j) {
-                const int sh = perm_at(order, j);
-                if (j >= ns || sh == cur) continue;
-                const int np_ = pr[j], nm = mn[j];
-                if (nm > cur_mhd && nm > best_mhd) {
-                    best_mhd = nm;
-                    best = sh;
-                } else if (best_mhd == -1 && nm == cur_mhd && np_ > cur_pair && np_ > best_pair) {
-                    best_pair = np_;
-                    best = sh;
-                }
-            }
-            if (best_mhd != -1 || best_pair != -1) {
-                set_syn(A, cds1, k, best);
-                dirty[cds1] = 1;
+        for (int j = 0; j < MAX_SYN; ++j) {
+            const int sh = perm_at(order, j);
+            if (j >= ns || sh == cur) continue;
+            const int np_ = pr[j], nm = mn[j];
+            if (nm > cur_mhd && nm > best_mhd) {
+                best_mhd = nm;
+                best = sh;
+            } else if (best_mhd == -1 && nm == cur_mhd && np_ > cur_pair && np_ > best_pair) {
+                best_pair = np_;
+                best = sh;
             }
         }
-    }
+        if (best_mhd != -1 || best_pair != -1) {
+            set_syn(A, cds1, k, best);
+            dirty[cds1] = 1;
+        }
+    };
+    hits_by_wave((L + 1) >> 1, seg, test, work);
 }
 
 // mutationGC3, mutation.cuh:536-653 (probability = the parent's GC3 deviation)
-__device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *dirty) {
+__device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *dirty, uint16_t *seg) {
     const DevJob &J = *A.J;
     const int L = J.L, Ls = A.D->Ls;
     const uint64_t thr = bern_threshold(dev);
-    for (int q = threadIdx.x; 2 * q < L; q += blockDim.x) {
+    auto test = [&](int q) -> uint32_t {
         uint32_t r[4];
         pair_draws(A, cds, q, r);
+        uint32_t hm = 0;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = 2 * q + h;
-            if (k >= L) break;
-            const int ns = A.D->pnum[k];
-            if (!bern(r[2 * h], thr) || ns == 1) continue;
-            const uint32_t order = shuffle(r[2 * h + 1], ns);
-            const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
-            const int curg = code_gc3(A.D->code[base + cur]);
-            for (int j = 0; j < ns; ++j) {
-                const int sh = perm_at(order, j);
-                if (sh == cur) continue;
-                const int ng = code_gc3(A.D->code[base + sh]);
-                if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
-                    set_syn(A, cds, k, sh);
-                    dirty[cds] = 1;
-                    break;
-                }
+            if (k < L && bern(r[2 * h], thr) && A.D->pnum[k] > 1) hm |= 1u << h;
+        }
+        return hm;
+    };
+    auto work = [&](int k) {
+        uint32_t r[4];
+        pair_draws(A, cds, k >> 1, r);
+        const int ns = A.D->pnum[k];
+        const uint32_t order = shuffle(r[2 * (k & 1) + 1], ns);
+        const int base = A.D->pbase[k], cur = A.syn[cds * Ls + k];
+        const int curg = code_gc3(A.D->code[base + cur]);
+        for (int j = 0; j < ns; ++j) {
+            const int sh = perm_at(order, j);
+            if (sh == cur) continue;
+            const int ng = code_gc3(A.D->code[base + sh]);
+            if ((dir == GC_UP && ng > curg) || (dir == GC_DOWN && ng < curg)) {
+                set_syn(A, cds, k, sh);
+                dirty[cds] = 1;
+                break;
             }
         }
-    }
+    };
+    hits_by_wave((L + 1) >> 1, seg, test, work);
 }
 
 // mutationSL, mutation.cuh:775-969.  The accept test of the reference
@@ -792,7 +860,8 @@ __device__ void mut_gc3(const MutArgs &A, int cds, int dir, float dev, int32_t *
 // integer quotient l_new / ((n-4)/2); it only needs to know whether the CDS
 // still has any stem and whether it is one full-length hairpin, which two
 // O(n) workgroup checks over the CDS's codes decide.
-__device__ void mut_sl(const MutArgs &A, DesignShared &sh, int32_t *dirty) {
+template <class SH>
+__device__ void mut_sl(const MutArgs &A, SH &sh, int32_t *dirty) {
     const DevJob &J = *A.J;
     const int L = J.L, n = 3 * L, Ls = A.D->Ls, tid = threadIdx.x, NT = blockDim.x;
     const int cds = sh.pidx[10];
@@ -1013,16 +1082,16 @@ __device__ int survivor_at(const uint64_t *selm, int W, int c) {
 // A design's genome row is m CDS rows of stride Ls; it moves between HBM and
 // LDS in 16-byte words.
 // ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
                                                 uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_group,
                                                 int stem_cap, int item_base) {
     extern __shared__ __align__(16) uint8_t dyn[];
-    __shared__ DesignShared sh;
+    __shared__ DesignShared<NT / 64> sh;
     const int2 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
     const int L = J.L, m = J.m, Ls = J.Ls, N = J.N;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tid = threadIdx.x;
     const int G16 = (m * Ls) >> 4;  // 16-byte words of a design row
     const int SA = syn_stride(L);
     uint8_t *pbase = dyn + DL_POS;
@@ -1175,11 +1244,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         switch (type) {  // main.cu:181-211
         case 0: mut_random(A, sh.dirty); break;
         case 1: mut_cai(A, sh.pidx[0], sh.dirty); break;
-        case 2: mut_cbp(A, sh.pidx[2], sh.dirty); break;
-        case 3: mut_hsc(A, sh.pidx[4], sh.dirty); break;
-        case 4: mut_hd(A, sh.pidx[6], sh.pidx[7], sh.dirty); break;
+        case 2: mut_cbp(A, sh.pidx[2], sh.dirty, sh.hitq[tid >> 6]); break;
+        case 3: mut_hsc(A, sh.pidx[4], sh.dirty, sh.hitq[tid >> 6]); break;
+        case 4: mut_hd(A, sh.pidx[6], sh.pidx[7], sh.dirty, sh.hitq[tid >> 6]); break;
         case 5:
-            if (sh.pidx[9] == GC_UP || sh.pidx[9] == GC_DOWN) mut_gc3(A, sh.pidx[8], sh.pidx[9], sh.pobj[4], sh.dirty);
+            if (sh.pidx[9] == GC_UP || sh.pidx[9] == GC_DOWN)
+                mut_gc3(A, sh.pidx[8], sh.pidx[9], sh.pobj[4], sh.dirty, sh.hitq[tid >> 6]);
             break;
         default: mut_sl(A, sh, sh.dirty); break;
         }
@@ -2244,16 +2314,22 @@ __global__ void k_zero_np(const DevJob *__restrict__ jobs) {
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_design(int mode, const DevJob *jobs, const int2 *items, int n, size_t lds, hipStream_t s,
-                         uint32_t *stem_list, int32_t *stem_ctl, StemList sl, int item_base) {
+                         uint32_t *stem_list, int32_t *stem_ctl, StemList sl, int item_base, int threads) {
     if (n <= 0) return hipSuccess;
     switch (mode) {
-    case 0: hipLaunchKernelGGL(k_design<0>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 1, 0, 0); break;
-    case 1: hipLaunchKernelGGL(k_design<1>, dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 1, 0, 0); break;
+    case 0: hipLaunchKernelGGL((k_design<0, 256>), dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 1, 0, 0); break;
+    case 1: hipLaunchKernelGGL((k_design<1, 256>), dim3(n), dim3(256), lds, s, jobs, items, nullptr, nullptr, 1, 0, 0); break;
     default:
         if (!stem_list || !stem_ctl || sl.cap <= 0 || sl.group <= 0 || sl.groups > STEM_MAX_GROUPS || item_base < 0)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_design<2>, dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.group, sl.cap,
-                           item_base);
+        if (threads == 256)
+            hipLaunchKernelGGL((k_design<2, 256>), dim3(n), dim3(256), lds, s, jobs, items, stem_list, stem_ctl,
+                               sl.group, sl.cap, item_base);
+        else if (threads == 512)
+            hipLaunchKernelGGL((k_design<2, 512>), dim3(n), dim3(512), lds, s, jobs, items, stem_list, stem_ctl,
+                               sl.group, sl.cap, item_base);
+        else
+            return hipErrorInvalidValue;
         break;
     }
     return hipGetLastError();
@@ -2324,7 +2400,8 @@ hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int de
     return hipGetLastError();
 }
 bool shapes_valid(const Shapes &h) {
-    return h.design_split >= 0 && h.stem_waves >= 5 && h.stem_waves <= 8 &&
+    return h.design_split >= 0 && (h.design_threads == 256 || h.design_threads == 512) && h.stem_waves >= 5 &&
+           h.stem_waves <= 8 &&
            (h.dom_rows == 64 || h.dom_rows == 128 || h.dom_rows == 256) &&
            (h.fronts_threads == 256 || h.fronts_threads == 512 || h.fronts_threads == 1024) &&
            (h.assoc_designs == 4 || h.assoc_designs == 8 || h.assoc_designs == 16) &&
@@ -2379,11 +2456,13 @@ hipError_t launch_zero_np(const DevJob *jobs, int njobs, hipStream_t s) {
 
 hipError_t set_design_lds_limit(size_t bytes) {
     return raise_lds_limit(g_design_lds, bytes, [](int b) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_design<0>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-        if (e != hipSuccess) return e;
-        e = hipFuncSetAttribute((const void *)k_design<1>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-        if (e != hipSuccess) return e;
-        return hipFuncSetAttribute((const void *)k_design<2>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        const void *fns[] = {(const void *)k_design<0, 256>, (const void *)k_design<1, 256>,
+                             (const void *)k_design<2, 256>, (const void *)k_design<2, 512>};
+        for (const void *f : fns) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     });
 }
 hipError_t set_stem_lds_limit(size_t bytes) {

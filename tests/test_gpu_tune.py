@@ -20,8 +20,8 @@ def build(cdp, spec):
 
 def alone(cdp, spec, G):
     j = cdp.Job(*build(cdp, spec)).init()
-    j.shapes(set=dict(design_split=0, stem_waves=6, dom_rows=64, fronts_threads=512, assoc_designs=4,
-                      niche_threads=1024))
+    j.shapes(set=dict(design_split=0, design_threads=256, stem_waves=6, dom_rows=64, fronts_threads=512,
+                      assoc_designs=4, niche_threads=1024))
     j.step(G)
     r = j.read()
     j.close()
@@ -70,7 +70,7 @@ def test_invalid_shapes_rejected(cdp):
     j = cdp.Job(*build(cdp, SPECS[0])).init()
     good = j.shapes()
     for k, v in (("stem_waves", 9), ("dom_rows", 96), ("fronts_threads", 128), ("assoc_designs", 2),
-                 ("niche_threads", 2048), ("design_split", -1)):
+                 ("niche_threads", 2048), ("design_split", -1), ("design_threads", 384)):
         bad = dict(good, **{k: v})
         with pytest.raises(cdp.CdError, match="CD_E_INVALID"):
             j.shapes(set=bad)
