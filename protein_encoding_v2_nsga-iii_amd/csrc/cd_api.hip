@@ -138,7 +138,7 @@ namespace {
 
 // Lay out every device buffer of a job in one allocation.
 struct Layout {
-    size_t pos, code, lw6, cpfx6, cps6, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], planes, stem_max, slist,
+    size_t pos, code, lw6, cpfx6, cps6, ref, genome[2], obj[2], oidx[2], pql[2], cst[2], hdp[2], planes, stem_max, slist,
         sctl,
         front,
         np, dom,
@@ -160,6 +160,7 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
             o.oidx[b] = s.add((size_t)P * 12);
             o.pql[b] = s.add((size_t)P * 16);
             o.cst[b] = s.add((size_t)P * m * sizeof(cd::CdsStat));
+            o.hdp[b] = s.add((size_t)P * (m * (m - 1) / 2) * 4);
         }
         o.off = s.add((size_t)N * sizeof(int2));
         o.planes = s.add((size_t)P * m * cd::cds_plane_bytes(L));
@@ -357,6 +358,7 @@ int create_job(const cd_job_desc *d, bool sel_only, int P_override, cd_job **out
             h.oidx[b] = (int8_t *)(base + o.oidx[b]);
             h.pql[b] = (int32_t *)(base + o.pql[b]);
             h.cst[b] = (cd::CdsStat *)(base + o.cst[b]);
+            h.hdp[b] = (int32_t *)(base + o.hdp[b]);
         }
         std::vector<int2> off(J->N);
         for (int c = 0; c < J->N; ++c) off[c] = make_int2(0, c);

@@ -84,7 +84,6 @@ struct DesignShared {
     int32_t hd[MAX_M * (MAX_M - 1) / 2];
     int64_t red[8 * NW];
     int32_t red32[8 * NW];
-    float ov[4][MAX_M];  // per CDS: CAI, CPB, HSC quotient, GC3 fraction
     uint64_t redu[16];
     int32_t misc[16];
     int32_t slw[NW * 11];  // mut_sl: per wave, first [4], last [4], full [3]
@@ -92,31 +91,24 @@ struct DesignShared {
     uint16_t hitq[NW][HITQ];  // per wave: codons that passed an operator's test, awaiting their work
 };
 
-// CdsStat records in global memory, field by field through the global
+// CdsStat records in global memory, four 16-byte words through the global
 // address space (an aggregate copy would go through flat accesses).
+struct CdsWords {
+    u32x4 w[4];
+};
+static_assert(sizeof(CdsWords) == sizeof(CdsStat), "a CdsStat record is four 16-byte words");
 __device__ __forceinline__ CdsStat gload_stat(const CdsStat *p) {
-    CdsStat s;
-    s.cai = gload(&p->cai);
-    s.cpb = gload(&p->cpb);
-    s.hsc = gload(&p->hsc);
-    s.gc3 = gload(&p->gc3);
-    s.l = gload(&p->l);
-    s.p = gload(&p->p);
-    s.q = gload(&p->q);
-    s.pad = gload(&p->pad);
-    s.key = gload(&p->key);
-    return s;
+    const u32x4 *q = (const u32x4 *)p;
+    CdsWords v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v.w[i] = gload(q + i);
+    return __builtin_bit_cast(CdsStat, v);
 }
 __device__ __forceinline__ void gstore_stat(CdsStat *p, const CdsStat &s) {
-    gstore(&p->cai, s.cai);
-    gstore(&p->cpb, s.cpb);
-    gstore(&p->hsc, s.hsc);
-    gstore(&p->gc3, s.gc3);
-    gstore(&p->l, s.l);
-    gstore(&p->p, s.p);
-    gstore(&p->q, s.q);
-    gstore(&p->pad, s.pad);
-    gstore(&p->key, s.key);
+    u32x4 *q = (u32x4 *)p;
+    const CdsWords v = __builtin_bit_cast(CdsWords, s);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) gstore(q + i, v.w[i]);
 }
 
 // k_design's LDS tables and the per-position synonym ranges.
@@ -213,10 +205,18 @@ __device__ __forceinline__ bool stem_qgram(int n, const uint2 *sp, const uint2 *
     for (;;) {
         int cnt = 0;  // the same in every lane of the wave
         while (cnt <= QG_SEG - 64) {
-            const bool live = !qg_lane_done(ln, nch) && budget > 0;
-            if (!__any(live)) break;
+            const bool live = !qg_lane_done(ln, nch) && budget > 0 && !qg_chunk_done(ln);
+            if (!__any(live)) {
+                // every lane of the wave has walked its chunk: the lanes with
+                // chunks left load their next ones together (a wave's steps
+                // then only read chain entries)
+                const bool adv = !qg_lane_done(ln, nch) && budget > 0;
+                if (!__any(adv)) break;
+                if (adv) qg_lane_next(ln, rp, head, nch, NT, xe);
+                continue;
+            }
             int a = 0, x = 0;
-            const bool hit = live && qg_lane_step(ln, rp, head, next, n, st, nch, NT, budget, a, x);
+            const bool hit = live && qg_walk_step(ln, next, n, st, budget, a, x);
             const uint64_t bal = __ballot(hit);
             const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
@@ -346,12 +346,28 @@ __host__ __device__ inline int stem_parts(int n) {
 // Evaluation of the CDS flagged dirty, then objectives 0-4
 // (calMinimumCAI/CBP/HSC/HD, calMaximumGC3, utils.cuh:271-626), on codon
 // codes.  A changed CDS is marked for k_stem (pad = 1); its stem fields keep
-// the parent's stem, which seeds the scan, or are cleared when there is no
-// parent.  Up to four dirty CDS share one pass over the positions; the
-// Hamming distances take four codons per 32-bit word of two code rows, eight
-// CDS pairs per pass; the sums are wave sums (DPP) added over the waves by
-// one thread per value.
+// the parent's stem, which seeds the scan (have_parent), or are cleared.  Up
+// to four changed CDS share one pass over the positions; the sums are wave
+// sums (DPP) added over the waves by one thread per value, which also
+// computes the CDS's values (CAI, CPB, HSC quotient, GC3 fraction) kept in
+// its record.  Hamming distances take four codons per 32-bit word of two code
+// rows, a wave per pair: every pair without a parent, only the pairs of a
+// changed CDS with one (sh.hd holds the parent's distances on entry).
 // ---------------------------------------------------------------------------
+// Per-CDS values of the sums: CAI, CPB, the HSC quotient and the GC3 fraction.
+__device__ __forceinline__ void cds_values(CdsStat &s, int L) {
+    s.ov[0] = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
+    s.ov[1] = (float)(((double)s.cpb * 0x1p-40) / (double)(L - 2));
+    s.ov[2] = (float)s.hsc / (float)(L - 1);
+    s.ov[3] = (float)s.gc3 / (float)L;
+}
+
+// The pick-th (0-based) set bit of a candidate mask.
+__device__ __forceinline__ int nth_bit(uint32_t mask, uint32_t pick) {
+    for (uint32_t q = 0; q < pick; ++q) mask &= mask - 1u;
+    return __builtin_ctz(mask);
+}
+
 template <class SH>
 __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6, CdsStat *cst, bool have_parent,
                          SH &sh, float *obj, int32_t *oidx) {
@@ -359,80 +375,85 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = NT >> 6;
     uint32_t dmask = 0;  // the same in every thread
     for (int c = 0; c < m; ++c) dmask |= (uint32_t)(sh.dirty[c] != 0) << c;
-    while (dmask) {
-        int cs[4];
+    {
+        uint32_t left = dmask;
+        while (left) {
+            int cs[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            cs[u] = dmask ? __builtin_ctz(dmask) : -1;
-            dmask &= dmask - 1u;
-        }
-        // one CDS at a time (its sums reduced over the wave into LDS before
-        // the next), four codons a thread per round: the pair scores of all
-        // four (global table, one dependent load each) are fetched together
+            for (int u = 0; u < 4; ++u) {
+                cs[u] = left ? __builtin_ctz(left) : -1;
+                left &= left - 1u;
+            }
+            // one CDS at a time (its sums reduced over the wave into LDS before
+            // the next), four codons a thread per round: the pair scores of all
+            // four (global table, one dependent load each) are fetched together
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (cs[u] < 0) continue;
-            const uint8_t *g = cd6 + cs[u] * Ls;
-            int64_t cai = 0, cpb = 0;
-            int hsc = 0, gcn = 0;
-            for (int k0 = tid; k0 < L; k0 += 4 * NT) {
-                int64_t ps[4];
+            for (int u = 0; u < 4; ++u) {
+                if (cs[u] < 0) continue;
+                const uint8_t *g = cd6 + cs[u] * Ls;
+                int64_t cai = 0, cpb = 0;
+                int hsc = 0, gcn = 0;
+                for (int k0 = tid; k0 < L; k0 += 4 * NT) {
+                    int64_t ps[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int k = k0 + v * NT;
-                    const bool pair = k < L - 2;
-                    const int kk = pair ? k : 0;  // an index inside the row either way
-                    ps[v] = pair ? gload(J.cpfx6 + g[kk] * 64 + g[kk + 1]) : 0;
+                    for (int v = 0; v < 4; ++v) {
+                        const int k = k0 + v * NT;
+                        const bool pair = k < L - 2;
+                        const int kk = pair ? k : 0;  // an index inside the row either way
+                        ps[v] = pair ? gload(J.cpfx6 + g[kk] * 64 + g[kk + 1]) : 0;
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int k = k0 + v * NT;
+                        if (k >= L) break;
+                        const int a = g[k];
+                        cai += D.lw6[a];
+                        gcn += code_gc3(a);
+                        if (k < L - 1) hsc += code_hidden(D.stop6, a, g[k + 1]);
+                        cpb += ps[v];
+                    }
                 }
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int k = k0 + v * NT;
-                    if (k >= L) break;
-                    const int a = g[k];
-                    cai += D.lw6[a];
-                    gcn += code_gc3(a);
-                    if (k < L - 1) hsc += code_hidden(D.stop6, a, g[k + 1]);
-                    cpb += ps[v];
+                cai = wave_sum_i64(cai);
+                cpb = wave_sum_i64(cpb);
+                hsc = wave_sum_i32(hsc);
+                gcn = wave_sum_i32(gcn);
+                if (lane == 0) {
+                    sh.red[wid * 8 + 2 * u] = cai;
+                    sh.red[wid * 8 + 2 * u + 1] = cpb;
+                    sh.red32[wid * 8 + 2 * u] = hsc;
+                    sh.red32[wid * 8 + 2 * u + 1] = gcn;
                 }
             }
-            cai = wave_sum_i64(cai);
-            cpb = wave_sum_i64(cpb);
-            hsc = wave_sum_i32(hsc);
-            gcn = wave_sum_i32(gcn);
-            if (lane == 0) {
-                sh.red[wid * 8 + 2 * u] = cai;
-                sh.red[wid * 8 + 2 * u + 1] = cpb;
-                sh.red32[wid * 8 + 2 * u] = hsc;
-                sh.red32[wid * 8 + 2 * u + 1] = gcn;
-            }
-        }
-        lds_barrier();
-        if (tid < 4) {
-            const int c = tid == 0 ? cs[0] : tid == 1 ? cs[1] : tid == 2 ? cs[2] : cs[3];
-            if (c >= 0) {
-                int64_t a = 0, b = 0;
-                int h = 0, g = 0;
-                for (int w = 0; w < nw; ++w) {
-                    a += sh.red[w * 8 + 2 * tid];
-                    b += sh.red[w * 8 + 2 * tid + 1];
-                    h += sh.red32[w * 8 + 2 * tid];
-                    g += sh.red32[w * 8 + 2 * tid + 1];
+            lds_barrier();
+            if (tid < 4) {
+                const int c = tid == 0 ? cs[0] : tid == 1 ? cs[1] : tid == 2 ? cs[2] : cs[3];
+                if (c >= 0) {
+                    int64_t a = 0, b = 0;
+                    int h = 0, g = 0;
+                    for (int w = 0; w < nw; ++w) {
+                        a += sh.red[w * 8 + 2 * tid];
+                        b += sh.red[w * 8 + 2 * tid + 1];
+                        h += sh.red32[w * 8 + 2 * tid];
+                        g += sh.red32[w * 8 + 2 * tid + 1];
+                    }
+                    CdsStat s = cst[c];
+                    if (!have_parent) s.l = s.p = s.q = 0;
+                    s.cai = a;
+                    s.cpb = b;
+                    s.hsc = h;
+                    s.gc3 = g;
+                    s.pad = 1;
+                    s.key = 0;
+                    cds_values(s, L);
+                    cst[c] = s;
                 }
-                CdsStat s = cst[c];
-                if (!have_parent) s.l = s.p = s.q = 0;
-                s.cai = a;
-                s.cpb = b;
-                s.hsc = h;
-                s.gc3 = g;
-                s.pad = 1;
-                s.key = 0;
-                cst[c] = s;
             }
+            lds_barrier();
         }
-        lds_barrier();
     }
     // pairwise Hamming distances, pairs (i, j), i < j, in lexicographic order,
-    // a wave each (pair p on wave p mod nw); the rows' padding codes are 0 in
+    // a wave each (pair p on wave p mod nw): every pair without a parent,
+    // those with a changed CDS otherwise; the rows' padding codes are 0 in
     // every CDS and add nothing
     {
         const int npair = m * (m - 1) / 2, nwd = Ls >> 2;
@@ -444,24 +465,18 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
                 j = i + 1;
             }
         for (int p = wid; p < npair; p += nw) {
-            int acc = 0;
-            for (int q = lane; q < nwd; q += 64) acc += code_mismatch4(w6[i * nwd + q] ^ w6[j * nwd + q]);
-            acc = wave_sum_i32(acc);
-            if (lane == 0) sh.hd[p] = acc;
+            if (!have_parent || ((dmask >> i) & 1u) || ((dmask >> j) & 1u)) {
+                int acc = 0;
+                for (int q = lane; q < nwd; q += 64) acc += code_mismatch4(w6[i * nwd + q] ^ w6[j * nwd + q]);
+                acc = wave_sum_i32(acc);
+                if (lane == 0) sh.hd[p] = acc;
+            }
             for (int t = 0; t < nw; ++t)  // the pair nw further on
                 if (++j == m) {
                     ++i;
                     j = i + 1;
                 }
         }
-    }
-    lds_barrier();
-    if (tid < m) {  // per-CDS values, one lane each
-        const CdsStat s = cst[tid];
-        sh.ov[0][tid] = (float)det_exp(((double)s.cai * 0x1p-40) / (double)L);
-        sh.ov[1][tid] = (float)(((double)s.cpb * 0x1p-40) / (double)(L - 2));
-        sh.ov[2][tid] = (float)s.hsc / (float)(L - 1);
-        sh.ov[3][tid] = (float)s.gc3 / (float)L;
     }
     lds_barrier();
     // The objectives (calMinimumCAI .. calMaximumGC3, utils.cuh:271-626): each
@@ -476,10 +491,11 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
         auto key = [](bool ok, float v, int i) -> uint64_t {
             return ok ? ((uint64_t)ord_bits(v) << 32) | (uint32_t)i : 0ull;
         };
-        const uint64_t k_cai = wave_max_u64(key(cl, -sh.ov[0][c], lane));
-        const uint64_t k_cpb = wave_max_u64(key(cl, -sh.ov[1][c], lane));
-        const uint64_t k_hsc = wave_max_u64(key(cl, -sh.ov[2][c], lane));
-        const uint64_t k_gc = wave_max_u64(key(cl, fabsf(sh.ov[3][c] - gc3_ref), lane));
+        const CdsStat &sc = cst[c];
+        const uint64_t k_cai = wave_max_u64(key(cl, -sc.ov[0], lane));
+        const uint64_t k_cpb = wave_max_u64(key(cl, -sc.ov[1], lane));
+        const uint64_t k_hsc = wave_max_u64(key(cl, -sc.ov[2], lane));
+        const uint64_t k_gc = wave_max_u64(key(cl, fabsf(sc.ov[3] - gc3_ref), lane));
         const int npair = m * (m - 1) / 2;
         uint64_t kh = 0;
         for (int q = lane; q < npair; q += 64) {
@@ -503,12 +519,12 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
             oidx[6] = i;
             oidx[7] = j;
             oidx[8] = cg;
-            oidx[9] = (sh.ov[3][cg] - gc3_ref) < 0 ? GC_UP : GC_DOWN;
-            obj[0] = -sh.ov[0][ca];
-            obj[1] = -sh.ov[1][cb];
-            obj[2] = -sh.ov[2][ch];
+            oidx[9] = (cst[cg].ov[3] - gc3_ref) < 0 ? GC_UP : GC_DOWN;
+            obj[0] = -cst[ca].ov[0];
+            obj[1] = -cst[cb].ov[1];
+            obj[2] = -cst[ch].ov[2];
             obj[3] = -((float)sh.hd[qh] / (float)n);
-            obj[4] = fabsf(sh.ov[3][cg] - gc3_ref);
+            obj[4] = fabsf(cst[cg].ov[3] - gc3_ref);
         }
     }
     lds_barrier();
@@ -523,11 +539,6 @@ __device__ void evaluate(const DevJob &J, const DesignLds &D, const uint8_t *cd6
 // shuffle) of Philox(gen, child, c*L + (i >> 1), TAG_MUT), so one call serves
 // two slots and a thread takes the two slots of one call.
 // ---------------------------------------------------------------------------
-// The pick-th (0-based) set bit of a candidate mask.
-__device__ __forceinline__ int nth_bit(uint32_t mask, uint32_t pick) {
-    for (uint32_t q = 0; q < pick; ++q) mask &= mask - 1u;
-    return __builtin_ctz(mask);
-}
 
 struct MutArgs {
     const DevJob *J;
@@ -1099,6 +1110,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     uint8_t *syn = pnum + Ls;       // rows at stride Ls; the planes' stage afterwards (stride SA)
     uint8_t *cd6 = syn + m * SA;
     CdsStat *cst = (CdsStat *)(cd6 + m * Ls);
+    const int npair = m * (m - 1) / 2;
     const DesignLds D{(const int64_t *)(dyn + DL_LW6), dyn + DL_CODE, pbase, pnum, J.stop6, Ls};
 
     // The descriptor fields of the staging phase, read before its stores (a
@@ -1112,6 +1124,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     int8_t *const oidx0 = J.oidx[0], *const oidx1 = J.oidx[1];
     int32_t *const pql0 = J.pql[0], *const pql1 = J.pql[1];
     CdsStat *const cst0 = J.cst[0], *const cst1 = J.cst[1];
+    int32_t *const hdp0 = J.hdp[0], *const hdp1 = J.hdp[1];
     int32_t *const np = J.np;
     SelAcc *const acc = J.acc;
     // stage the tables and the synonym ranges (rows pbase and pnum, as the
@@ -1165,6 +1178,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
         int32_t *const pql_d = src ? pql0 : pql1;
         const CdsStat *const cst_s = src ? cst1 : cst0;
         CdsStat *const cst_d = src ? cst0 : cst1;
+        const int32_t *const hdp_s = src ? hdp1 : hdp0;
+        int32_t *const hdp_d = src ? hdp0 : hdp1;
         const u32x4 *pg = (const u32x4 *)((src ? genome1 : genome0) + (size_t)parent * m * Ls);
         u32x4 *sg = (u32x4 *)((dst ? genome1 : genome0) + (size_t)c * m * Ls);
         // the records' loads first (lanes of wave 0), then the genome row's
@@ -1172,6 +1187,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
         int8_t vi = 0;
         int32_t vp = 0;
         CdsStat cs{};
+        int32_t vh = 0;
+        const int hp = tid - 64;  // lanes 64 .. 64 + npair - 1: the pairs' distances
+        if (hp >= 0 && hp < npair) vh = gload(hdp_s + (size_t)parent * npair + hp);
         if (tid < 16) {
             vo = gload(obj_s + (size_t)parent * NOBJ + min(tid, NOBJ - 1));
             vi = gload(oidx_s + (size_t)parent * 2 * NOBJ + min(tid, 2 * NOBJ - 1));
@@ -1207,7 +1225,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
                 gstore_stat(cst_d + (size_t)c * m + tid, cs);
                 cst[tid] = cs;
             }
-        } else if (tid == 64) {  // the sort of this generation starts from cleared counts
+        }
+        if (hp >= 0 && hp < npair) {
+            gstore(hdp_d + (size_t)c * npair + hp, vh);
+            sh.hd[hp] = vh;
+        }
+        if (tid == NT - 1) {  // the sort of this generation starts from cleared counts
             gstore(np + c, 0);
             gstore(np + N + c, 0);
         }
@@ -1256,12 +1279,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
         lds_barrier();
     }
 
-    float *oobj = J.obj[dst] + (size_t)out_slot * NOBJ;
+    float *oobj = (dst ? obj1 : obj0) + (size_t)out_slot * NOBJ;
     __shared__ float s_obj[NOBJ];
     __shared__ int32_t s_idx[2 * NOBJ];
-    evaluate(J, D, cd6, cst, MODE == 2, sh, s_obj, s_idx);
-    u32x4 *og = (u32x4 *)(J.genome[dst] + (size_t)out_slot * m * Ls);
+    // the design's genome row is final: its stores drain during the evaluation
+    u32x4 *og = (u32x4 *)((dst ? genome1 : genome0) + (size_t)out_slot * m * Ls);
     for (int i = tid; i < G16; i += NT) gstore(og + i, syn4[i]);
+    evaluate(J, D, cd6, cst, MODE == 2, sh, s_obj, s_idx);
     {  // planes of the changed CDS for k_stem, staged where the syn rows were
         int nd = 0;
         for (int c = 0; c < m; ++c) {
@@ -1273,8 +1297,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
                      J.planes + (size_t)out_slot * m * 2 * scan_words(3 * L));
     }
     if (tid < NOBJ - 1) gstore(oobj + tid, s_obj[tid]);  // objective 5: k_stem_finish
-    if (tid < 2 * NOBJ && tid != 10) gstore(J.oidx[dst] + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
-    if (tid < m) gstore_stat(J.cst[dst] + (size_t)out_slot * m + tid, cst[tid]);
+    if (tid < 2 * NOBJ && tid != 10) gstore((dst ? oidx1 : oidx0) + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
+    if (tid < m) gstore_stat((dst ? cst1 : cst0) + (size_t)out_slot * m + tid, cst[tid]);
+    if (tid < npair) gstore((dst ? hdp1 : hdp0) + (size_t)out_slot * npair + tid, sh.hd[tid]);
     if (tid == 0) {  // CDS evaluated (k_stem scans the same ones); the unchanged CDS's longest stem
         unsigned long long nd = 0;
         int lmax = 0;
@@ -1564,7 +1589,10 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
     for (int q = 0; q < WR; ++q) {
         const int r = r0 + q;
         const float *row = rows + (wv * WR + q) * NOBJ;
-        const uint64_t bits = __ballot(i < P && i != r && dominates(cand, row));
+        // the test runs in every lane (no branch around the row's LDS
+        // reads, so that the unrolled rows' reads issue together)
+        const bool dom = dominates(cand, row);
+        const uint64_t bits = __ballot(dom && i < P && i != r);
         if (lane == q) mine = bits;
     }
     if (lane < WR && r0 + lane < P) {

@@ -35,7 +35,9 @@ struct CdsStat {
     int32_t l, p, q;  // longest stem and its position (canonical tie rule)
     int32_t pad;      // 1 while the stem fields await k_stem
     uint64_t key;     // best stem key of the parts of k_stem (atomic max), valid while pad is 1
+    float ov[4];      // the CDS's CAI, CPB, HSC quotient and GC3 fraction, from the sums above
 };
+static_assert(sizeof(CdsStat) == 64, "CdsStat records move as four 16-byte words");
 
 // Selection state of one job (the reference keeps it in __device__ globals,
 // common.cuh:46-59).
@@ -132,6 +134,7 @@ struct DevJob {
     int8_t *oidx[2];       // [P][12]
     int32_t *pql[2];       // [P][4] (p, q, l, 0)
     CdsStat *cst[2];       // [P][m]
+    int32_t *hdp[2];       // [P][m(m-1)/2] Hamming distances of the design's CDS pairs (i < j, lexicographic)
     uint2 *planes;         // [P][m][2][scan_words(3L)] bit planes (sequence, reverse complement) of the CDS
                            // k_design evaluated, read by k_stem
     int32_t *front_of;     // [P]

@@ -157,17 +157,22 @@ __host__ __device__ inline void qg_lane_start(QgLane &s, const uint2 *rp, const 
 
 __host__ __device__ inline bool qg_lane_done(const QgLane &s, int nch) { return s.ch >= nch; }
 
-// One step of a lane that is not done: returns true with the hit (a, b)
-// when the step read a chain entry whose q-gram is valid.  Each chain entry
+// The lane's current chunk has no chain entry left to read.
+__host__ __device__ inline bool qg_chunk_done(const QgLane &s) { return s.e == QG_NIL && s.mask == 0u; }
+
+// The lane's next chunk (ch + stride), loaded when there is one.
+__host__ __device__ inline void qg_lane_next(QgLane &s, const uint2 *rp, const uint16_t *head, int nch, int stride,
+                                             int xe) {
+    s.ch += stride;
+    if (s.ch < nch) qg_lane_load(s, rp, head, xe);
+}
+
+// One chain entry of the lane's current chunk (not qg_chunk_done): returns
+// true with the hit (a, b) when the entry's q-gram is valid.  Each entry
 // read costs one unit of budget.
-__host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const uint16_t *head, const uint16_t *next,
-                                             int n, int st, int nch, int stride, int &budget, int &a, int &b) {
+__host__ __device__ inline bool qg_walk_step(QgLane &s, const uint16_t *next, int n, int st, int &budget, int &a,
+                                             int &b) {
     if (s.e == QG_NIL) {
-        if (!s.mask) {  // the chunk is done: the lane's next chunk
-            s.ch += stride;
-            if (s.ch < nch) qg_lane_load(s, rp, head, n - QG_Q + 1);
-            return false;
-        }
         const int j = __builtin_ctz(s.mask);
         s.j = j;
         s.mask &= s.mask - 1u;
@@ -182,6 +187,17 @@ __host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const u
     a = st * (int)e;
     b = (s.ch << 3) + s.j;
     return a + b + 2 * (QG_Q - 1) <= n - 6;
+}
+
+// One step of a lane that is not done: the next chunk when the current one
+// is walked (returns false), else one chain entry (qg_walk_step).
+__host__ __device__ inline bool qg_lane_step(QgLane &s, const uint2 *rp, const uint16_t *head, const uint16_t *next,
+                                             int n, int st, int nch, int stride, int &budget, int &a, int &b) {
+    if (qg_chunk_done(s)) {
+        qg_lane_next(s, rp, head, nch, stride, n - QG_Q + 1);
+        return false;
+    }
+    return qg_walk_step(s, next, n, st, budget, a, b);
 }
 
 // A hit packed for the queue of hits to measure (both below 2^16).

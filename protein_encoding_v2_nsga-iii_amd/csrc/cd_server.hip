@@ -76,6 +76,12 @@ double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::d
 
 struct cd_server {
     std::mutex mu;
+    // held by the worker while it captures a tick, and by the other threads
+    // around the HIP calls that create, initialise, collect or destroy a job
+    // (stream and event creation, waits and destruction, stream-ordered
+    // allocation): such calls from another thread can end a capture in
+    // progress
+    std::mutex capture_mu;
     std::condition_variable cv_admit, cv_work, cv_collect, cv_done;
     std::thread admitter, worker, collector;
     bool stop = false;
@@ -103,7 +109,8 @@ struct cd_server {
     size_t cap_jobs = 0, cap_off = 0, cap_dom = 0, cap_asc = 0, cap_slist = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    std::string fatal;  // a device error that stopped the server
+    std::string fatal;        // a device error that stopped the server
+    std::string fatal_stage;  // the step of the tick's capture that failed, if it was one (worker only)
 };
 
 namespace {
@@ -137,13 +144,16 @@ int collect(Entry *E) {
 
 // Release a job's device memory and end its request.
 void finish(cd_server *S, Entry *E, int state, const std::string &err) {
-    if (E->job) {
-        cd_job_destroy(E->job);
-        E->job = nullptr;
+    {
+        std::lock_guard<std::mutex> cap(S->capture_mu);
+        if (E->job) {
+            cd_job_destroy(E->job);
+            E->job = nullptr;
+        }
+        if (E->ready) (void)hipEventDestroy(E->ready);
+        if (E->last) (void)hipEventDestroy(E->last);
+        E->ready = E->last = nullptr;
     }
-    if (E->ready) (void)hipEventDestroy(E->ready);
-    if (E->last) (void)hipEventDestroy(E->last);
-    E->ready = E->last = nullptr;
     std::lock_guard<std::mutex> lk(S->mu);
     E->state = state;
     E->error = err;
@@ -191,10 +201,15 @@ void admit_loop(cd_server *S) {
             S->used += E->req.bytes;
             S->admitted += 1;
         }
-        int rc = cd_job_create(&E->req.desc, &E->job);
-        if (rc == CD_OK) rc = cd_job_init(E->job);  // enqueued on the job's own stream
-        if (rc == CD_OK && hipEventCreateWithFlags(&E->ready, hipEventDisableTiming) != hipSuccess) rc = CD_E_DEVICE;
-        if (rc == CD_OK && hipEventRecord(E->ready, E->job->stream) != hipSuccess) rc = CD_E_DEVICE;
+        int rc = CD_OK;
+        {
+            std::lock_guard<std::mutex> cap(S->capture_mu);
+            rc = cd_job_create(&E->req.desc, &E->job);
+            if (rc == CD_OK) rc = cd_job_init(E->job);  // enqueued on the job's own stream
+            if (rc == CD_OK && hipEventCreateWithFlags(&E->ready, hipEventDisableTiming) != hipSuccess)
+                rc = CD_E_DEVICE;
+            if (rc == CD_OK && hipEventRecord(E->ready, E->job->stream) != hipSuccess) rc = CD_E_DEVICE;
+        }
         if (rc != CD_OK) {
             finish(S, E, CD_SERVER_FAILED, cd_last_error());
             continue;
@@ -218,8 +233,13 @@ void collect_loop(cd_server *S) {
             S->done.pop_front();
         }
         int rc = CD_OK;
-        if (E->last && hipEventSynchronize(E->last) != hipSuccess) rc = CD_E_DEVICE;
-        if (rc == CD_OK) rc = collect(E);
+        {
+            // E->last was recorded on the tick stream, which the worker may be
+            // capturing: waiting on it then would end the capture
+            std::lock_guard<std::mutex> cap(S->capture_mu);
+            if (E->last && hipEventSynchronize(E->last) != hipSuccess) rc = CD_E_DEVICE;
+            if (rc == CD_OK) rc = collect(E);
+        }
         finish(S, E, rc == CD_OK ? CD_SERVER_DONE : CD_SERVER_FAILED, rc == CD_OK ? "" : cd_last_error());
     }
 }
@@ -253,14 +273,26 @@ hipError_t rebuild(cd_server *S, const std::vector<Entry *> &res) {
     if (S->graph) (void)hipGraphDestroy(S->graph);
     S->gexec = nullptr;
     S->graph = nullptr;
-    // relaxed: the admitting and collecting threads go on allocating, copying
-    // and freeing on their own streams while this thread captures
+    // job creation, initialisation, collection and destruction on the other
+    // threads wait for the capture (capture_mu)
+    std::lock_guard<std::mutex> cap(S->capture_mu);
     e = hipStreamBeginCapture(S->stream, hipStreamCaptureModeRelaxed);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        S->fatal_stage = "begin capture";
+        return e;
+    }
     const hipError_t ec = cdhost::enqueue_generation(L, S->stream, nullptr);
     e = hipStreamEndCapture(S->stream, &S->graph);
-    if (ec != hipSuccess) return ec;
-    if (e == hipSuccess) e = hipGraphInstantiate(&S->gexec, S->graph, nullptr, nullptr, 0);
+    if (ec != hipSuccess) {
+        S->fatal_stage = "capture";
+        return ec;
+    }
+    if (e != hipSuccess) {
+        S->fatal_stage = "end capture";
+        return e;
+    }
+    e = hipGraphInstantiate(&S->gexec, S->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) S->fatal_stage = "graph instantiation";
     return e;
 }
 
@@ -344,7 +376,8 @@ void work_loop(cd_server *S) {
         if (e == hipSuccess) e = hipEventRecord(ev[slot][1], S->stream);
         const Clock::time_point w2 = Clock::now();
         if (e != hipSuccess) {  // a device error ends every request and the server
-            fail_all(S, std::string("serving loop: ") + hipGetErrorString(e));
+            fail_all(S, std::string("serving loop: ") + (S->fatal_stage.empty() ? "" : S->fatal_stage + ": ") +
+                            hipGetErrorString(e));
             break;
         }
         inflight[slot] = true;
