@@ -1446,30 +1446,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 // stems k_stem found and clears the marks k_design left.
 __device__ int stem_finish(const DevJob &J, int buf, int slot) {
     const int m = J.m;
-    CdsStat *cs = J.cst[buf] + (size_t)slot * m;
+    // a record's 16-byte words 1 (hsc, gc3, l, p) and 2 (q, pad, key), four
+    // records' loads issued together
+    u32x4 *cs = (u32x4 *)(J.cst[buf] + (size_t)slot * m);
     int bl = 0, bp = 0, bq = 0, bc = m - 1, changed = 0;
-    for (int c = 0; c < m; ++c) {
-        CdsStat s = cs[c];
-        if (s.pad) {
-            stem_unkey(s.key, s.l, s.p, s.q);
-            s.pad = 0;
-            cs[c] = s;
-            ++changed;
+    for (int c0 = 0; c0 < m; c0 += 4) {
+        u32x4 w1[4], w2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = min(c0 + u, m - 1);
+            w1[u] = gload(cs + 4 * c + 1);
+            w2[u] = gload(cs + 4 * c + 2);
         }
-        if (s.l >= bl) {
-            bl = s.l;
-            bp = s.p;
-            bq = s.q;
-            bc = c;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = c0 + u;
+            if (c >= m) break;
+            int l = (int)w1[u].z, p = (int)w1[u].w, q = (int)w2[u].x;
+            if (w2[u].y) {  // pad: the stem k_stem found
+                stem_unkey((uint64_t)w2[u].z | ((uint64_t)w2[u].w << 32), l, p, q);
+                w1[u].z = (uint32_t)l;
+                w1[u].w = (uint32_t)p;
+                w2[u].x = (uint32_t)q;
+                w2[u].y = 0u;
+                gstore(cs + 4 * c + 1, w1[u]);
+                gstore(cs + 4 * c + 2, w2[u]);
+                ++changed;
+            }
+            if (l >= bl) {
+                bl = l;
+                bp = p;
+                bq = q;
+                bc = c;
+            }
         }
     }
-    J.obj[buf][(size_t)slot * NOBJ + 5] = (float)bl / (float)J.den_sl;
-    J.oidx[buf][(size_t)slot * 2 * NOBJ + 10] = (int8_t)bc;
+    gstore(J.obj[buf] + (size_t)slot * NOBJ + 5, (float)bl / (float)J.den_sl);
+    gstore(J.oidx[buf] + (size_t)slot * 2 * NOBJ + 10, (int8_t)bc);
     int32_t *pq = J.pql[buf] + (size_t)slot * 4;
-    pq[0] = bp;
-    pq[1] = bq;
-    pq[2] = bl;
-    pq[3] = 0;
+    gstore(pq, bp);
+    gstore(pq + 1, bq);
+    gstore(pq + 2, bl);
+    gstore(pq + 3, 0);
     return changed;
 }
 
@@ -1552,23 +1570,37 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         for (int g = threadIdx.x; g < ctl_n; g += blockDim.x) gstore(stem_ctl + g, 0);
     const int4 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
-    const int P = J.P, N = J.N, word = it.z, buf = pool_buffer(J);
-    const float *obj = J.obj[buf];
+    const int P = J.P, N = J.N, word = it.z;
+    const uint32_t gen = __builtin_amdgcn_readfirstlane(gload(&J.st->gen));
+    const int buf = (int)((gen + 1u) & 1u);
+    const float *const obj = J.obj[buf];
+    const int32_t *const smax = J.stem_max;
+    const float den_sl = (float)J.den_sl;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // in the offspring generation a child's objective 5 is its longest stem
-    // over (3L-4)/2, the value stem_finish stores
-    auto obj_at = [&](int d, int k) -> float {
-        if (finish && k == NOBJ - 1 && d >= N) return (float)gload(J.stem_max + d) / (float)J.den_sl;
-        return gload(obj + (size_t)d * NOBJ + k);
-    };
-    for (int t = threadIdx.x; t < ROWS * NOBJ; t += blockDim.x) {
-        const int r = it.y + t / NOBJ;
-        rows[t] = r < P ? obj_at(r, t % NOBJ) : 0.f;
+    // over (3L-4)/2, the value stem_finish stores.  Every load is issued
+    // first, at a clamped index, then the values are chosen.
+    constexpr int PER = (ROWS * NOBJ + 255) / 256;  // row values per thread
+    float v[PER];
+    int32_t sv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int t = min((int)threadIdx.x + 256 * u, ROWS * NOBJ - 1), r = min(it.y + t / NOBJ, P - 1);
+        v[u] = gload(obj + (size_t)r * NOBJ + t % NOBJ);
+        sv[u] = finish ? gload(smax + r) : 0;  // (a sort alone has no stem_max)
     }
-    const int i = word * 64 + lane;
+    const int i = word * 64 + lane, ic = min(i, P - 1);
     float cand[NOBJ];
 #pragma unroll
-    for (int k = 0; k < NOBJ; ++k) cand[k] = i < P ? obj_at(i, k) : 0.f;
+    for (int k = 0; k < NOBJ; ++k) cand[k] = gload(obj + (size_t)ic * NOBJ + k);
+    const int32_t cs5 = finish ? gload(smax + ic) : 0;
+    if (finish && i >= N) cand[NOBJ - 1] = (float)cs5 / den_sl;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int t = (int)threadIdx.x + 256 * u, r = it.y + t / NOBJ;
+        if (t < ROWS * NOBJ)
+            rows[t] = r >= P ? 0.f : ((finish && t % NOBJ == NOBJ - 1 && r >= N) ? (float)sv[u] / den_sl : v[u]);
+    }
     // the workgroups of word 0 complete the stem records of their children
     // and count the CDS the children recomputed
     if (finish && word == 0) {
@@ -2065,24 +2097,38 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
                                                       const int2 *__restrict__ items) {
     const int2 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
+    const int lane = threadIdx.x & 63, R = J.R, P = J.P;
     const SelState *st = J.st;
-    if (st->n_cut) return;  // the whole job: no niching
-    const int lane = threadIdx.x & 63, R = J.R;
-    const int d = it.y + (int)(threadIdx.x >> 6);
-    if (d >= J.P) return;  // whole wave
-    const int fr = gload(J.front_of + d);
-    if (fr < 0) return;  // whole wave
-    const float *o = J.obj[pool_buffer(J)] + (size_t)d * NOBJ;
+    const float *const refpts = J.refpts;
+    // everything the wave reads before its first store, loaded together: the
+    // selection state, the design's front and objectives (at a clamped index)
+    const int n_cut = gload(&st->n_cut), last = gload(&st->last);
+    const uint32_t gen = gload(&st->gen);
+    float ideal[NOBJ], nadir[NOBJ];
+#pragma unroll
+    for (int k = 0; k < NOBJ; ++k) {
+        ideal[k] = gload(&st->ideal[k]);
+        nadir[k] = gload(&st->nadir[k]);
+    }
+    const int d = it.y + (int)(threadIdx.x >> 6), dc = min(d, P - 1);
+    // the row in 8-byte words (rows are 24 bytes), each feeding two objectives,
+    // so that the loads stay ahead of the per-objective branches
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const f32x2 *o2 = (const f32x2 *)(J.obj[(gen + 1u) & 1u] + (size_t)dc * NOBJ);
+    const f32x2 x01 = gload(o2), x23 = gload(o2 + 1), x45 = gload(o2 + 2);
+    const int fr = gload(J.front_of + dc);
+    const float x[NOBJ] = {x01.x, x01.y, x23.x, x23.y, x45.x, x45.y};
     float f[NOBJ];
 #pragma unroll
     for (int k = 0; k < NOBJ; ++k) {
-        const float den = st->nadir[k] - st->ideal[k];
-        f[k] = (den == 0.f) ? 0.f : (o[k] - st->ideal[k]) / den;
+        const float den = nadir[k] - ideal[k];
+        f[k] = (den == 0.f) ? 0.f : (x[k] - ideal[k]) / den;
     }
+    if (n_cut || d >= P || fr < 0) return;  // no niching, or the whole wave has no ranked design
     float best = FLT_MAX;
     int br = 0;
     for (int r = lane; r < R; r += 64) {
-        const float dd = perp_dist(f, J.refpts + (size_t)r * NOBJ);
+        const float dd = perp_dist(f, refpts + (size_t)r * NOBJ);
         if (dd < best) {
             best = dd;
             br = r;
@@ -2094,12 +2140,12 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
         J.ref_of[d] = br;
         J.dist[d] = best;
         const NicheView g = niche_view(J.niche, R);
-        if (fr < st->last) {
+        if (fr < last) {
             atomicAdd(&g.cr[br].x, 1);
         } else {
             atomicAdd(&g.cr[br].y, 1);
             atomicMin(g.nearest + br, asf_key(best, d));
-            J.prio[d] = (uint32_t)(niche_prio(J, st->gen, d) >> 32);
+            J.prio[d] = (uint32_t)(niche_prio(J, gen, d) >> 32);
         }
     }
 }
