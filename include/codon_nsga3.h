@@ -138,7 +138,7 @@ int cd_nds(const float *obj, int32_t P, int32_t N, int32_t *front_of, int32_t *n
  * same sort over device buffers -- obj [P][6] in, front_of [P] and
  * n_fronts [1] out -- enqueued on `stream` (a hipStream_t; NULL for the null
  * stream) without waiting.  It calls no allocator: its scratch is a
- * fixed-size device array (about 8.3 MiB) that every GPU the library runs on
+ * fixed-size device array (about 8.2 MiB) that every GPU the library runs on
  * holds until the process ends, so P is at most CD_NOALLOC_MAX_POOL, and
  * calls that overlap on one GPU (other streams or threads) race on it;
  * calls in one stream are ordered. */

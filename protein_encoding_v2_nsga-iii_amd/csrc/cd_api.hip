@@ -192,17 +192,20 @@ Layout layout(int L, int m, int N, int P, int R, int W, int n_dom, int n_asc, bo
 }
 
 // Room for the work lists of a job of pool P in any shape (the smallest
-// tile and group give the most items).
+// group gives the most items; the domination tiles are the same in every
+// shape).
 void work_counts(int P, int W, int *n_dom, int *n_asc) {
-    *n_dom = ((P + cd::DOM_ROWS - 1) / cd::DOM_ROWS) * W;
+    (void)P;
+    *n_dom = W * (W + 1) / 2;
     *n_asc = (P + cd::ASSOC_DESIGNS - 1) / cd::ASSOC_DESIGNS;
 }
 
 // Domination tiles and association groups of one job (job index j) in the
-// given shapes.
+// given shapes.  The tiles are the 64 x 64 blocks (row block b, column block
+// w) with b <= w; k_dominance takes dom_rows / 64 of them per workgroup.
 void job_lists(int P, int W, const cd::Shapes &sh, std::vector<int4> &dom, std::vector<int2> &asc, int j = 0) {
-    for (int t = 0; t < (P + sh.dom_rows - 1) / sh.dom_rows; ++t)
-        for (int w = 0; w < W; ++w) dom.push_back(make_int4(j, t * sh.dom_rows, w, 0));
+    for (int b = 0; b < W; ++b)
+        for (int w = b; w < W; ++w) dom.push_back(make_int4(j, 64 * b, w, 0));
     for (int t = 0; t < (P + sh.assoc_designs - 1) / sh.assoc_designs; ++t)
         asc.push_back(make_int2(j, t * sh.assoc_designs));
 }

@@ -128,6 +128,7 @@ __device__ __forceinline__ void gstore(T *p, T v) {
 // 16-byte words (a native 4 x u32 vector, which the global address space
 // takes as it is; HIP's vector class does not bind through that cast).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // Workgroup barrier that orders LDS only.  __syncthreads also waits for the
 // workgroup's outstanding global stores; code whose threads never read what

@@ -1509,18 +1509,6 @@ __global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int pool_buffer(const DevJob &J) { return (int)((J.st->gen + 1u) & 1u); }
 
-// dominate(), sorting.cuh:94-130: a dominates b
-__device__ __forceinline__ bool dominates(const float *a, const float *b) {
-    bool any = false, bad = false;
-#pragma unroll
-    for (int k = 0; k < NOBJ; ++k) {
-        const bool sig = fabsf(a[k] - b[k]) > F_PRECISION;
-        any |= sig;
-        bad |= sig && (a[k] > b[k]);
-    }
-    return any && !bad;
-}
-
 // The pool's extrema over rows row0 .. row0 + rows - 1 (updateIdealValue /
 // updateWorstValue, sorting.cuh:132-426; fminf / fmaxf from FLT_MAX /
 // -FLT_MAX), folded into the job's accumulators with one atomic minimum and
@@ -1553,84 +1541,136 @@ __device__ void pool_extrema(const DevJob &J, int row0, int rows, const float *v
         }
 }
 
-// Bit i of word w of row j (i = 64w + bit) is set when design i dominates
-// design j; the matrix is stored word-major, dom[w * P + j], so that the
-// front peeling reads one word of consecutive rows per wave.  np[j] counts
-// the dominators.  item = (job, first row, word, 0): ROWS rows of one word,
-// ROWS/4 per wave.  Lane b holds candidate 64w + b; a row's word is the
-// ballot of the 64 tests against that row (row values read from LDS as
-// broadcasts), and the wave stores its words and adds their counts at once.
+// Bit t of word w of row j is set when design 64w + t dominates design j;
+// the matrix is stored word-major, dom[w * P + j], so that the front peeling
+// reads one word of consecutive rows per wave.  np[j] counts the dominators.
+//
+// dominate() (sorting.cuh:94-130) compares |a - b| against the precision
+// and the sign of a - b;
+// with d = a - b per objective, a dominates b exactly when no d exceeds the
+// precision and some d is below minus the precision, and b dominates a
+// exactly when the reverse holds (b - a is the negation of a - b, bit for
+// bit).  One difference per objective therefore decides both directions,
+// and the pool is split into 64 x 64 tiles of (row block b, column block w)
+// with b <= w only: item = (job, 64b, w, 0).  Lane t holds column design
+// 64w + t.  A tile writes the words of block w of its rows (the ballot of
+// its column designs against each row) and, off the diagonal, the words of
+// block b of its column designs (each lane gathers one bit per row).  ROWS/64
+// tiles per workgroup of four waves, ROWS/4 rows of a tile per wave; the
+// partial column words of a tile's waves are combined in LDS.  The diagonal
+// tile of a row block also finishes its children's stem records (offspring
+// generation) and folds its rows into the pool's extrema.
 template <int ROWS>
 __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jobs, const int4 *__restrict__ items,
-                                                   bool finish, int32_t *stem_ctl, int ctl_n) {
-    static_assert(ROWS % 4 == 0 && ROWS / 4 <= 64, "a wave's rows fit one word per lane");
-    constexpr int WR = ROWS / 4;  // rows per wave
-    __shared__ float rows[ROWS * NOBJ];
+                                                   int n_items, bool finish, int32_t *stem_ctl, int ctl_n) {
+    static_assert(ROWS == 64 || ROWS == 128 || ROWS == 256, "one, two or four tiles per workgroup");
+    constexpr int TPW = ROWS / 64;  // tiles per workgroup
+    constexpr int G = 4 / TPW;      // waves per tile
+    constexpr int WR = 64 / G;      // rows of a tile per wave
+    __shared__ float rows[TPW][64 * NOBJ];
+    __shared__ uint64_t part[TPW][G][64];
+    __shared__ uint64_t amask[4][WR];  // per wave: the word of block wc of each of its rows
     if (stem_ctl && blockIdx.x == 0)  // k_stem_list has read the counts
         for (int g = threadIdx.x; g < ctl_n; g += blockDim.x) gstore(stem_ctl + g, 0);
-    const int4 it = items[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), tl = wv / G, sub = wv % G;
+    const int idx = (int)blockIdx.x * TPW + tl;
+    const bool have = idx < n_items;  // (the last workgroup may hold fewer tiles)
+    const int4 it = items[min(idx, n_items - 1)];
     const DevJob &J = jobs[it.x];
-    const int P = J.P, N = J.N, word = it.z;
+    const int P = J.P, N = J.N, r0 = it.y, wc = it.z, c0 = 64 * wc;
+    const bool diag = r0 == c0;
     const uint32_t gen = __builtin_amdgcn_readfirstlane(gload(&J.st->gen));
     const int buf = (int)((gen + 1u) & 1u);
     const float *const obj = J.obj[buf];
     const int32_t *const smax = J.stem_max;
     const float den_sl = (float)J.den_sl;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // in the offspring generation a child's objective 5 is its longest stem
-    // over (3L-4)/2, the value stem_finish stores.  Every load is issued
-    // first, at a clamped index, then the values are chosen.
-    constexpr int PER = (ROWS * NOBJ + 255) / 256;  // row values per thread
-    float v[PER];
-    int32_t sv[PER];
+    // In the offspring generation a child's objective 5 is its longest stem
+    // over (3L-4)/2, the value stem_finish stores.  The column design's row
+    // and the tile row of the lane are loaded at clamped indices, all before
+    // any value is used (the first wave of a tile stages the tile rows).
+    const int i = c0 + lane, ic = min(i, P - 1), rr = r0 + lane, rc = min(rr, P - 1);
+    float cand[NOBJ], wrow[NOBJ];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int t = min((int)threadIdx.x + 256 * u, ROWS * NOBJ - 1), r = min(it.y + t / NOBJ, P - 1);
-        v[u] = gload(obj + (size_t)r * NOBJ + t % NOBJ);
-        sv[u] = finish ? gload(smax + r) : 0;  // (a sort alone has no stem_max)
+    for (int k = 0; k < NOBJ; ++k) {
+        cand[k] = gload(obj + (size_t)ic * NOBJ + k);
+        wrow[k] = gload(obj + (size_t)rc * NOBJ + k);
     }
-    const int i = word * 64 + lane, ic = min(i, P - 1);
-    float cand[NOBJ];
+    const int32_t csm = finish ? gload(smax + ic) : 0;  // (a sort alone has no stem maxima)
+    const int32_t rsm = finish ? gload(smax + rc) : 0;
+    if (finish && i >= N) cand[NOBJ - 1] = (float)csm / den_sl;
+    if (finish && rr >= N) wrow[NOBJ - 1] = (float)rsm / den_sl;
+    const bool cv = i < P;
+    if (!cv)
 #pragma unroll
-    for (int k = 0; k < NOBJ; ++k) cand[k] = gload(obj + (size_t)ic * NOBJ + k);
-    const int32_t cs5 = finish ? gload(smax + ic) : 0;
-    if (finish && i >= N) cand[NOBJ - 1] = (float)cs5 / den_sl;
+        for (int k = 0; k < NOBJ; ++k) cand[k] = __builtin_inff();
+    if (sub == 0) {
+        // rows past the pool hold +inf: no design is dominated by them
+        // (d = -inf or NaN in every objective), and their words are not
+        // stored
+        float *const o = rows[tl] + lane * NOBJ;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int t = (int)threadIdx.x + 256 * u, r = it.y + t / NOBJ;
-        if (t < ROWS * NOBJ)
-            rows[t] = r >= P ? 0.f : ((finish && t % NOBJ == NOBJ - 1 && r >= N) ? (float)sv[u] / den_sl : v[u]);
+        for (int k = 0; k < NOBJ; ++k) o[k] = rr < P ? wrow[k] : __builtin_inff();
     }
-    // the workgroups of word 0 complete the stem records of their children
-    // and count the CDS the children recomputed
-    if (finish && word == 0) {
+    // the diagonal tiles complete the stem records of their children and
+    // count the CDS the children recomputed
+    if (finish && diag && sub == 0) {
         int changed = 0;
-        if (threadIdx.x < ROWS) {
-            const int r = it.y + (int)threadIdx.x;
-            if (r >= N && r < P) changed = stem_finish(J, buf, r);
-        }
+        if (rr >= N && rr < P && have) changed = stem_finish(J, buf, rr);
         changed = wave_sum_i32(changed);
-        if ((threadIdx.x & 63) == 0 && changed) atomicAdd(&J.st->cds_scanned, (unsigned long long)changed);
+        if (lane == 0 && changed) atomicAdd(&J.st->cds_scanned, (unsigned long long)changed);
     }
     __syncthreads();
-    // and fold their rows into the pool's extrema (one workgroup per row block)
-    if (word == 0 && wv == 0) pool_extrema(J, it.y, ROWS, rows);
-    const int r0 = it.y + wv * WR;
-    uint64_t mine = 0;  // lane q < WR: the word of row r0 + q
-#pragma unroll 4
-    for (int q = 0; q < WR; ++q) {
-        const int r = r0 + q;
-        const float *row = rows + (wv * WR + q) * NOBJ;
-        // the test runs in every lane (no branch around the row's LDS
-        // reads, so that the unrolled rows' reads issue together)
-        const bool dom = dominates(cand, row);
-        const uint64_t bits = __ballot(dom && i < P && i != r);
-        if (lane == q) mine = bits;
+    // and fold their rows into the pool's extrema
+    if (diag && have && sub == G - 1) pool_extrema(J, r0, 64, rows[tl]);
+    const float *const rw = rows[tl] + sub * WR * NOBJ;
+    uint64_t col = 0;  // bit q: row r0 + sub*WR + q dominates design i
+#pragma unroll 1
+    for (int q0 = 0; q0 < WR; q0 += 16) {  // 16 rows at a time, last row first
+        uint32_t c16 = 0;
+        uint64_t am[16];  // (wave-uniform: scalar registers)
+#pragma unroll
+        for (int u = 15; u >= 0; --u) {
+            const float *const row = rw + (q0 + u) * NOBJ;
+            float hi = cand[0] - row[0], lo = hi;  // (a NaN difference is never significant)
+#pragma unroll
+            for (int k = 1; k < NOBJ; ++k) {
+                const float d = cand[k] - row[k];
+                hi = fmaxf(hi, d);
+                lo = fminf(lo, d);
+            }
+            // the lanes whose design has a difference above the precision
+            // and those with one below minus the precision (wave-uniform
+            // masks); design i dominates the row when only the second holds
+            // (the lanes past the pool hold +inf, so the first), and the row
+            // dominates design i when only the first holds
+            const uint64_t U = __builtin_amdgcn_ballot_w64(hi > F_PRECISION);
+            const uint64_t D = __builtin_amdgcn_ballot_w64(lo < -F_PRECISION);
+            am[u] = D & ~U;
+            c16 = c16 + c16 + (uint32_t)__builtin_amdgcn_inverse_ballot_w64(U & ~D);
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < 16; ++u) amask[wv][q0 + u] = am[u];
+        col |= (uint64_t)c16 << q0;
     }
-    if (lane < WR && r0 + lane < P) {
-        const int r = r0 + lane;
-        gstore(J.dom + (size_t)word * P + r, mine);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t mine = amask[wv][min(lane, WR - 1)];  // lane q < WR: the word of row r0 + sub*WR + q
+    const int r = r0 + sub * WR + lane;
+    if (have && lane < WR && r < P) {
+        gstore(J.dom + (size_t)wc * P + r, mine);
         if (mine) atomicAdd(&J.np[r], (int)__popcll(mine));
+    }
+    if (G > 1) {
+        part[tl][sub][lane] = col << (sub * WR);
+        lds_barrier();
+        if (sub == 0)
+#pragma unroll
+            for (int g = 1; g < G; ++g) col |= part[tl][g][lane];
+    }
+    if (!diag && have && sub == 0 && cv) {
+        gstore(J.dom + (size_t)(r0 >> 6) * P + i, col);
+        if (col) atomicAdd(&J.np[i], (int)__popcll(col));
     }
 }
 
@@ -2113,7 +2153,6 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
     const int d = it.y + (int)(threadIdx.x >> 6), dc = min(d, P - 1);
     // the row in 8-byte words (rows are 24 bytes), each feeding two objectives,
     // so that the loads stay ahead of the per-objective branches
-    typedef float f32x2 __attribute__((ext_vector_type(2)));
     const f32x2 *o2 = (const f32x2 *)(J.obj[(gen + 1u) & 1u] + (size_t)dc * NOBJ);
     const f32x2 x01 = gload(o2), x23 = gload(o2 + 1), x45 = gload(o2 + 2);
     const int fr = gload(J.front_of + dc);
@@ -2446,10 +2485,11 @@ hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, i
 hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool finish, int32_t *stem_ctl, int ctl_n, int rows,
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    const int g = (n + rows / 64 - 1) / (rows / 64);
     switch (rows) {
-    case 64: hipLaunchKernelGGL(k_dominance<64>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl, ctl_n); break;
-    case 128: hipLaunchKernelGGL(k_dominance<128>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl, ctl_n); break;
-    case 256: hipLaunchKernelGGL(k_dominance<256>, dim3(n), dim3(256), 0, s, jobs, items, finish, stem_ctl, ctl_n); break;
+    case 64: hipLaunchKernelGGL(k_dominance<64>, dim3(g), dim3(256), 0, s, jobs, items, n, finish, stem_ctl, ctl_n); break;
+    case 128: hipLaunchKernelGGL(k_dominance<128>, dim3(g), dim3(256), 0, s, jobs, items, n, finish, stem_ctl, ctl_n); break;
+    case 256: hipLaunchKernelGGL(k_dominance<256>, dim3(g), dim3(256), 0, s, jobs, items, n, finish, stem_ctl, ctl_n); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

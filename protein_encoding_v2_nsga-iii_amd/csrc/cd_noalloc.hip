@@ -21,7 +21,7 @@ namespace {
 
 constexpr int MAX_POOL = CD_NOALLOC_MAX_POOL;
 constexpr int MAX_WORDS = (MAX_POOL + 63) / 64;
-constexpr int MAX_TILES = ((MAX_POOL + cd::DOM_ROWS - 1) / cd::DOM_ROWS) * MAX_WORDS;
+constexpr int MAX_TILES = MAX_WORDS * (MAX_WORDS + 1) / 2;  // 64 x 64 blocks (b, w), b <= w
 
 constexpr size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t OFF_JOB = 0;
@@ -34,7 +34,7 @@ constexpr size_t OFF_RANKED = OFF_ACC + a256(sizeof(cd::SelAcc));
 constexpr size_t SCRATCH_BYTES = OFF_RANKED + a256((size_t)MAX_WORDS * 8);
 static_assert(sizeof(cd::SelState) % 4 == 0, "selection state is cleared in words");
 
-// The scratch of cd_nds_noalloc on every GPU (about 8.4 MiB).  Host code uses
+// The scratch of cd_nds_noalloc on every GPU (about 8.2 MiB).  Host code uses
 // the address hipGetSymbolAddress returns for the current device, never this
 // name.
 static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
@@ -45,8 +45,11 @@ __global__ __launch_bounds__(1024) void k_noalloc_setup(cd::DevJob h, cd::DevJob
                                                         int words) {
     for (int i = threadIdx.x; i < h.P; i += blockDim.x) h.np[i] = 0;
     if (threadIdx.x < cd::ACC_WORDS) ((uint32_t *)h.acc)[threadIdx.x] = cd::acc_clear_word(threadIdx.x);
-    for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
-        tiles[t] = make_int4(0, (t / words) * cd::DOM_ROWS, t % words, 0);
+    for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) {  // row block b holds words - b tiles
+        int b = 0, first = 0;
+        while (first + words - b <= t) first += words - b++;
+        tiles[t] = make_int4(0, 64 * b, b + t - first, 0);
+    }
     uint32_t *st = (uint32_t *)h.st;
     for (int i = threadIdx.x; i < (int)(sizeof(cd::SelState) / 4); i += blockDim.x) st[i] = 0;
     if (threadIdx.x == 0) *dj = h;
@@ -91,7 +94,7 @@ extern "C" int cd_nds_noalloc(const float *obj, int32_t P, int32_t N, int32_t *f
     if (rc != CD_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
     const int W = (P + 63) / 64;
-    const int n_tiles = ((P + cd::DOM_ROWS - 1) / cd::DOM_ROWS) * W;
+    const int n_tiles = W * (W + 1) / 2;
     cd::DevJob h{};
     h.N = N;
     h.P = P;
