@@ -532,15 +532,40 @@ using cdhost::enqueue_generation;
 
 namespace {
 
-int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w) {
+int capture(hipStream_t s, hipGraph_t *g, hipGraphExec_t *ge, const Launch &w, int generations) {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_generation(w, s, nullptr);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < generations && e == hipSuccess; ++k) e = enqueue_generation(w, s, nullptr);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &graph);
-    if (e != hipSuccess) return fail(CD_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(CD_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    }
     if (e2 != hipSuccess) return fail(CD_E_DEVICE, std::string("end capture: ") + hipGetErrorString(e2));
-    HIPCHK(hipGraphInstantiate(ge, graph, nullptr, nullptr, 0));
+    e = hipGraphInstantiate(ge, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(CD_E_DEVICE, std::string("graph instantiation: ") + hipGetErrorString(e));
+    }
     *g = graph;
+    return CD_OK;
+}
+
+// Enqueue `generations` generations as graph launches: blocks of
+// GRAPH_BLOCK, then single generations (each graph captured on first use).
+int step_graphs(StepGraphs &G, hipStream_t s, const Launch &w, int generations) {
+    int g = generations;
+    if (g >= GRAPH_BLOCK && !G.exec[1]) {
+        const int rc = capture(s, &G.graph[1], &G.exec[1], w, GRAPH_BLOCK);
+        if (rc != CD_OK) return rc;
+    }
+    for (; g >= GRAPH_BLOCK; g -= GRAPH_BLOCK) HIPCHK(hipGraphLaunch(G.exec[1], s));
+    if (g > 0 && !G.exec[0]) {
+        const int rc = capture(s, &G.graph[0], &G.exec[0], w, 1);
+        if (rc != CD_OK) return rc;
+    }
+    for (; g > 0; --g) HIPCHK(hipGraphLaunch(G.exec[0], s));
     return CD_OK;
 }
 
@@ -718,12 +743,7 @@ int cd_job_step(cd_job *J, int32_t generations) {
     if (!J->inited) return fail(CD_E_STATE, "cd_job_step before cd_job_init");
     if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: advance it through the batch");
     if (generations < 0) return fail(CD_E_INVALID, "negative generation count");
-    if (!J->gexec) {
-        int rc = capture(J->stream, &J->graph, &J->gexec, job_launch(J));
-        if (rc != CD_OK) return rc;
-    }
-    for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(J->gexec, J->stream));
-    return CD_OK;
+    return step_graphs(J->graphs, J->stream, job_launch(J), generations);
 }
 
 int cd_job_step_timed(cd_job *J, int32_t generations, float *ms) {
@@ -835,8 +855,7 @@ int cd_job_destroy(cd_job *J) {
     if (!J) return CD_OK;
     if (!J->batches.empty()) return fail(CD_E_STATE, "the job is held by a batch: destroy the batch first");
     if (J->stream) (void)hipStreamSynchronize(J->stream);
-    if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
-    if (J->graph) (void)hipGraphDestroy(J->graph);
+    J->graphs.reset();
     if (J->mem) (void)hipFreeAsync(J->mem, J->stream);  // back to the pool, after the job's last work
     if (J->stream) {
         (void)hipStreamSynchronize(J->stream);
@@ -1038,12 +1057,8 @@ int cd_batch_profile(cd_batch *B, int32_t generations, float *kernel_ms) {
 
 int cd_batch_step(cd_batch *B, int32_t generations) {
     if (!B) return fail(CD_E_INVALID, "null batch");
-    if (!B->gexec) {
-        int rc = capture(B->stream, &B->graph, &B->gexec, batch_launch(B));
-        if (rc != CD_OK) return rc;
-    }
-    for (int g = 0; g < generations; ++g) HIPCHK(hipGraphLaunch(B->gexec, B->stream));
-    return CD_OK;
+    if (generations < 0) return fail(CD_E_INVALID, "negative generation count");
+    return step_graphs(B->graphs, B->stream, batch_launch(B), generations);
 }
 
 int cd_batch_sync(cd_batch *B) {
@@ -1084,10 +1099,7 @@ static int batch_apply_shapes(cd_batch *B, const cd::Shapes &sh) {
     B->n_small = wl.n_small;
     B->lds_small = wl.lds_small;
     B->shapes = sh;
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->gexec = nullptr;
-    B->graph = nullptr;
+    B->graphs.reset();
     return CD_OK;
 }
 
@@ -1101,10 +1113,7 @@ static int job_apply_shapes(cd_job *J, const cd::Shapes &sh) {
     J->n_dom = (int)dom.size();
     J->n_asc = (int)asc.size();
     J->shapes = sh;
-    if (J->gexec) (void)hipGraphExecDestroy(J->gexec);
-    if (J->graph) (void)hipGraphDestroy(J->graph);
-    J->gexec = nullptr;
-    J->graph = nullptr;
+    J->graphs.reset();
     return CD_OK;
 }
 
@@ -1190,8 +1199,7 @@ int cd_batch_destroy(cd_batch *B) {
     if (!B) return CD_OK;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     for (cd_job *J : B->jobs) J->batches.erase(std::remove(J->batches.begin(), J->batches.end(), B), J->batches.end());
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->graphs.reset();
     if (B->stream) (void)hipStreamDestroy(B->stream);
     if (B->d_jobs) (void)hipFree(B->d_jobs);
     if (B->d_off) (void)hipFree(B->d_off);

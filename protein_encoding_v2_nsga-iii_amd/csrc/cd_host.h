@@ -14,6 +14,24 @@
 struct cd_batch;
 
 // One job: tables, pools and selection state in one device allocation.
+// The captured generation graphs of a job or batch: [0] one generation,
+// [1] GRAPH_BLOCK generations back to back.  A long step launches the block
+// graph (consecutive graph launches leave several microseconds between
+// them; the kernels inside one graph follow each other closely).
+constexpr int GRAPH_BLOCK = 8;
+struct StepGraphs {
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    void reset() {
+        for (int k = 0; k < 2; ++k) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+    }
+};
+
 struct cd_job {
     // live batches that hold the job: while any does, the job is advanced
     // only through them (cd_job_step and cd_job_destroy fail) and reads wait
@@ -34,8 +52,7 @@ struct cd_job {
     cd::Shapes shapes = cd::SHAPES_JOB;  // launch shapes of its generations (the lists above follow them)
     size_t lds = 0, stem_lds = 0;
     int stem_parts = 1;  // k_stem workgroups per CDS (initialisation and evaluation)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    StepGraphs graphs;
     bool inited = false;
     bool selection_only = false;
 };
@@ -58,8 +75,7 @@ struct cd_batch {
     int mg = 0;          // largest m of the batch's jobs
     int stem_parts = 0;  // largest k_stem workgroups per CDS of the batch's jobs
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    StepGraphs graphs;
 };
 
 namespace cdhost {

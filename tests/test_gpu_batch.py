@@ -121,3 +121,58 @@ def test_cds_scanned_counts(cdp):
     b.close()
     for j in jobs:
         j.close()
+
+
+KEYS = ("genome", "obj", "idx", "pql", "sorted", "front_of", "extremes", "nadir")
+
+
+def test_block_graphs_equal_single_generations(cdp, orc):
+    """Steps of eight generations or more launch a graph that holds eight
+    generations back to back.  Each replay gives exactly the pools of
+    single-generation graph launches and of eager launches (profile), read
+    back between replays, and the oracle agrees after all of them."""
+    print("block graphs: jobs", flush=True)
+    specs = make_jobs(cdp, [SPECS[1], SPECS[3]])
+    total = 0
+    for si, s in enumerate(specs):
+        blk, one, eager = (cdp.Job(*s).init() for _ in range(3))
+        total = 0
+        for G in (8, 8, 11, 16, 3):
+            blk.step(G)
+            for _ in range(G):
+                one.step(1)
+            eager.profile(G)
+            total += G
+            rb, r1, re = blk.read(), one.read(), eager.read()
+            for k in KEYS:
+                assert np.array_equal(rb[k], r1[k]), (si, total, k)
+                assert np.array_equal(rb[k], re[k]), (si, total, k)
+        if si == 0:
+            o = orc.Job(*s)
+            o.init()
+            o.step(total)
+            ro = o.read()
+            for k in ("genome", "obj", "sorted", "front_of"):
+                assert np.array_equal(blk.read()[k], ro[k]), k
+        for j in (blk, one, eager):
+            j.close()
+    print("block graphs: batch", flush=True)
+    specs = make_jobs(cdp, SPECS)
+    alone = []
+    for s in specs:
+        j = cdp.Job(*s).init()
+        for _ in range(19):
+            j.step(1)
+        alone.append(j.read())
+        j.close()
+    jobs = [cdp.Job(*s).init() for s in specs]
+    b = cdp.Batch(jobs)
+    b.step(19)
+    b.sync()
+    for i, (j, a) in enumerate(zip(jobs, alone)):
+        r = j.read()
+        for k in KEYS:
+            assert np.array_equal(r[k], a[k]), (i, k)
+    b.close()
+    for j in jobs:
+        j.close()
