@@ -44,7 +44,7 @@ struct cd_job {
     size_t mem_bytes = 0;
     hipStream_t stream = nullptr;
     int2 *d_off = nullptr;
-    int4 *d_dom = nullptr;        // domination tiles, room for the smallest tile (64 rows)
+    int4 *d_dom = nullptr;        // domination tiles: the 64 x 64 blocks (b, w), b <= w
     int2 *d_asc = nullptr;        // association groups, room for the smallest group (4 designs)
     uint32_t *d_slist = nullptr;  // changed CDS of a generation (stem_list_entries) for k_stem_list
     int32_t *d_sctl = nullptr;    // its counts (STEM_MAX_GROUPS), zero between generations (k_dominance clears them)
@@ -62,7 +62,7 @@ struct cd_batch {
     cd::Shapes shapes = cd::SHAPES_BATCH;  // launch shapes of its ticks (the lists below follow them)
     cd::DevJob *d_jobs = nullptr;
     int2 *d_off = nullptr;
-    int4 *d_dom = nullptr;  // room for 64-row tiles
+    int4 *d_dom = nullptr;  // the domination tiles of every job
     int2 *d_asc = nullptr;  // room for 4-design groups
     uint32_t *d_slist = nullptr;  // changed CDS of a generation (stem_list_entries)
     int32_t *d_sctl = nullptr;
@@ -99,7 +99,7 @@ struct Launch {
     int njobs;
     const int2 *off;  // (job, child) of every child
     int n_off;
-    const int4 *dom;  // (job, first row, word, 0) domination tiles of sh.dom_rows rows
+    const int4 *dom;  // (job, 64b, w, 0) domination tiles, b <= w, taken sh.dom_rows / 64 per workgroup
     int n_dom;
     const int2 *asc;  // (job, first design) association groups of sh.assoc_designs designs
     int n_asc;

@@ -11,7 +11,7 @@ namespace cd {
 enum DesignMode { MODE_INIT = 0, MODE_EVAL = 1, MODE_OFFSPRING = 2 };
 
 constexpr size_t LDS_BYTES = 160 * 1024;  // per workgroup on gfx950
-constexpr int DOM_ROWS = 64;               // rows per k_dominance workgroup of the standalone sort and selection
+constexpr int DOM_ROWS = 64;               // tile rows per k_dominance workgroup of the standalone sort and selection
 constexpr int ASSOC_DESIGNS = 4;           // designs per k_associate workgroup (a wave each), the smallest shape
 constexpr int STEM_PARTS = 4;              // most k_stem workgroups per CDS (contiguous tile ranges)
 constexpr int STEM_CUS = 256;              // compute units of an MI355X: k_stem_list runs a resident grid
@@ -23,7 +23,7 @@ struct Shapes {
     int design_split;    // k_design LDS bytes up to which children launch apart from the rest (0: one launch)
     int design_threads;  // k_design workgroup of the other children (a lone job: all): 256 or 512 threads
     int stem_waves;      // k_stem_list waves per SIMD (workgroups per CU): 5, 6, 7 or 8
-    int dom_rows;        // rows per k_dominance workgroup: 64, 128 or 256 (a quarter per wave)
+    int dom_rows;        // k_dominance tile rows per workgroup (dom_rows / 64 tiles of 64 x 64): 64, 128 or 256
     int fronts_threads;  // k_fronts workgroup: 256, 512 or 1024 threads
     int assoc_designs;   // designs per k_associate workgroup, a wave each: 4, 8 or 16
     int niche_threads;   // k_niche workgroup: 256, 512 or 1024 threads
@@ -87,7 +87,7 @@ hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_
                             const int32_t *stem_ctl, StemList sl, int waves, size_t lds, hipStream_t s);
 // one thread per item: objective 5, clears the marks
 hipError_t launch_stem_finish(int mode, const DevJob *jobs, const int2 *items, int n, hipStream_t s);
-// items: (job, first row, word, 0), rows (64, 128 or 256) rows per item.  finish: the offspring generation, whose children's stem
+// items: (job, 64b, w, 0), the 64 x 64 tiles b <= w, rows / 64 (rows: 64, 128 or 256) per workgroup.  finish: the offspring generation, whose children's stem
 // objective (and stem records) k_dominance completes in place of
 // launch_stem_finish; stem_ctl (or null): the ctl_n changed-CDS counts of
 // that generation's k_design, cleared for the next one
