@@ -1992,6 +1992,17 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
         newm[w] = 0;
     }
     __syncthreads();
+    // candidate rows of the extremes (below): the current extreme points and
+    // the rows of the first front's ASF minima, loaded now so that the loads
+    // overlap the peeling
+    float crv = 0.f;
+    if (tid < 2 * NOBJ * NOBJ) {
+        const int c = tid / NOBJ, k = tid % NOBJ;
+        if (c < NOBJ)
+            crv = ext[c * NOBJ + k];
+        else if (pid[c - NOBJ] < P)
+            crv = gload(obj + (size_t)pid[c - NOBJ] * NOBJ + k);
+    }
     int f = 0, last_size = s_cnt;
     while (s_cnt < N) {
         ++f;
@@ -2049,14 +2060,8 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
             gstore(g.nearest + r, ~0ull);
         }
     }
-    // candidate rows of the extremes and their ASF values for every axis
-    if (tid < 2 * NOBJ * NOBJ) {
-        const int c = tid / NOBJ, k = tid % NOBJ;
-        if (c < NOBJ)
-            crow[c][k] = ext[c * NOBJ + k];
-        else
-            crow[c][k] = pid[c - NOBJ] < P ? gload(obj + (size_t)pid[c - NOBJ] * NOBJ + k) : 0.f;
-    }
+    // the candidate rows of the extremes and their ASF values for every axis
+    if (tid < 2 * NOBJ * NOBJ) crow[tid / NOBJ][tid % NOBJ] = crv;
     __syncthreads();
     if (tid < 2 * NOBJ * NOBJ) casf[tid / NOBJ][tid % NOBJ] = asf(crow[tid / NOBJ], ideal, tid % NOBJ);
     __syncthreads();
@@ -2290,7 +2295,13 @@ __global__ __launch_bounds__(1024) void k_place(const DevJob *__restrict__ jobs)
     const SelState *st = J.st;
     const int P = J.P, R = J.R, NT = blockDim.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int d0 = blockIdx.y * NT;
-    if (st->n_cut || d0 >= P) return;
+    if (d0 >= P) return;
+    // the design's own fields, issued with the state's at a clamped index
+    // (ref_of and prio are read only for designs of the last front)
+    const int d = d0 + tid, dc = min(d, P - 1);
+    const int fo = gload(J.front_of + dc), rf = gload(J.ref_of + dc);
+    const uint32_t pr = gload(J.prio + dc);
+    if (st->n_cut) return;
     const NicheView g = niche_view(J.niche, R);
     const bool staged = R <= NICHE_STAGE;
     __shared__ int2 s_cr[NICHE_STAGE];
@@ -2317,12 +2328,11 @@ __global__ __launch_bounds__(1024) void k_place(const DevJob *__restrict__ jobs)
         lvl0 = s_l0;
         coff = s_co;
     }
-    const int d = d0 + tid;
-    const bool mine = d < P && gload(J.front_of + d) == st->last;
+    const bool mine = d < P && fo == st->last;
     int r = 0, co = -1, loc = 0;
     bool sel = false;
     if (mine) {
-        r = gload(J.ref_of + d);
+        r = rf;
         const int l0 = lvl0[r];
         co = coff[r];
         sel = d == l0;
@@ -2339,7 +2349,7 @@ __global__ __launch_bounds__(1024) void k_place(const DevJob *__restrict__ jobs)
     }
     if (mine && co >= 0) {
         const int slot = co + (staged ? l_base[r] + loc : atomicAdd(g.fill + r, 1));
-        gstore(J.cutl + slot, ((uint64_t)gload(J.prio + d) << 32) | (uint32_t)d);
+        gstore(J.cutl + slot, ((uint64_t)pr << 32) | (uint32_t)d);
         gstore(J.cutr + slot, r);
     }
     const uint64_t b = __ballot(sel);  // a wave's 64 designs are one word of the mask
@@ -2365,24 +2375,23 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     const int ncut = J.acc->ncut;
     const NicheView g = niche_view(J.niche, R);
     __shared__ int s_a, s_b;
-    const int s = s0 + tid;
-    uint64_t my = 0;
-    int r = 0;
-    if (s < ncut) {
-        my = gload(J.cutl + s);
-        r = gload(J.cutr + s);
-    }
-    if (tid == 0) s_a = gload(g.coff + r);  // the first slot's bucket starts here
-    if (s == min(s0 + NT, ncut) - 1) s_b = gload(g.coff + r) + g.cr[r].y;
+    const int s = s0 + tid, sc = min(s, ncut - 1);
+    const uint64_t my = gload(J.cutl + sc);
+    const int r = gload(J.cutr + sc);
+    // the bucket's fields, loaded together
+    const int co = gload(g.coff + r), nr = gload((const int32_t *)(g.cr + r) + 1), l0 = gload(g.lvl0 + r),
+              kr = gload(g.kp + r);
+    if (tid == 0) s_a = co;  // the first slot's bucket starts here
+    if (s == min(s0 + NT, ncut) - 1) s_b = co + nr;
     __syncthreads();
     const int a = s_a, nk = s_b - s_a;
     for (int u = tid; u < nk; u += NT) keys[u] = gload(J.cutl + a + u);
     __syncthreads();
     if (s < ncut) {
-        const int d = (int)(uint32_t)my, l0 = gload(g.lvl0 + r);
+        const int d = (int)(uint32_t)my;
         if (d != l0) {
-            const int kk = gload(g.kp + r) - (l0 >= 0 ? 1 : 0);
-            const int off = gload(g.coff + r) - a, nr = g.cr[r].y;
+            const int kk = kr - (l0 >= 0 ? 1 : 0);
+            const int off = co - a;
             int rank = 0;
 #pragma unroll 4
             for (int u = 0; u < nr; ++u) rank += (int)(keys[off + u] < my);
