@@ -94,24 +94,6 @@ __device__ __forceinline__ void stem_unkey(uint64_t k, int &l, int &p, int &q) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Wave64 / workgroup reductions
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint64_t w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
 // Global-memory access through the global address space.  Pointers held in
 // the job descriptor are generic, and generic (flat) accesses also count
 // against the LDS wait counter, so an LDS-only barrier would still wait for
@@ -140,6 +122,9 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// ---------------------------------------------------------------------------
+// Wave64 / workgroup reductions
+// ---------------------------------------------------------------------------
 // Sums over a full wave, wave-uniform: DPP row shifts 1, 2, 4, 8 give each
 // lane the sum of its row up to itself (lane 15 the row's sum), the row
 // broadcasts of lanes 15 and 31 carry the sums across rows, and lane 63 ends
@@ -246,25 +231,24 @@ __device__ __forceinline__ int wave_max_nonneg(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
-// Sum of up to 4 int64 values over the workgroup; every thread gets the
-// totals.  red must hold 4*(blockDim/64) entries.
-template <int NV>
-__device__ inline void block_sum_i64(int64_t (&v)[NV], int64_t *red) {
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
-    __syncthreads();
-    if (lane == 0)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) red[i * 16 + wid] = v[i];
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        int64_t s = 0;
-        for (int w = 0; w < nw; ++w) s += red[i * 16 + w];
-        v[i] = s;
-    }
-    __syncthreads();
+// Largest 64-bit key over a full wave, wave-uniform, by the same DPP steps
+// (0, the identity, enters from outside a shifted or masked row: every key is
+// at least 0).  Every lane of the wave must be active.
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ uint64_t dpp_max_u64(uint64_t v) {
+    const uint64_t o = (uint64_t)dpp_i64<CTRL, ROWS, BOUND>((int64_t)v);
+    return o > v ? o : v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    v = dpp_max_u64<0x111, 0xf, true>(v);
+    v = dpp_max_u64<0x112, 0xf, true>(v);
+    v = dpp_max_u64<0x114, 0xf, true>(v);
+    v = dpp_max_u64<0x118, 0xf, true>(v);
+    v = dpp_max_u64<0x142, 0xa, false>(v);
+    v = dpp_max_u64<0x143, 0xc, false>(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ inline uint64_t block_max_u64(uint64_t v, uint64_t *red) {

@@ -1061,9 +1061,10 @@ __device__ int survivor_at(const uint64_t *selm, int W, int c) {
         const int incl = wave_scan_i32((int)__popcll(x));  // inclusive prefix of the words' counts
         const uint64_t hit = __ballot(base + incl > c);
         if (hit) {
-            const int q = (int)__builtin_ctzll(hit);
-            uint64_t y = ((uint64_t)(uint32_t)__shfl((int)(x >> 32), q, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)x, q, 64);
-            int k = c - (base + __shfl(incl, q, 64) - (int)__popcll(y));  // rank within the word
+            const int q = (int)__builtin_ctzll(hit);  // wave-uniform: read lane q directly
+            uint64_t y = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), q) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, q);
+            int k = c - (base + __builtin_amdgcn_readlane(incl, q) - (int)__popcll(y));  // rank within the word
             int pos = 0;
 #pragma unroll
             for (int h = 32; h > 0; h >>= 1) {  // the k-th set bit of y
@@ -2178,8 +2179,7 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
             br = r;
         }
     }
-#pragma unroll
-    for (int o2 = 32; o2 > 0; o2 >>= 1) argmin_pair(best, br, __shfl_xor(best, o2, 64), __shfl_xor(br, o2, 64));
+    wave_argmin(best, br);  // (every lane of the wave is here: one design a wave)
     if (lane == 0) {
         J.ref_of[d] = br;
         J.dist[d] = best;
