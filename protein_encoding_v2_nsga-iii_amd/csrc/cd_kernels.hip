@@ -1301,7 +1301,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     if (tid < 2 * NOBJ && tid != 10) gstore((dst ? oidx1 : oidx0) + (size_t)out_slot * 2 * NOBJ + tid, (int8_t)s_idx[tid]);
     if (tid < m) gstore_stat((dst ? cst1 : cst0) + (size_t)out_slot * m + tid, cst[tid]);
     if (tid < npair) gstore((dst ? hdp1 : hdp0) + (size_t)out_slot * npair + tid, sh.hd[tid]);
-    if (tid == 0) {  // CDS evaluated (k_stem scans the same ones); the unchanged CDS's longest stem
+    // CDS evaluated (k_stem scans the same ones); the unchanged CDS's longest
+    // stem.  The first lane of the last wave does this: a wave's wait for the
+    // list atomic's result also waits for the wave's earlier stores, and the
+    // rows, planes and records go to the first waves first.
+    if (tid == NT - 64) {
         unsigned long long nd = 0;
         int lmax = 0;
         for (int c = 0; c < m; ++c) {
