@@ -627,7 +627,7 @@ int tune_shapes(const cd::Shapes &start, int njobs, int gens, hipStream_t s, App
         {&cd::Shapes::stem_waves, {5, 6, 7, 8}, 0x2u},
         {&cd::Shapes::dom_rows, {64, 128, 256}, 0x4u},
         {&cd::Shapes::fronts_threads, {256, 512, 1024}, 0x8u},
-        {&cd::Shapes::assoc_designs, {4, 8, 16}, 0x10u},
+        {&cd::Shapes::assoc_designs, {4, 8, 16, 32, 64}, 0x10u},
         {&cd::Shapes::niche_threads, {256, 512, 1024}, 0x20u},
     };
     cd::Shapes cur = start;

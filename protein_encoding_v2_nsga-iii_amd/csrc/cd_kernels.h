@@ -25,7 +25,7 @@ struct Shapes {
     int stem_waves;      // k_stem_list waves per SIMD (workgroups per CU): 5, 6, 7 or 8
     int dom_rows;        // k_dominance tile rows per workgroup (dom_rows / 64 tiles of 64 x 64): 64, 128 or 256
     int fronts_threads;  // k_fronts workgroup: 256, 512 or 1024 threads
-    int assoc_designs;   // designs per k_associate workgroup, a wave each: 4, 8 or 16
+    int assoc_designs;   // designs per k_associate workgroup: 4, 8 or 16 (a wave each), 32 or 64 (eight lanes each)
     int niche_threads;   // k_niche workgroup: 256, 512 or 1024 threads
 };
 // Untuned shapes: a lone job, and a batch or served tick.
@@ -98,7 +98,7 @@ hipError_t launch_dominance(const DevJob *jobs, const int4 *items, int n, bool f
 // 1024) per job)
 constexpr int FRONT0_THREADS = 256;
 hipError_t launch_fronts(const DevJob *jobs, int njobs, int max_pool, int threads, hipStream_t s);
-// items: (job, first design), `designs` (4, 8 or 16) designs per item, a wave each
+// items: (job, first design), `designs` designs per item: 4, 8 or 16 (a wave each), 32 or 64 (eight lanes each)
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int designs, hipStream_t s);
 // k_levels (one workgroup per job) when some job has more reference points
 // than NICHE_STAGE (max_refs), then k_place and k_niche (grid: jobs x chunks

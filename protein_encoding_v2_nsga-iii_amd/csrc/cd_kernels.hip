@@ -2133,21 +2133,23 @@ __device__ __forceinline__ uint64_t niche_prio(const DevJob &J, uint32_t gen, in
     return ((uint64_t)r[0] << 32) | (uint32_t)d;
 }
 
-// Nearest reference line of every ranked design (sorting.cuh:1105-1180): one
-// wave per design, D designs a workgroup (item = (job, first design)); lanes
-// take reference lines r = lane, lane + 64, ... and the wave keeps the
-// smallest distance, ties to the lowest index (the first minimum of a sweep
-// in index order).  The niche arrays are folded in with atomics: designs of
-// the fronts before the last count toward their niche, those of the last
-// front toward its bucket, with the bucket's nearest design as a 64-bit
-// (distance, design) minimum.  A distance is never NaN or above FLT_MAX: the
-// sweep starts at FLT_MAX and takes only smaller values.
-template <int D>
-__global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__ jobs,
-                                                      const int2 *__restrict__ items) {
+// Nearest reference line of every ranked design (sorting.cuh:1105-1180):
+// LPD lanes per design (a wave, or eight lanes), D designs a workgroup (item
+// = (job, first design)); the lanes of a design take reference lines
+// r = sub, sub + LPD, ... and keep the smallest distance, ties to the lowest
+// index (the first minimum of a sweep in index order).  The niche arrays are
+// folded in with atomics: designs of the fronts before the last count toward
+// their niche, those of the last front toward its bucket, with the bucket's
+// nearest design as a 64-bit (distance, design) minimum.  A distance is never
+// NaN or above FLT_MAX: the sweep starts at FLT_MAX and takes only smaller
+// values.
+template <int D, int LPD>
+__global__ __launch_bounds__(D * LPD) void k_associate(const DevJob *__restrict__ jobs,
+                                                       const int2 *__restrict__ items) {
+    static_assert(LPD == 64 || LPD == 8, "a wave or eight lanes per design");
     const int2 it = items[blockIdx.x];
     const DevJob &J = jobs[it.x];
-    const int lane = threadIdx.x & 63, R = J.R, P = J.P;
+    const int sub = (int)threadIdx.x % LPD, R = J.R, P = J.P;
     const SelState *st = J.st;
     const float *const refpts = J.refpts;
     // everything the wave reads before its first store, loaded together: the
@@ -2160,7 +2162,7 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
         ideal[k] = gload(&st->ideal[k]);
         nadir[k] = gload(&st->nadir[k]);
     }
-    const int d = it.y + (int)(threadIdx.x >> 6), dc = min(d, P - 1);
+    const int d = it.y + (int)threadIdx.x / LPD, dc = min(d, P - 1);
     // the row in 8-byte words (rows are 24 bytes), each feeding two objectives,
     // so that the loads stay ahead of the per-objective branches
     const f32x2 *o2 = (const f32x2 *)(J.obj[(gen + 1u) & 1u] + (size_t)dc * NOBJ);
@@ -2173,18 +2175,31 @@ __global__ __launch_bounds__(64 * D) void k_associate(const DevJob *__restrict__
         const float den = nadir[k] - ideal[k];
         f[k] = (den == 0.f) ? 0.f : (x[k] - ideal[k]) / den;
     }
-    if (n_cut || d >= P || fr < 0) return;  // no niching, or the whole wave has no ranked design
+    if (n_cut || d >= P || fr < 0) return;  // no niching, or no ranked design (all the design's lanes)
     float best = FLT_MAX;
     int br = 0;
-    for (int r = lane; r < R; r += 64) {
+    for (int r = sub; r < R; r += LPD) {
         const float dd = perp_dist(f, refpts + (size_t)r * NOBJ);
         if (dd < best) {
             best = dd;
             br = r;
         }
     }
-    wave_argmin(best, br);  // (every lane of the wave is here: one design a wave)
-    if (lane == 0) {
+    bool writer;
+    if (LPD == 64) {
+        wave_argmin(best, br);  // (every lane of the wave is here: one design a wave)
+        writer = sub == 0;
+    } else {
+        // the design's eight lanes (two quads of a DPP row): the quads by
+        // quad permutations, then the first quad into the second by a row
+        // shift, whose last lane ends with the design's minimum (the lanes
+        // read only their own design's lanes, which are all here)
+        argmin_dpp<0xB1, 0xf>(best, br);   // quad_perm [1,0,3,2]
+        argmin_dpp<0x4E, 0xf>(best, br);   // quad_perm [2,3,0,1]
+        argmin_dpp<0x114, 0xf>(best, br);  // row_shr:4
+        writer = sub == 7;
+    }
+    if (writer) {
         J.ref_of[d] = br;
         J.dist[d] = best;
         const NicheView g = niche_view(J.niche, R);
@@ -2519,9 +2534,11 @@ hipError_t launch_fronts(const DevJob *jobs, int njobs, int max_pool, int thread
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int designs, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     switch (designs) {
-    case 4: hipLaunchKernelGGL(k_associate<4>, dim3(n), dim3(256), 0, s, jobs, items); break;
-    case 8: hipLaunchKernelGGL(k_associate<8>, dim3(n), dim3(512), 0, s, jobs, items); break;
-    case 16: hipLaunchKernelGGL(k_associate<16>, dim3(n), dim3(1024), 0, s, jobs, items); break;
+    case 4: hipLaunchKernelGGL((k_associate<4, 64>), dim3(n), dim3(256), 0, s, jobs, items); break;
+    case 8: hipLaunchKernelGGL((k_associate<8, 64>), dim3(n), dim3(512), 0, s, jobs, items); break;
+    case 16: hipLaunchKernelGGL((k_associate<16, 64>), dim3(n), dim3(1024), 0, s, jobs, items); break;
+    case 32: hipLaunchKernelGGL((k_associate<32, 8>), dim3(n), dim3(256), 0, s, jobs, items); break;
+    case 64: hipLaunchKernelGGL((k_associate<64, 8>), dim3(n), dim3(512), 0, s, jobs, items); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2531,7 +2548,8 @@ bool shapes_valid(const Shapes &h) {
            h.stem_waves <= 8 &&
            (h.dom_rows == 64 || h.dom_rows == 128 || h.dom_rows == 256) &&
            (h.fronts_threads == 256 || h.fronts_threads == 512 || h.fronts_threads == 1024) &&
-           (h.assoc_designs == 4 || h.assoc_designs == 8 || h.assoc_designs == 16) &&
+           (h.assoc_designs == 4 || h.assoc_designs == 8 || h.assoc_designs == 16 || h.assoc_designs == 32 ||
+            h.assoc_designs == 64) &&
            (h.niche_threads == 256 || h.niche_threads == 512 || h.niche_threads == 1024);
 }
 size_t niche_lds_bytes(int P) { return (size_t)8 * P; }

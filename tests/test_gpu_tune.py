@@ -76,3 +76,20 @@ def test_invalid_shapes_rejected(cdp):
             j.shapes(set=bad)
     assert j.shapes() == good
     j.close()
+
+
+@pytest.mark.parametrize("assoc", [32, 64])
+def test_association_shapes_keep_results(cdp, assoc):
+    """Eight lanes per design in k_associate (32 or 64 designs a workgroup)
+    choose the same reference lines as a wave per design."""
+    print("association shape", assoc, flush=True)
+    for spec in (SPECS[2], SPECS[3]):
+        want = alone(cdp, spec, 4)
+        j = cdp.Job(*build(cdp, spec)).init()
+        j.shapes(set=dict(design_split=0, design_threads=256, stem_waves=6, dom_rows=64, fronts_threads=512,
+                          assoc_designs=assoc, niche_threads=1024))
+        j.step(4)
+        got = j.read()
+        for k in KEYS:
+            assert np.array_equal(got[k], want[k]), (spec, k)
+        j.close()
