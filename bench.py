@@ -210,7 +210,9 @@ def bench_single(cdp, residues, m, N, H, seed, G, label):
     T = cdp.tables
     aa, tab = T.synth_protein(residues, seed), T.synth_tables(seed)
     t = cdp.Job(aa, tab, m, N, H, 0.1, seed).init()  # a copy to tune on; the shapes are cached
-    shapes, _ = t.tune(2)
+    # eight generations a candidate: a lone generation's phases differ by a few microseconds between
+    # shapes, about the spread of two generations' timings
+    shapes, _ = t.tune(8)
     t.close()
     log(f"{label}: shapes tuned: {shapes}")
     j = cdp.Job(aa, tab, m, N, H, 0.1, seed).init()
