@@ -100,11 +100,16 @@ constexpr int FRONT0_THREADS = 256;
 hipError_t launch_fronts(const DevJob *jobs, int njobs, int max_pool, int threads, hipStream_t s);
 // items: (job, first design), `designs` designs per item: 4, 8 or 16 (a wave each), 32 or 64 (eight lanes each)
 hipError_t launch_associate(const DevJob *jobs, const int2 *items, int n, int designs, hipStream_t s);
-// k_levels (one workgroup per job) when some job has more reference points
-// than NICHE_STAGE (max_refs), then k_place and k_niche (grid: jobs x chunks
-// of `threads` (256, 512 or 1024) designs of the largest pool, max_pool;
-// k_niche takes niche_lds_bytes(max_pool) of dynamic LDS, set_niche_lds_limit
-// above 48 KiB)
+// Pools of at most PLACE_NICHE_POOL designs with at most NICHE_STAGE
+// reference points: k_place_niche (one workgroup of `threads` per job,
+// place_niche_lds_bytes(max_pool) of dynamic LDS).  Otherwise k_levels (one
+// workgroup per job) when some job has more reference points than
+// NICHE_STAGE (max_refs), then k_place and k_niche (grid: jobs x chunks of
+// `threads` (256, 512 or 1024) designs of the largest pool, max_pool; k_niche
+// takes niche_lds_bytes(max_pool) of dynamic LDS, set_niche_lds_limit above
+// 48 KiB)
+constexpr int PLACE_NICHE_POOL = 2048;
+inline size_t place_niche_lds_bytes(int P) { return (size_t)12 * P; }
 size_t niche_lds_bytes(int P);
 hipError_t set_niche_lds_limit(size_t bytes);
 hipError_t launch_niche(const DevJob *jobs, int njobs, int max_pool, int max_refs, int threads, hipStream_t s);

@@ -2421,6 +2421,80 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
     }
 }
 
+// k_place and k_niche in one workgroup per job, for pools of at most
+// PLACE_NICHE_POOL designs whose niche arrays fit in LDS (R <= NICHE_STAGE):
+// the levels settled in LDS, the designs of the last front placed (designs
+// d = tid, tid + NT, ...), the cut buckets' keys (priority, design) and
+// buckets kept in LDS (dynamic, 12 bytes a design of the pool) instead of the
+// global cut list, and the cut designs ranked within their buckets after one
+// barrier instead of a kernel boundary.  The same selection as k_place then
+// k_niche: a design is selected when it is its niche's level-0 pick, when
+// the rest of its bucket is taken whole, or when fewer than kk of the rest of
+// its cut bucket have a smaller key.  Ends the generation, as k_niche does.
+__global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t dyn[];
+    const DevJob &J = jobs[blockIdx.x];
+    SelState *st = J.st;
+    const int P = J.P, R = J.R, NT = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+    const int n_cut = gload(&st->n_cut), last = gload(&st->last);
+    if (tid == 0) st->gen = st->gen + 1u;  // (nothing here reads the generation count)
+    if (n_cut) return;
+    const NicheView g = niche_view(J.niche, R);
+    uint64_t *keys = (uint64_t *)dyn;          // [P] keys of the cut buckets' designs, bucket by bucket
+    int32_t *kbuck = (int32_t *)(keys + P);    // [P] the bucket of each key
+    __shared__ int2 s_cr[NICHE_STAGE];
+    __shared__ unsigned long long s_near[NICHE_STAGE];
+    __shared__ int s_kp[NICHE_STAGE], s_l0[NICHE_STAGE], s_co[NICHE_STAGE], l_cnt[NICHE_STAGE];
+    for (int r = tid; r < R; r += NT) {
+        s_cr[r] = g.cr[r];
+        s_near[r] = gload(g.nearest + r);
+        l_cnt[r] = 0;
+    }
+    __syncthreads();
+    const int ncut = niche_levels(J, s_cr, s_near, s_kp, s_l0, s_co);
+    for (int r = tid; r < R; r += NT) {
+        gstore(g.kp + r, s_kp[r]);
+        gstore(g.lvl0 + r, s_l0[r]);
+        gstore(g.coff + r, s_co[r]);
+    }
+    if (tid == 0) J.acc->ncut = ncut;
+    // place the designs of the last front
+    for (int d0 = 0; d0 < P; d0 += NT) {
+        const int d = d0 + tid, dc = min(d, P - 1);
+        const int fo = gload(J.front_of + dc), rf = gload(J.ref_of + dc);
+        const uint32_t pr = gload(J.prio + dc);
+        const bool mine = d < P && fo == last;
+        bool sel = false;
+        if (mine) {
+            const int l0 = s_l0[rf], co = s_co[rf];
+            sel = d == l0;
+            if (co >= 0) {
+                const int slot = co + atomicAdd(&l_cnt[rf], 1);
+                keys[slot] = ((uint64_t)pr << 32) | (uint32_t)d;
+                kbuck[slot] = rf;
+            } else if (!sel) {  // not cut: the rest of the bucket is taken whole, or nothing of it
+                sel = s_kp[rf] - (l0 >= 0 ? 1 : 0) > 0;
+            }
+        }
+        const uint64_t b = __ballot(sel);  // a wave's 64 designs are one word of the mask
+        if (lane == 0 && b) atomicOr((unsigned long long *)J.selm + ((d0 + (tid & ~63)) >> 6), b);
+    }
+    __syncthreads();  // the cut buckets' keys are complete
+    // rank the designs of the cut buckets
+    for (int sl = tid; sl < ncut; sl += NT) {
+        const uint64_t my = keys[sl];
+        const int r = kbuck[sl], d = (int)(uint32_t)my, l0 = s_l0[r];
+        if (d == l0) continue;
+        const int kk = s_kp[r] - (l0 >= 0 ? 1 : 0), off = s_co[r], nr = s_cr[r].y;
+        int rank = 0;
+#pragma unroll 4
+        for (int u = 0; u < nr; ++u) rank += (int)(keys[off + u] < my);
+        // the level-0 pick of the bucket is not part of the rest
+        if (l0 >= 0 && ((((uint64_t)gload(J.prio + l0)) << 32) | (uint32_t)l0) < my) --rank;
+        if (rank < kk) atomicOr((unsigned long long *)J.selm + (d >> 6), 1ull << (d & 63));
+    }
+}
+
 // Initial state after k_design<INIT>: the first N designs survive, extreme points
 // from the first designs (main.cu:101, 106-113).
 __global__ void k_init_state(const DevJob *__restrict__ jobs) {
@@ -2582,6 +2656,10 @@ hipError_t launch_niche(const DevJob *jobs, int njobs, int max_pool, int max_ref
         hipLaunchKernelGGL(k_levels, dim3(njobs), dim3(256), 0, s, jobs);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (max_pool <= PLACE_NICHE_POOL && max_refs <= NICHE_STAGE) {  // one workgroup per job
+        hipLaunchKernelGGL(k_place_niche, dim3(njobs), dim3(threads), place_niche_lds_bytes(max_pool), s, jobs);
+        return hipGetLastError();
     }
     const dim3 grid(njobs, (max_pool + threads - 1) / threads);
     hipLaunchKernelGGL(k_place, grid, dim3(threads), 0, s, jobs);
