@@ -110,7 +110,7 @@ def bench_headline(cdp, args):
     b.step(2)
     b.sync()
     t_untuned = b.step_timed(5) / 5
-    shapes, tune_gens = b.tune(2)
+    shapes, tune_gens = b.tune(3)  # three ticks a candidate: the candidates differ by about the ticks' spread
     log(f"batch shapes tuned over {tune_gens} generations: {shapes} (untuned tick {t_untuned:.3f} ms)")
     b.step(args.warmup)
     b.sync()
