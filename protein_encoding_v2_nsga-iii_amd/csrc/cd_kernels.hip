@@ -1095,7 +1095,7 @@ __device__ int survivor_at(const uint64_t *selm, int W, int c) {
 // LDS in 16-byte words.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_design(const DevJob *__restrict__ jobs, const int2 *__restrict__ items,
                                                 uint32_t *__restrict__ stem_list, int32_t *stem_ctl, int stem_group,
                                                 int stem_cap, int item_base) {
     extern __shared__ __align__(16) uint8_t dyn[];
