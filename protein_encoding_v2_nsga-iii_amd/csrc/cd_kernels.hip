@@ -2565,11 +2565,16 @@ hipError_t launch_stem_list(const DevJob *jobs, const int2 *items, const uint32_
     const int capacity = sl.groups * sl.cap;
     if (lds < 4 * ((size_t)sl.groups + 1)) lds = 4 * ((size_t)sl.groups + 1);  // the prefix sums share the planes' LDS
     if (!stem_list || !stem_ctl || waves < 5 || waves > 8) return hipErrorInvalidValue;
-    // workgroups take the entries round-robin, so every one must be resident
-    // from the start: per CU, the waves per SIMD or what the LDS (160 KiB,
-    // with the kernel's static 256 bytes) holds
-    const int by_lds = (int)((160u * 1024u) / (lds + 256u));
-    const int per_cu = by_lds < waves ? (by_lds > 1 ? by_lds : 1) : waves;
+    // workgroups take the entries round-robin, so every one should be
+    // resident from the start: per CU, the waves per SIMD or what the
+    // registers and the LDS (dynamic and static) hold, as the runtime counts it
+    const void *fn = waves == 5   ? (const void *)k_stem_list<5>
+                     : waves == 6 ? (const void *)k_stem_list<6>
+                     : waves == 7 ? (const void *)k_stem_list<7>
+                                  : (const void *)k_stem_list<8>;
+    int fit = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, 256, lds) != hipSuccess || fit < 1) fit = 1;
+    const int per_cu = fit < waves ? fit : waves;
     const int grid = capacity < STEM_CUS * per_cu ? capacity : STEM_CUS * per_cu;
     switch (waves) {
     case 5: hipLaunchKernelGGL(k_stem_list<5>, dim3(grid), dim3(256), lds, s, jobs, items, stem_list, stem_ctl, sl.groups, sl.cap); break;
