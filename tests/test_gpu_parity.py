@@ -55,7 +55,7 @@ def test_nds_bit_exact(cdp, orc, P, N):
 # but H = 9, whose 2002 points take k_levels on the arrays in global memory);
 # at P = 8192 the last front and the cut list span many workgroups
 @pytest.mark.parametrize("N,H,seed", [(8, 1, 1), (50, 2, 2), (128, 4, 3), (1024, 5, 4), (300, 3, 5), (4096, 5, 6),
-                                      (4096, 9, 7)])
+                                      (4096, 9, 7), (512, 7, 8)])
 def test_select_bit_exact(cdp, orc, N, H, seed):
     _say("select", N, H, seed)
     rng = np.random.default_rng(seed)
