@@ -267,12 +267,7 @@ __device__ inline uint64_t block_max_u64(uint64_t v, uint64_t *red) {
 // prefix and sets *total.  red holds blockDim/64 ints.
 __device__ inline int block_scan_excl(int v, int *red, int *total) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    int incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
+    const int incl = wave_scan_i32(v);  // (every thread calls this)
     __syncthreads();
     if (lane == 63) red[wid] = incl;
     __syncthreads();
