@@ -178,7 +178,7 @@ typedef struct cd_shapes {
     int32_t dom_rows;       /* k_dominance tile rows per workgroup (64 x 64 tiles): 64, 128 or 256 */
     int32_t fronts_threads; /* k_fronts workgroup size: 256, 512 or 1024 */
     int32_t assoc_designs;  /* k_associate designs per workgroup: 4, 8 or 16 (a wave each), 32 or 64 (8 lanes each) */
-    int32_t niche_threads;  /* k_niche workgroup size: 256, 512 or 1024 */
+    int32_t niche_threads;  /* niching workgroup size (k_place_niche, or k_place and k_niche above 2048 designs): 256, 512 or 1024 */
 } cd_shapes;
 
 /* Autotuner: a coordinate search over the shapes, each candidate timed

@@ -26,7 +26,7 @@ struct Shapes {
     int dom_rows;        // k_dominance tile rows per workgroup (dom_rows / 64 tiles of 64 x 64): 64, 128 or 256
     int fronts_threads;  // k_fronts workgroup: 256, 512 or 1024 threads
     int assoc_designs;   // designs per k_associate workgroup: 4, 8 or 16 (a wave each), 32 or 64 (eight lanes each)
-    int niche_threads;   // k_niche workgroup: 256, 512 or 1024 threads
+    int niche_threads;   // niching workgroup (k_place_niche, or k_place and k_niche above 2048 designs): 256, 512 or 1024 threads
 };
 // Untuned shapes: a lone job, and a batch or served tick.
 constexpr Shapes SHAPES_JOB = {0, 512, 6, 64, 512, 4, 1024};
