@@ -1512,7 +1512,15 @@ __global__ __launch_bounds__(256) void k_stem_finish(const DevJob *__restrict__ 
 // ---------------------------------------------------------------------------
 // Selection
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int pool_buffer(const DevJob &J) { return (int)((J.st->gen + 1u) & 1u); }
+// The objectives of the pool, buffer (gen + 1) & 1.  Both buffers' pointers
+// are read from the descriptor before the choice and held in scalar
+// registers, so that the rows' loads wait for the generation counter only,
+// not for a further descriptor load at an offset the counter decides.
+__device__ __forceinline__ const float *pool_obj(const DevJob &J, uint32_t gen) {
+    const float *o0 = J.obj[0], *o1 = J.obj[1];
+    asm volatile("" : "+s"(o0), "+s"(o1));
+    return (gen & 1u) ? o0 : o1;
+}
 
 // The pool's extrema over rows row0 .. row0 + rows - 1 (updateIdealValue /
 // updateWorstValue, sorting.cuh:132-426; fminf / fmaxf from FLT_MAX /
@@ -1585,11 +1593,20 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
     const DevJob &J = jobs[it.x];
     const int P = J.P, N = J.N, r0 = it.y, wc = it.z, c0 = 64 * wc;
     const bool diag = r0 == c0;
-    const uint32_t gen = __builtin_amdgcn_readfirstlane(gload(&J.st->gen));
+    // the generation counter's load issued, then the descriptor fields read
+    // (held in scalar registers: no load waits for the counter's value)
+    const uint32_t g0 = gload(&J.st->gen);
+    const float *o0 = J.obj[0], *o1 = J.obj[1];
+    const int32_t *smax = J.stem_max;
+    int den = J.den_sl;
+    asm volatile("" : "+s"(o0), "+s"(o1), "+s"(smax), "+s"(den));
+    const uint32_t gen = __builtin_amdgcn_readfirstlane(g0);
     const int buf = (int)((gen + 1u) & 1u);
-    const float *const obj = J.obj[buf];
-    const int32_t *const smax = J.stem_max;
-    const float den_sl = (float)J.den_sl;
+    const float *const obj = buf ? o1 : o0;
+    // (a sort alone has no stem maxima: its loads read the objectives
+    // instead, so that they need no branch and issue with the rows')
+    if (!finish) smax = (const int32_t *)obj;
+    const float den_sl = (float)den;
     // In the offspring generation a child's objective 5 is its longest stem
     // over (3L-4)/2, the value stem_finish stores.  The column design's row
     // and the tile row of the lane are loaded at clamped indices, all before
@@ -1601,8 +1618,7 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         cand[k] = gload(obj + (size_t)ic * NOBJ + k);
         wrow[k] = gload(obj + (size_t)rc * NOBJ + k);
     }
-    const int32_t csm = finish ? gload(smax + ic) : 0;  // (a sort alone has no stem maxima)
-    const int32_t rsm = finish ? gload(smax + rc) : 0;
+    const int32_t csm = gload(smax + ic), rsm = gload(smax + rc);
     if (finish && i >= N) cand[NOBJ - 1] = (float)csm / den_sl;
     if (finish && rr >= N) wrow[NOBJ - 1] = (float)rsm / den_sl;
     const bool cv = i < P;
@@ -1900,21 +1916,32 @@ __device__ float gauss_wave(const float *E) {
 // into the job's accumulators with one atomic per quantity.
 __global__ __launch_bounds__(FRONT0_THREADS) void k_front0(const DevJob *__restrict__ jobs) {
     const DevJob &J = jobs[blockIdx.x];
-    const int P = J.P, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // the descriptor fields held in scalar registers, then the generation
+    // counter, the ideal point and the dominator count loaded together (the
+    // rows' loads wait for the counter alone)
+    SelState *st = J.st;
+    SelAcc *acc = J.acc;
+    const int32_t *np = J.np;
+    const float *o0 = J.obj[0], *o1 = J.obj[1];
+    int P = J.P;
+    asm volatile("" : "+s"(st), "+s"(acc), "+s"(np), "+s"(o0), "+s"(o1), "+s"(P));
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     constexpr int NW = FRONT0_THREADS / 64;
     const int d0 = blockIdx.y * FRONT0_THREADS;
     if (d0 >= P) return;
-    SelAcc *acc = J.acc;
-    const float *obj = J.obj[pool_buffer(J)];
-    __shared__ float ideal[NOBJ], red_v[NW * 2 * NOBJ];
-    __shared__ int red_i[NW * NOBJ], s_cnt;
-    if (tid < NOBJ) ideal[tid] = ord_float(acc->lo[tid]);
-    if (tid == 0) s_cnt = 0;
+    const uint32_t g0 = gload(&st->gen);
     const int d = d0 + tid, dc = min(d, P - 1);
-    const int npv = gload(J.np + dc);
+    const int npv = gload(np + dc);
+    const uint32_t lo = gload(acc->lo + min(tid, NOBJ - 1));
+    const uint32_t gen = __builtin_amdgcn_readfirstlane(g0);
+    const float *const obj = (gen & 1u) ? o0 : o1;  // buffer (gen + 1) & 1
     float x[NOBJ];
 #pragma unroll
     for (int k = 0; k < NOBJ; ++k) x[k] = gload(obj + (size_t)dc * NOBJ + k);
+    __shared__ float ideal[NOBJ], red_v[NW * 2 * NOBJ];
+    __shared__ int red_i[NW * NOBJ], s_cnt;
+    if (tid < NOBJ) ideal[tid] = ord_float(lo);
+    if (tid == 0) s_cnt = 0;
     __syncthreads();
     const bool f0 = d < P && npv == 0;
     if (d < P) gstore(J.front_of + d, f0 ? 0 : -1);
@@ -1969,29 +1996,35 @@ __global__ __launch_bounds__(1024) void k_fronts(const DevJob *__restrict__ jobs
     const DevJob &J = jobs[blockIdx.x];
     const int P = J.P, W = J.W, N = J.N, R = J.R;
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wid = tid >> 6;
-    const float *obj = J.obj[pool_buffer(J)];
-    const SelAcc *acc = J.acc;
+    // the descriptor fields first, held in scalar registers, then every
+    // load of the set-up at once
     SelState *st = J.st;
+    const SelAcc *acc = J.acc;
+    const uint64_t *rk = J.ranked;
+    const float *o0 = J.obj[0], *o1 = J.obj[1];
+    asm volatile("" : "+s"(st), "+s"(acc), "+s"(rk), "+s"(o0), "+s"(o1));
+    const uint32_t g0 = gload(&st->gen);
     __shared__ uint64_t ranked[MAX_W], newm[MAX_W], lastm[MAX_W];
     __shared__ int s_cnt, s_new;
     __shared__ float ext[NOBJ * NOBJ], ideal[NOBJ], worst[NOBJ], f0max[NOBJ];
     __shared__ float pv[NOBJ], crow[2 * NOBJ][NOBJ], casf[2 * NOBJ][NOBJ];
     __shared__ int pid[NOBJ];
     if (tid < NOBJ) {
-        ideal[tid] = ord_float(acc->lo[tid]);
-        worst[tid] = ord_float(acc->hi[tid]);
-        f0max[tid] = ord_float(acc->f0hi[tid]);
-        const unsigned long long k = acc->asf[tid];
+        ideal[tid] = ord_float(gload(acc->lo + tid));
+        worst[tid] = ord_float(gload(acc->hi + tid));
+        f0max[tid] = ord_float(gload(acc->f0hi + tid));
+        const unsigned long long k = gload(acc->asf + tid);
         pv[tid] = k == ~0ull ? FLT_MAX : ord_float((uint32_t)(k >> 32));
         pid[tid] = k == ~0ull ? 0x7FFFFFFF : (int)(uint32_t)k;
     }
-    if (tid < NOBJ * NOBJ) ext[tid] = st->ext[tid];
+    if (tid < NOBJ * NOBJ) ext[tid] = gload(st->ext + tid);
     if (tid == 0) {
-        s_cnt = acc->f0;
+        s_cnt = gload(&acc->f0);
         s_new = 0;
     }
+    const float *const obj = (g0 & 1u) ? o0 : o1;  // buffer (gen + 1) & 1
     for (int w = tid; w < W; w += NT) {
-        const uint64_t f0w = gload(J.ranked + w);
+        const uint64_t f0w = gload(rk + w);
         ranked[w] = f0w;
         lastm[w] = f0w;
         newm[w] = 0;
@@ -2165,7 +2198,7 @@ __global__ __launch_bounds__(D * LPD) void k_associate(const DevJob *__restrict_
     const int d = it.y + (int)threadIdx.x / LPD, dc = min(d, P - 1);
     // the row in 8-byte words (rows are 24 bytes), each feeding two objectives,
     // so that the loads stay ahead of the per-objective branches
-    const f32x2 *o2 = (const f32x2 *)(J.obj[(gen + 1u) & 1u] + (size_t)dc * NOBJ);
+    const f32x2 *o2 = (const f32x2 *)(pool_obj(J, gen) + (size_t)dc * NOBJ);
     const f32x2 x01 = gload(o2), x23 = gload(o2 + 1), x45 = gload(o2 + 2);
     const int fr = gload(J.front_of + dc);
     const float x[NOBJ] = {x01.x, x01.y, x23.x, x23.y, x45.x, x45.y};
@@ -2434,12 +2467,21 @@ __global__ __launch_bounds__(1024) void k_niche(const DevJob *__restrict__ jobs)
 __global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t dyn[];
     const DevJob &J = jobs[blockIdx.x];
+    // the descriptor fields first, held in scalar registers, then the state,
+    // the reference points' counts and the first designs' rows at once
     SelState *st = J.st;
-    const int P = J.P, R = J.R, NT = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+    int32_t *niche = J.niche;
+    const int32_t *front_of = J.front_of, *ref_of = J.ref_of;
+    const uint32_t *prio = J.prio;
+    int P = J.P, R = J.R;
+    asm volatile("" : "+s"(st), "+s"(niche), "+s"(front_of), "+s"(ref_of), "+s"(prio), "+s"(P), "+s"(R));
+    const int NT = blockDim.x, tid = threadIdx.x, lane = tid & 63;
     const int n_cut = gload(&st->n_cut), last = gload(&st->last);
+    int fo = gload(front_of + min(tid, P - 1)), rf = gload(ref_of + min(tid, P - 1));
+    uint32_t pr = gload(prio + min(tid, P - 1));
     if (tid == 0) st->gen = st->gen + 1u;  // (nothing here reads the generation count)
     if (n_cut) return;
-    const NicheView g = niche_view(J.niche, R);
+    const NicheView g = niche_view(niche, R);
     uint64_t *keys = (uint64_t *)dyn;          // [P] keys of the cut buckets' designs, bucket by bucket
     int32_t *kbuck = (int32_t *)(keys + P);    // [P] the bucket of each key
     __shared__ int2 s_cr[NICHE_STAGE];
@@ -2460,9 +2502,13 @@ __global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__
     if (tid == 0) J.acc->ncut = ncut;
     // place the designs of the last front
     for (int d0 = 0; d0 < P; d0 += NT) {
-        const int d = d0 + tid, dc = min(d, P - 1);
-        const int fo = gload(J.front_of + dc), rf = gload(J.ref_of + dc);
-        const uint32_t pr = gload(J.prio + dc);
+        const int d = d0 + tid;
+        if (d0) {  // (the first designs' rows were loaded before the levels)
+            const int dc = min(d, P - 1);
+            fo = gload(front_of + dc);
+            rf = gload(ref_of + dc);
+            pr = gload(prio + dc);
+        }
         const bool mine = d < P && fo == last;
         bool sel = false;
         if (mine) {
@@ -2490,7 +2536,7 @@ __global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__
 #pragma unroll 4
         for (int u = 0; u < nr; ++u) rank += (int)(keys[off + u] < my);
         // the level-0 pick of the bucket is not part of the rest
-        if (l0 >= 0 && ((((uint64_t)gload(J.prio + l0)) << 32) | (uint32_t)l0) < my) --rank;
+        if (l0 >= 0 && ((((uint64_t)gload(prio + l0)) << 32) | (uint32_t)l0) < my) --rank;
         if (rank < kk) atomicOr((unsigned long long *)J.selm + (d >> 6), 1ull << (d & 63));
     }
 }
