@@ -312,13 +312,12 @@ __device__ __noinline__ uint64_t stem_tiles_whole(int n, const lds_uint2 *planes
 // guess (exact when one is found), then of at least 7; the tiles follow a
 // fruitless guess of 7, a search out of budget, or a CDS the search does not
 // take.
-__device__ __forceinline__ uint64_t stem_child(int n, const uint2 *planes, const CdsStat *parent, int32_t *bcast,
+__device__ __forceinline__ uint64_t stem_child(int n, const uint2 *planes, int pl, int pp, int pq, int32_t *bcast,
                                                uint64_t *redu, uint16_t *qhead) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint2 *sp = planes, *rp = planes + scan_words(n);
     if (wid == 0) {
-        const int pl = parent->l;
-        const int seed = stem_seed(sp, n, pl, parent->p, parent->q, lane);
+        const int seed = stem_seed(sp, n, pl, pp, pq, lane);
         if (lane == 0) {
             bcast[0] = seed;
             bcast[2] = pl;
@@ -1052,12 +1051,15 @@ __device__ void build_planes(const uint8_t *rows, int Ls, int L, const int *dl, 
 
 // The index of the c-th survivor (0-based, ascending pool index) in the
 // survivor mask of W words: wave-uniform, every lane of the wave takes part.
-__device__ int survivor_at(const uint64_t *selm, int W, int c) {
+__device__ int survivor_at(const uint64_t *selm, int W, int c, uint64_t first) {
     const int lane = threadIdx.x & 63;
     int base = 0;
+    uint64_t x = lane < W ? first : 0ull;  // (word `lane` of the mask, loaded by the caller)
     for (int w0 = 0; w0 < W; w0 += 64) {
-        const int w = w0 + lane;
-        const uint64_t x = w < W ? gload(selm + w) : 0ull;
+        if (w0) {
+            const int w = w0 + lane;
+            x = w < W ? gload(selm + w) : 0ull;
+        }
         const int incl = wave_scan_i32((int)__popcll(x));  // inclusive prefix of the words' counts
         const uint64_t hit = __ballot(base + incl > c);
         if (hit) {
@@ -1130,20 +1132,43 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     SelAcc *const acc = J.acc;
     // stage the tables and the synonym ranges (rows pbase and pnum, as the
     // job keeps them: padding positions have one synonym at codon 0)
+    // Every load of the set-up is issued before the first LDS store (a store
+    // under a branch would wait for its own load first): the tables, the
+    // first words of the synonym ranges, the generation counter and the first
+    // words of the survivor mask.
+    const int npw = (2 * Ls) >> 4;  // 16-byte words of the rows pbase and pnum
+    const uint8_t *p_pos = pos, *p_code = ctab;
+    const int64_t *p_lw = lw6;
+    const uint64_t *p_sel = J.selm;
+    const SelState *p_st = J.st;
+    int W = J.W;
+    asm volatile("" : "+s"(p_pos), "+s"(p_code), "+s"(p_lw), "+s"(p_sel), "+s"(p_st), "+s"(W));
+    int64_t v_lw = gload(p_lw + (tid & 63));
+    uint32_t v_code = gload(p_code + (tid & 63));
+    u32x4 v_pos = gload((const u32x4 *)p_pos + min(tid, npw - 1));
+    uint32_t g0 = 0;
+    uint64_t sel0 = 0;
+    if (MODE == 2) {
+        g0 = gload(&p_st->gen);
+        sel0 = gload(p_sel + min(tid & 63, W - 1));
+    }
+    // (taken here, so that no load is moved into the branches below)
+    asm volatile("" : "+v"(v_lw), "+v"(v_code), "+v"(v_pos), "+v"(sel0));
     if (tid < 64)
-        ((int64_t *)(dyn + DL_LW6))[tid] = gload(lw6 + tid);
+        ((int64_t *)(dyn + DL_LW6))[tid] = v_lw;
     else if (tid < 128)
-        dyn[DL_CODE + tid - 64] = gload(ctab + tid - 64);
-    for (int i = tid; i < (2 * Ls) >> 4; i += NT) ((u32x4 *)pbase)[i] = gload((const u32x4 *)pos + i);
+        dyn[DL_CODE + tid - 64] = (uint8_t)v_code;
+    if (tid < npw) ((u32x4 *)pbase)[tid] = v_pos;
+    for (int i = tid + NT; i < npw; i += NT) ((u32x4 *)pbase)[i] = gload((const u32x4 *)pos + i);
     if (tid < MAX_M) sh.dirty[tid] = (MODE != 2);
     uint32_t gen = 0;
     int dst = 0, out_slot = it.y, src = 0, parent = 0;
     if (MODE == 2) {
-        gen = __builtin_amdgcn_readfirstlane(gload(&J.st->gen));
+        gen = __builtin_amdgcn_readfirstlane(g0);
         src = (int)(gen & 1u);
         dst = src ^ 1;
         out_slot = N + it.y;
-        parent = survivor_at(J.selm, J.W, it.y);
+        parent = survivor_at(p_sel, W, it.y, sel0);
     }
     u32x4 *syn4 = (u32x4 *)syn;
     if (MODE == 0) {
@@ -1427,20 +1452,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             const int2 it = items[ent >> 4];
             const int c = (int)(ent & 15u);
             const DevJob &J = jobs[it.x];
-            const int L = J.L, m = J.m, n = 3 * L;
-            int buf, slot;
-            item_slot(J, MODE_OFFSPRING, it.y, buf, slot);
-            CdsStat *cs = J.cst[buf] + (size_t)slot * m + c;
-            uint64_t *planes = (uint64_t *)dyn;  // the CDS's planes as k_design left them
+            // The descriptor fields first, held in scalar registers; then the
+            // generation counter, the planes and the parent's stem from both
+            // record buffers, loaded together (only the choice of record
+            // waits for the counter).  The child sits at slot N + child of
+            // the next buffer, (gen + 1) & 1.
+            const SelState *p_st = J.st;
+            CdsStat *cs0 = J.cst[0], *cs1 = J.cst[1];
+            const uint2 *p_pl = J.planes;
+            int32_t *p_sm = J.stem_max;
+            int L = J.L, m = J.m, N = J.N;
+            asm volatile("" : "+s"(p_st), "+s"(cs0), "+s"(cs1), "+s"(p_pl), "+s"(p_sm), "+s"(L), "+s"(m), "+s"(N));
+            const int n = 3 * L, slot = N + it.y, tid = (int)threadIdx.x;
+            const size_t rec = (size_t)slot * m + c;
             const int nwords = 2 * scan_words(n);
-            const uint64_t *gp = (const uint64_t *)(J.planes + ((size_t)slot * m + c) * nwords);
+            const uint64_t *gp = (const uint64_t *)(p_pl + rec * nwords);
+            const uint32_t g0 = gload(&p_st->gen);
+            const int32_t *r0 = (const int32_t *)(cs0 + rec), *r1 = (const int32_t *)(cs1 + rec);
+            const int l0 = gload(r0 + 6), p0 = gload(r0 + 7), q0 = gload(r0 + 8);  // CdsStat::l, p, q
+            const int l1 = gload(r1 + 6), p1 = gload(r1 + 7), q1 = gload(r1 + 8);
+            const uint64_t v0 = gload(gp + min(tid, nwords - 1));
+            uint64_t *planes = (uint64_t *)dyn;  // the CDS's planes as k_design left them
             __syncthreads();  // the previous entry's LDS is no longer read
-            for (int i = threadIdx.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
+            if (tid < nwords) planes[tid] = v0;
+            for (int i = tid + (int)blockDim.x; i < nwords; i += blockDim.x) planes[i] = gload(gp + i);
+            const bool b1 = ((__builtin_amdgcn_readfirstlane(g0) + 1u) & 1u) != 0;
+            CdsStat *cs = (b1 ? cs1 : cs0) + rec;
             __syncthreads();
-            const uint64_t key = stem_child(n, (const uint2 *)planes, cs, bcast, redu, (uint16_t *)(planes + nwords));
-            if (threadIdx.x == 0 && key) {
+            const uint64_t key = stem_child(n, (const uint2 *)planes, b1 ? l1 : l0, b1 ? p1 : p0, b1 ? q1 : q0, bcast,
+                                            redu, (uint16_t *)(planes + nwords));
+            if (tid == 0 && key) {
                 atomicMax((unsigned long long *)&cs->key, (unsigned long long)key);
-                atomicMax(&J.stem_max[slot], (int)(key >> 40));
+                atomicMax(&p_sm[slot], (int)(key >> 40));
             }
         }
     }
