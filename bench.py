@@ -59,7 +59,7 @@ DEFAULT_BATCH_SHAPES = dict(design_split=24 * 1024, design_threads=512, stem_wav
 # summarised in profiles/pmc_batched_mix_64jobs.md: HBM bytes (FETCH_SIZE x
 # its copy-kernel factor + WRITE_SIZE x its factor), VALU wave-instructions,
 # LDS instructions and LDS bank-conflict cycles.
-TICK_COUNTERS = {"hbm_bytes": 735901433, "valu_wave_instr": 243455009, "lds_instr": 24728583,
+TICK_COUNTERS = {"hbm_bytes": 736134804, "valu_wave_instr": 244725985, "lds_instr": 24882951,
                  "lds_conflict_cycles": 6257523}
 
 
