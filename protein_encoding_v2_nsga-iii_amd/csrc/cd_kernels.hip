@@ -1492,11 +1492,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 // Objective 5 of a design: the longest stem over its CDS, ties to the last
 // CDS (calMaximumSL's obj5 = l / ((3L-4)/2), utils.cuh:880).  Stores the
 // stems k_stem found and clears the marks k_design left.
-__device__ int stem_finish(const DevJob &J, int buf, int slot) {
-    const int m = J.m;
+__device__ int stem_finish_at(CdsStat *cst, float *obj, int8_t *oidx, int32_t *pql, int m, int den_sl, int slot) {
     // a record's 16-byte words 1 (hsc, gc3, l, p) and 2 (q, pad, key), four
     // records' loads issued together
-    u32x4 *cs = (u32x4 *)(J.cst[buf] + (size_t)slot * m);
+    u32x4 *cs = (u32x4 *)(cst + (size_t)slot * m);
     int bl = 0, bp = 0, bq = 0, bc = m - 1, changed = 0;
     for (int c0 = 0; c0 < m; c0 += 4) {
         u32x4 w1[4], w2[4];
@@ -1529,14 +1528,17 @@ __device__ int stem_finish(const DevJob &J, int buf, int slot) {
             }
         }
     }
-    gstore(J.obj[buf] + (size_t)slot * NOBJ + 5, (float)bl / (float)J.den_sl);
-    gstore(J.oidx[buf] + (size_t)slot * 2 * NOBJ + 10, (int8_t)bc);
-    int32_t *pq = J.pql[buf] + (size_t)slot * 4;
+    gstore(obj + (size_t)slot * NOBJ + 5, (float)bl / (float)den_sl);
+    gstore(oidx + (size_t)slot * 2 * NOBJ + 10, (int8_t)bc);
+    int32_t *pq = pql + (size_t)slot * 4;
     gstore(pq, bp);
     gstore(pq + 1, bq);
     gstore(pq + 2, bl);
     gstore(pq + 3, 0);
     return changed;
+}
+__device__ int stem_finish(const DevJob &J, int buf, int slot) {
+    return stem_finish_at(J.cst[buf], J.obj[buf], J.oidx[buf], J.pql[buf], J.m, J.den_sl, slot);
 }
 
 // stem_finish for every work item's design (initialisation and evaluation;
@@ -1639,13 +1641,19 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
     // the generation counter's load issued, then the descriptor fields read
     // (held in scalar registers: no load waits for the counter's value)
     const uint32_t g0 = gload(&J.st->gen);
-    const float *o0 = J.obj[0], *o1 = J.obj[1];
+    float *o0 = J.obj[0], *o1 = J.obj[1];
     const int32_t *smax = J.stem_max;
     int den = J.den_sl;
     asm volatile("" : "+s"(o0), "+s"(o1), "+s"(smax), "+s"(den));
+    // (and those of the stem records' completion below)
+    CdsStat *cs0 = J.cst[0], *cs1 = J.cst[1];
+    int8_t *oi0 = J.oidx[0], *oi1 = J.oidx[1];
+    int32_t *pq0 = J.pql[0], *pq1 = J.pql[1];
+    int m = J.m;
+    asm volatile("" : "+s"(cs0), "+s"(cs1), "+s"(oi0), "+s"(oi1), "+s"(pq0), "+s"(pq1), "+s"(m));
     const uint32_t gen = __builtin_amdgcn_readfirstlane(g0);
     const int buf = (int)((gen + 1u) & 1u);
-    const float *const obj = buf ? o1 : o0;
+    float *const obj = buf ? o1 : o0;
     // (a sort alone has no stem maxima: its loads read the objectives
     // instead, so that they need no branch and issue with the rows')
     if (!finish) smax = (const int32_t *)obj;
@@ -1662,6 +1670,17 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         wrow[k] = gload(obj + (size_t)rc * NOBJ + k);
     }
     const int32_t csm = gload(smax + ic), rsm = gload(smax + rc);
+    // the diagonal tiles complete the stem records of their children and
+    // count the CDS the children recomputed (before the rows are used, so
+    // that the records' loads issue while the rows' are in flight; the
+    // objective 5 it stores is the value the rows take below)
+    if (finish && diag && sub == 0) {
+        int changed = 0;
+        if (rr >= N && rr < P && have)
+            changed = stem_finish_at(buf ? cs1 : cs0, obj, buf ? oi1 : oi0, buf ? pq1 : pq0, m, den, rr);
+        changed = wave_sum_i32(changed);
+        if (lane == 0 && changed) atomicAdd(&J.st->cds_scanned, (unsigned long long)changed);
+    }
     if (finish && i >= N) cand[NOBJ - 1] = (float)csm / den_sl;
     if (finish && rr >= N) wrow[NOBJ - 1] = (float)rsm / den_sl;
     const bool cv = i < P;
@@ -1675,14 +1694,6 @@ __global__ __launch_bounds__(256) void k_dominance(const DevJob *__restrict__ jo
         float *const o = rows[tl] + lane * NOBJ;
 #pragma unroll
         for (int k = 0; k < NOBJ; ++k) o[k] = rr < P ? wrow[k] : __builtin_inff();
-    }
-    // the diagonal tiles complete the stem records of their children and
-    // count the CDS the children recomputed
-    if (finish && diag && sub == 0) {
-        int changed = 0;
-        if (rr >= N && rr < P && have) changed = stem_finish(J, buf, rr);
-        changed = wave_sum_i32(changed);
-        if (lane == 0 && changed) atomicAdd(&J.st->cds_scanned, (unsigned long long)changed);
     }
     __syncthreads();
     // and fold their rows into the pool's extrema
