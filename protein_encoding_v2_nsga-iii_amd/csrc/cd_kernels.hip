@@ -2533,15 +2533,25 @@ __global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__
     const int n_cut = gload(&st->n_cut), last = gload(&st->last);
     int fo = gload(front_of + min(tid, P - 1)), rf = gload(ref_of + min(tid, P - 1));
     uint32_t pr = gload(prio + min(tid, P - 1));
+    // and the reference points' counts and nearest designs (first chunk)
+    const NicheView g = niche_view(niche, R);
+    const int rc0 = min(tid, R - 1);
+    const uint64_t cr0 = gload((const uint64_t *)g.cr + rc0);  // int2 (x low, y high)
+    const unsigned long long nn0 = gload(g.nearest + rc0);
     if (tid == 0) st->gen = st->gen + 1u;  // (nothing here reads the generation count)
     if (n_cut) return;
-    const NicheView g = niche_view(niche, R);
     uint64_t *keys = (uint64_t *)dyn;          // [P] keys of the cut buckets' designs, bucket by bucket
     int32_t *kbuck = (int32_t *)(keys + P);    // [P] the bucket of each key
     __shared__ int2 s_cr[NICHE_STAGE];
     __shared__ unsigned long long s_near[NICHE_STAGE];
     __shared__ int s_kp[NICHE_STAGE], s_l0[NICHE_STAGE], s_co[NICHE_STAGE], l_cnt[NICHE_STAGE];
-    for (int r = tid; r < R; r += NT) {
+    __shared__ uint32_t s_pl0[NICHE_STAGE];  // priority of each bucket's level-0 pick
+    if (tid < R) {
+        s_cr[tid] = make_int2((int)(uint32_t)cr0, (int)(uint32_t)(cr0 >> 32));
+        s_near[tid] = nn0;
+        l_cnt[tid] = 0;
+    }
+    for (int r = tid + NT; r < R; r += NT) {
         s_cr[r] = g.cr[r];
         s_near[r] = gload(g.nearest + r);
         l_cnt[r] = 0;
@@ -2568,6 +2578,7 @@ __global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__
         if (mine) {
             const int l0 = s_l0[rf], co = s_co[rf];
             sel = d == l0;
+            if (sel) s_pl0[rf] = pr;  // (the level-0 pick is a last-front design of its bucket)
             if (co >= 0) {
                 const int slot = co + atomicAdd(&l_cnt[rf], 1);
                 keys[slot] = ((uint64_t)pr << 32) | (uint32_t)d;
@@ -2590,7 +2601,7 @@ __global__ __launch_bounds__(1024) void k_place_niche(const DevJob *__restrict__
 #pragma unroll 4
         for (int u = 0; u < nr; ++u) rank += (int)(keys[off + u] < my);
         // the level-0 pick of the bucket is not part of the rest
-        if (l0 >= 0 && ((((uint64_t)gload(prio + l0)) << 32) | (uint32_t)l0) < my) --rank;
+        if (l0 >= 0 && ((((uint64_t)s_pl0[r]) << 32) | (uint32_t)l0) < my) --rank;
         if (rank < kk) atomicOr((unsigned long long *)J.selm + (d >> 6), 1ull << (d & 63));
     }
 }
